@@ -1,0 +1,158 @@
+/*
+ * onionpir_gpu.h -- C ABI of the MI355X OnionPIR v2 server answer path.
+ *
+ * Plain integers and pointers only; no exceptions cross this boundary.  Every
+ * function returns ONION_OK (0) or a negative ONION_E* code, and
+ * onion_gpu_last_error() returns the message of the calling thread's last
+ * failure.  Host buffers are borrowed for the duration of a call; the server
+ * handle owns all device memory.  Calls on one handle are serialised by an
+ * internal mutex.
+ *
+ * Data formats (N = ring degree, one data limb):
+ *   ciphertext      2*N uint64, c0 then c1, coefficient form, values < q
+ *   Galois keys     h levels x 2 components x 2 limbs (q, p) x N uint64, NTT
+ *                   form; level a belongs to Galois element 1 + (N >> a)
+ *   RGSW key        2*l_key rows x [c0|c1] x N uint64, coefficient form
+ *   response        2*N uint64 after the switch to small_q (values <= small_q),
+ *                   or its bit-packed wire form (ceil(log2 small_q) bits per
+ *                   coefficient, LSB-first)
+ *   plaintexts      N uint64 coefficients < t per database entry slot
+ *
+ * Reference interface each entry point replaces (paths relative to the
+ * reference repository root) is given beside it.
+ */
+#ifndef ONIONPIR_GPU_H
+#define ONIONPIR_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ONION_OK 0
+#define ONION_EINVAL (-1)   /* bad argument or shape (std::invalid_argument)       */
+#define ONION_ENOKEY (-2)   /* unknown client id (std::out_of_range from map::at)   */
+#define ONION_ENOMEM (-3)   /* device memory budget exceeded                        */
+#define ONION_EHIP (-4)     /* HIP runtime failure                                  */
+#define ONION_ESTATE (-5)   /* call out of order (e.g. no database yet)             */
+
+typedef struct onion_params {
+  uint32_t poly_degree; /* N; the device path serves N = 2048                    */
+  uint32_t plain_bits;  /* PlainMod: t = smallest prime above 2^(plain_bits-1)   */
+  uint32_t gsw_l;       /* l of the data RGSW (GSW_L)                             */
+  uint32_t gsw_l_key;   /* l of the key RGSW (GSW_L_KEY)                          */
+  uint32_t fst_dim;     /* first dimension size                                   */
+  uint32_t other_dim;   /* product of the further dimensions (power of two)      */
+} onion_params;
+
+/* Values derived from onion_params by the reference's rules. */
+typedef struct onion_param_info {
+  uint64_t q;        /* data prime (CoeffModulus::Create(N, {60, 60})[0])        */
+  uint64_t p;        /* special key-switching prime ([1])                          */
+  uint64_t t;        /* plaintext modulus                                          */
+  uint64_t small_q;  /* response modulus (CoeffModulus::Create(N, {27, 60})[0])   */
+  uint32_t base_log2, base_log2_key;
+  uint32_t ndim;     /* 1 + log2(other_dim)                                        */
+  uint32_t expan_height; /* h = ceil(log2(fst + l*(ndim-1)))                       */
+  uint32_t useful_cnt;   /* u = fst + l*(ndim-1)                                   */
+  uint64_t num_pt;
+  uint32_t galois_key_words;  /* h * 4 * N                                         */
+  uint32_t rgsw_key_words;    /* 2 * l_key * 2 * N                                 */
+  uint32_t response_bytes;    /* packed response size                              */
+} onion_param_info;
+
+/* Per-stage device time of the last answer, milliseconds, section names of the
+ * reference's TimerLogger (src/includes/logging.h:27-65). */
+typedef struct onion_stage_times {
+  double expand, convert, first_dim, other_dim, mod_switch, total;
+  double first_dim_core; /* the database-streaming kernel alone                   */
+} onion_stage_times;
+
+typedef struct onion_server onion_server;
+
+const char *onion_gpu_last_error(void);
+
+/* Number of visible HIP devices (0 when none); no reference counterpart. */
+int onion_gpu_device_count(int *count);
+
+/* PirParams::PirParams + PirServer::PirServer (src/pir.cpp:29-95,
+ * src/server.cpp:25-39): validate the shape, derive the moduli, allocate the
+ * NTT-form database and the per-query scratch on the current HIP device. */
+int onion_gpu_create(const onion_params *params, onion_server **out);
+int onion_gpu_destroy(onion_server *srv);
+int onion_gpu_param_info(const onion_server *srv, onion_param_info *out);
+
+/* PirServer::gen_data (src/server.cpp:47-78): seeded database generated on the
+ * device, lifted, NTT'd and realigned.  mode 0: coefficients uniform in
+ * [0, t); mode 1: reference-style entries (8-byte big-endian index + random
+ * bytes, 16 bits per coefficient). */
+int onion_gpu_db_generate(onion_server *srv, uint64_t seed, int mode);
+
+/* PirServer::push_database_chunk + preprocess_ntt + realign_db
+ * (src/server.cpp:635-748): plaintexts [pt_lo, pt_lo + n_pt), coefficients
+ * < t, N per plaintext.  The database is usable once every plaintext was
+ * written (by this call or onion_gpu_db_generate). */
+int onion_gpu_db_upload_plain(onion_server *srv, uint64_t pt_lo, uint64_t n_pt,
+                              const uint64_t *coeffs);
+
+/* Read back NTT-form database words db[slot * num_pt + pt] for one plaintext
+ * (debug/parity tap). */
+int onion_gpu_db_read_ntt(onion_server *srv, uint64_t pt, uint64_t *out_n);
+
+/* PirServer::set_client_galois_key (src/server.cpp:480-484). */
+int onion_gpu_set_galois_key(onion_server *srv, uint64_t client_id, const uint64_t *keys,
+                             size_t n_words);
+/* PirServer::set_client_gsw_key (src/server.cpp:486-500): coefficient-form
+ * rows, NTT'd on the device. */
+int onion_gpu_set_rgsw_key(onion_server *srv, uint64_t client_id, const uint64_t *rows,
+                           size_t n_words);
+int onion_gpu_drop_client(onion_server *srv, uint64_t client_id);
+
+/* PirServer::make_query (src/server.cpp:520-579): one query ciphertext in,
+ * the mod-switched response ciphertext out (2*N uint64). */
+int onion_gpu_answer(onion_server *srv, uint64_t client_id, const uint64_t *query,
+                     uint64_t *resp);
+
+/* make_query for n queries of (possibly) different clients in one batched
+ * pipeline; queries and resps hold n ciphertexts back to back. */
+int onion_gpu_answer_batch(onion_server *srv, uint32_t n, const uint64_t *client_ids,
+                           const uint64_t *queries, uint64_t *resps);
+
+/* PirServer::save_resp_to_stream (src/server.cpp:581-633): bit-pack a
+ * response ciphertext on the device; *n_bytes receives the size. */
+int onion_gpu_pack_response(onion_server *srv, const uint64_t *resp, uint8_t *out,
+                            size_t out_cap, size_t *n_bytes);
+
+int onion_gpu_last_stage_times(const onion_server *srv, onion_stage_times *out);
+
+/* ---- stage taps: each runs one stage of the device path on host-given
+ * inputs, for parity tests against the CPU oracle -------------------------- */
+
+/* fast_expand_qry (src/server.cpp:382-428): u leaves, coefficient form. */
+int onion_gpu_tap_expand(onion_server *srv, uint64_t client_id, const uint64_t *query,
+                         uint64_t *leaves);
+/* query_to_gsw for every further dimension (src/server.cpp:533-546):
+ * (ndim-1) RGSWs x 2l rows x 2N, NTT form, from the u leaves. */
+int onion_gpu_tap_convert(onion_server *srv, uint64_t client_id, const uint64_t *leaves,
+                          uint64_t *rgsw);
+/* evaluate_first_dim (src/server.cpp:123-175): fst selection cts in
+ * coefficient form -> other cts in coefficient form. */
+int onion_gpu_tap_first_dim(onion_server *srv, const uint64_t *sel, uint64_t *out);
+/* other_dim_mux (src/server.cpp:295-334): n cts and one RGSW (2l rows x 2N,
+ * NTT form) -> n/2 cts. */
+int onion_gpu_tap_mux(onion_server *srv, const uint64_t *cts, uint32_t n, const uint64_t *rgsw,
+                      uint64_t *out);
+/* SEAL NTT / INTT used on the path: which = 0 (q) or 1 (p). */
+int onion_gpu_tap_ntt(onion_server *srv, int which, int inverse, uint64_t *polys, uint32_t n);
+/* apply_galois_inplace with the client's key of level a (src/server.cpp:409). */
+int onion_gpu_tap_galois(onion_server *srv, uint64_t client_id, uint32_t level,
+                         const uint64_t *ct, uint64_t *out);
+/* mod_switch_inplace (src/server.cpp:779-801) on 2N values. */
+int onion_gpu_tap_mod_switch(onion_server *srv, const uint64_t *ct, uint64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

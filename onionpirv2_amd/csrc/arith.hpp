@@ -1,0 +1,93 @@
+// arith.hpp -- exact 64-bit modular arithmetic on CDNA4 (gfx950).
+//
+// Every value is an unsigned residue below a 60-bit prime.  64x64->128
+// products are built from 32-bit multiplies (v_mad_u64_u32); nothing here
+// touches floating point or the matrix cores.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace onion {
+
+// Modulus with the two-word Barrett ratio floor(2^128 / q).
+struct Mod {
+  uint64_t q;
+  uint64_t r_lo, r_hi;
+};
+
+// NTT tables for one prime, resident in device memory.
+//   rp[k]  = psi^bitrev(k),  rps  = Shoup companions floor(rp * 2^64 / q)
+//   irp[k] = psi^-bitrev(k), irps = Shoup companions
+struct NttTab {
+  const uint64_t *rp, *rps, *irp, *irps;
+  uint64_t q, n_inv, n_inv_s;
+};
+
+__device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) { return __umul64hi(a, b); }
+
+// x * w mod q in [0, 2q) for any 64-bit x (Shoup; q < 2^62)
+__device__ __forceinline__ uint64_t mul_shoup_lazy(uint64_t x, uint64_t w, uint64_t ws,
+                                                   uint64_t q) {
+  return x * w - mulhi64(x, ws) * q;
+}
+
+__device__ __forceinline__ uint64_t add_mod(uint64_t a, uint64_t b, uint64_t q) {
+  uint64_t s = a + b;
+  return s >= q ? s - q : s;
+}
+__device__ __forceinline__ uint64_t sub_mod(uint64_t a, uint64_t b, uint64_t q) {
+  return a >= b ? a - b : a + q - b;
+}
+
+// Barrett reduction of the 128-bit value hi*2^64 + lo, the same algorithm as
+// SEAL's barrett_reduce_128: quotient estimate from the two-word ratio with
+// the lowest partial product dropped, one conditional subtraction.
+__device__ __forceinline__ uint64_t barrett128(uint64_t lo, uint64_t hi, const Mod &m) {
+  const uint64_t p0h = mulhi64(lo, m.r_lo);
+  const uint64_t p1l = lo * m.r_hi, p1h = mulhi64(lo, m.r_hi);
+  const uint64_t p2l = hi * m.r_lo, p2h = mulhi64(hi, m.r_lo);
+  uint64_t mid = p0h + p1l;
+  uint64_t c = mid < p0h;
+  mid += p2l;
+  c += mid < p2l;
+  const uint64_t qhat = hi * m.r_hi + p1h + p2h + c;
+  const uint64_t r = lo - qhat * m.q;
+  return r >= m.q ? r - m.q : r;
+}
+
+// Full product a*b as (lo, hi).
+__device__ __forceinline__ void mul128(uint64_t a, uint64_t b, uint64_t &lo, uint64_t &hi) {
+  lo = a * b;
+  hi = mulhi64(a, b);
+}
+
+__device__ __forceinline__ void add128(uint64_t &lo, uint64_t &hi, uint64_t blo, uint64_t bhi) {
+  lo += blo;
+  hi += bhi + (lo < blo);
+}
+
+__device__ __forceinline__ uint64_t mulmod_barrett(uint64_t a, uint64_t b, const Mod &m) {
+  uint64_t lo, hi;
+  mul128(a, b, lo, hi);
+  return barrett128(lo, hi, m);
+}
+
+// Counter-based generator shared by the database ingest and the seeded client
+// (SplitMix64 finaliser over seed, stream and counter).
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z ^= z >> 30;
+  z *= 0xbf58476d1ce4e5b9ULL;
+  z ^= z >> 27;
+  z *= 0x94d049bb133111ebULL;
+  z ^= z >> 31;
+  return z;
+}
+__host__ __device__ __forceinline__ uint64_t stream_key(uint64_t seed, uint64_t stream) {
+  return mix64(seed * 0x9e3779b97f4a7c15ULL + mix64(stream ^ 0x632be59bd9b4e019ULL));
+}
+__host__ __device__ __forceinline__ uint64_t rand64_keyed(uint64_t key, uint64_t ctr) {
+  return mix64(key + (ctr + 1) * 0x9e3779b97f4a7c15ULL);
+}
+
+}  // namespace onion

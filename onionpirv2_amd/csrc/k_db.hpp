@@ -1,0 +1,79 @@
+// k_db.hpp -- database ingest on the device: seeded generation (or uploaded
+// plaintext coefficients), plain lift, NTT, and the slot-major realignment
+// (PirServer::gen_data / push_database_chunk / preprocess_ntt / realign_db,
+//  src/server.cpp:47-78, 635-748).
+#pragma once
+
+#include "k_ntt.hpp"
+
+namespace onion {
+
+struct LiftConst {
+  uint64_t t;    // plaintext modulus
+  uint64_t thr;  // (t + 1) / 2
+  uint64_t inc;  // q - t
+};
+
+// SEAL transform_to_ntt_inplace(Plaintext): v >= (t+1)/2 becomes v + (q - t)
+__device__ __forceinline__ uint64_t plain_lift(uint64_t v, const LiftConst &L) {
+  return v >= L.thr ? v + L.inc : v;
+}
+
+// NTT of the lifted plaintext in LDS, then db[slot * num_pt + pt] for all slots.
+__device__ __forceinline__ void ntt_and_scatter(uint64_t *s, uint64_t *db, uint64_t pt,
+                                                uint64_t num_pt, const NttTab &T) {
+  __syncthreads();
+  ntt_fwd_lds<kN>(s, T);
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    const int e = threadIdx.x + r * kNttThreads;
+    db[(size_t)e * num_pt + pt] = ntt_fwd_final(s[e], T.q);
+  }
+}
+
+// Seeded generation, one workgroup per plaintext pt_lo + blockIdx.x.
+// mode 0: coefficient e = floor(r * t / 2^64), r = rand64(key, pt*N + e)
+// mode 1: entry bytes (8-byte big-endian entry index, then random bytes
+//         r = rand64(key, pt*N + b/8) byte b%8), 16 bits per coefficient.
+__global__ __launch_bounds__(kNttThreads) void k_db_generate(uint64_t *db, uint64_t pt_lo,
+                                                             uint64_t num_pt, uint64_t key,
+                                                             int mode, LiftConst L, NttTab T) {
+  __shared__ uint64_t s[kN];
+  const uint64_t pt = pt_lo + blockIdx.x;
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    const int e = threadIdx.x + r * kNttThreads;
+    uint64_t v;
+    if (mode == 0) {
+      v = mulhi64(rand64_keyed(key, pt * kN + e), L.t);
+    } else {
+      uint32_t bytes[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const uint32_t b = 2 * e + h;
+        bytes[h] = b < 8 ? (uint32_t)((pt >> (8 * (7 - b))) & 0xFF)
+                         : (uint32_t)((rand64_keyed(key, pt * kN + b / 8) >> (8 * (b % 8))) & 0xFF);
+      }
+      v = bytes[0] | (bytes[1] << 8);
+    }
+    s[e] = plain_lift(v, L);
+  }
+  ntt_and_scatter(s, db, pt, num_pt, T);
+}
+
+// Uploaded plaintexts: coeffs[i*N + e] (< t) for pt = pt_lo + i.
+__global__ __launch_bounds__(kNttThreads) void k_db_plain(const uint64_t *coeffs, uint64_t *db,
+                                                          uint64_t pt_lo, uint64_t num_pt,
+                                                          LiftConst L, NttTab T) {
+  __shared__ uint64_t s[kN];
+  const uint64_t pt = pt_lo + blockIdx.x;
+  const uint64_t *src = coeffs + (size_t)blockIdx.x * kN;
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    const int e = threadIdx.x + r * kNttThreads;
+    s[e] = plain_lift(src[e], L);
+  }
+  ntt_and_scatter(s, db, pt, num_pt, T);
+}
+
+}  // namespace onion

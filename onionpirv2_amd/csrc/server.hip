@@ -1,0 +1,621 @@
+// server.hip -- the C ABI of include/onionpir_gpu.h: device-resident
+// database, per-client key store, and the answer pipeline
+//   expand -> convert -> first dimension -> other dimensions -> mod switch
+// (PirServer::make_query, src/server.cpp:520-579) as a chain of HIP kernels on
+// one stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/onionpir_gpu.h"
+#include "host_math.hpp"
+#include "k_db.hpp"
+#include "k_expand.hpp"
+#include "k_first_dim.hpp"
+#include "k_gsw.hpp"
+#include "k_ntt.hpp"
+#include "k_tail.hpp"
+
+using namespace onion;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                      \
+  do {                                                                                 \
+    hipError_t e_ = (x);                                                               \
+    if (e_ != hipSuccess) return fail(ONION_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct ClientKeys {
+  uint64_t *gk = nullptr;  // h x 4N, NTT form
+  uint64_t *rk = nullptr;  // 2 l_key x 2N, NTT form
+};
+
+uint32_t ceil_log2(uint64_t x) {
+  uint32_t r = 0;
+  while ((uint64_t(1) << r) < x) ++r;
+  return r;
+}
+
+}  // namespace
+
+struct onion_server {
+  onion_params P{};
+  onion_param_info I{};
+  int device = 0;
+  hipStream_t st = nullptr;
+  uint64_t w = 0;  // 2^h
+  // device NTT tables: q then p, each rp rps irp irps (4N words)
+  uint64_t *tabs = nullptr;
+  NttTab Tq{}, Tp{};
+  Mod mq{}, mp{};
+  KsConst ks{};
+  LiftConst lift{};
+  std::vector<uint32_t> ginv;  // inverse of each level's Galois element mod 2N
+  uint64_t ms_M = 0;
+  uint32_t ms_f = 0;
+  // database
+  uint64_t *db = nullptr;
+  std::vector<uint8_t> pt_written;
+  uint64_t n_written = 0;
+  // keys
+  std::map<uint64_t, ClientKeys> clients;
+  // scratch
+  uint64_t *tree = nullptr;    // 2w cts
+  uint64_t *rgsw = nullptr;    // (ndim-1) x 2l x 2N
+  uint64_t *fd = nullptr;      // other cts
+  uint64_t *digits = nullptr;  // digit polys
+  uint32_t *Qs = nullptr;      // N x fst x 4 uint32
+  uint64_t *io = nullptr;      // staging for taps (max of the above shapes)
+  uint8_t *pack = nullptr;
+  size_t io_words = 0, digit_words = 0;
+  hipEvent_t ev[8]{};
+  onion_stage_times last{};
+  std::mutex mu;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(T **p, size_t count) {
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(e == hipErrorOutOfMemory ? ONION_ENOMEM : ONION_EHIP,
+                std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
+                    " bytes failed: " + hipGetErrorString(e));
+  }
+  return ONION_OK;
+}
+
+void dfree(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+int rows_per_block(uint32_t other) { return other >= 512 ? 512 : (other < 32 ? 32 : (int)other); }
+
+// ---- pipeline stages ------------------------------------------------------------
+
+// Expansion of the query already stored in tree slot 1.
+void run_expand(onion_server *s, const ClientKeys &ck) {
+  const uint32_t h = s->I.expan_height, u = s->I.useful_cnt;
+  for (uint32_t a = 0; a < h; ++a) {
+    // active nodes at level a: those whose left-most leaf is < u
+    const uint64_t span = s->w >> a;  // leaves under one node
+    const uint64_t n_active = (u + span - 1) / span;
+    k_expand_level<<<(uint32_t)n_active, kNttThreads, 0, s->st>>>(
+        s->tree, ck.gk + (size_t)a * 4 * kN, a, s->ginv[a], s->Tq, s->Tp, s->mq, s->mp, s->ks);
+  }
+}
+
+// Conversion of the (ndim-1)*l RGSW leaves (starting at `conv_leaves`).
+void run_convert(onion_server *s, const ClientKeys &ck, const uint64_t *conv_leaves) {
+  const uint32_t nd = s->I.ndim - 1, l = s->P.gsw_l, lk = s->P.gsw_l_key;
+  if (nd == 0) return;
+  const uint32_t n_ct = nd * l;
+  k_rgsw_top_rows<<<n_ct * 2, kNttThreads, 0, s->st>>>(conv_leaves, s->rgsw, l, s->Tq);
+  k_decomp_ntt<<<n_ct * 2 * lk, kNttThreads, 0, s->st>>>(conv_leaves, nullptr, s->digits, lk,
+                                                         s->I.base_log2_key, s->Tq);
+  k_ext_mac_store<<<n_ct * 2, kNttThreads, 0, s->st>>>(s->digits, ck.rk, 2 * lk, s->rgsw, l, s->mq);
+}
+
+// First dimension: fst selection cts (coefficient form) -> s->fd (other cts).
+void run_first_dim(onion_server *s, const uint64_t *sel, bool time_core) {
+  const uint32_t fst = s->P.fst_dim, other = s->P.other_dim;
+  k_sel_ntt<<<fst * 2, kNttThreads, 0, s->st>>>(sel, s->Qs, fst, s->Tq);
+  if (time_core) (void)hipEventRecord(s->ev[6], s->st);
+  if (fst % 16 == 0 && fst <= (uint32_t)kMaxFst) {
+    const uint32_t rpb = rows_per_block(other);
+    dim3 grid(kN, (other + rpb - 1) / rpb);
+    k_first_dim_b1<<<grid, 256, 0, s->st>>>(s->db, s->Qs, s->fd, fst, other, s->I.num_pt, rpb, s->mq);
+  } else {
+    k_first_dim_generic<<<kN, 256, 0, s->st>>>(s->db, s->Qs, s->fd, fst, other, s->I.num_pt, s->mq);
+  }
+  if (time_core) (void)hipEventRecord(s->ev[7], s->st);
+  k_ntt_inv<<<other * 2, kNttThreads, 0, s->st>>>(s->fd, s->fd, s->Tq);
+}
+
+// One multiplexer level over n cts in `cts` with RGSW `g`.
+void run_mux_level(onion_server *s, uint64_t *cts, uint64_t n, const uint64_t *g) {
+  const uint32_t l = s->P.gsw_l;
+  const uint64_t half = n / 2;
+  k_decomp_ntt<<<(uint32_t)(half * 2 * l), kNttThreads, 0, s->st>>>(
+      cts, cts + half * 2 * kN, s->digits, l, s->I.base_log2, s->Tq);
+  k_ext_mac_intt_add<<<(uint32_t)(half * 2), kNttThreads, 0, s->st>>>(s->digits, g, 2 * l, cts,
+                                                                      s->Tq, s->mq);
+}
+
+void run_mod_switch(onion_server *s, uint64_t *ct) {
+  k_mod_switch<<<(2 * kN + 255) / 256, 256, 0, s->st>>>(ct, 2 * kN, s->ms_M, s->ms_f);
+}
+
+int lookup(onion_server *s, uint64_t id, ClientKeys &out, bool need_rk) {
+  auto it = s->clients.find(id);
+  if (it == s->clients.end() || !it->second.gk)
+    return fail(ONION_ENOKEY, "no Galois keys for client " + std::to_string(id));
+  if (need_rk && !it->second.rk)
+    return fail(ONION_ENOKEY, "no RGSW key for client " + std::to_string(id));
+  out = it->second;
+  return ONION_OK;
+}
+
+int check_db(onion_server *s) {
+  if (!s->db || s->n_written != s->I.num_pt)
+    return fail(ONION_ESTATE, "database incomplete: " + std::to_string(s->n_written) + " of " +
+                                  std::to_string(s->I.num_pt) + " plaintexts written");
+  return ONION_OK;
+}
+
+// Full answer for the query at device pointer tree slot 1; response -> s->fd[0]
+int answer_device(onion_server *s, const ClientKeys &ck, bool timed) {
+  const uint32_t fst = s->P.fst_dim, l = s->P.gsw_l;
+  const uint64_t *leaves = s->tree + s->w * 2 * kN;
+  if (timed) HIPCHK(hipEventRecord(s->ev[0], s->st));
+  run_expand(s, ck);
+  if (timed) HIPCHK(hipEventRecord(s->ev[1], s->st));
+  run_convert(s, ck, leaves + (size_t)fst * 2 * kN);
+  if (timed) HIPCHK(hipEventRecord(s->ev[2], s->st));
+  run_first_dim(s, leaves, timed);
+  if (timed) HIPCHK(hipEventRecord(s->ev[3], s->st));
+  uint64_t n = s->P.other_dim;
+  for (uint32_t d = 0; d + 1 < s->I.ndim; ++d) {
+    run_mux_level(s, s->fd, n, s->rgsw + (size_t)d * 2 * l * 2 * kN);
+    n /= 2;
+  }
+  if (timed) HIPCHK(hipEventRecord(s->ev[4], s->st));
+  run_mod_switch(s, s->fd);
+  if (timed) HIPCHK(hipEventRecord(s->ev[5], s->st));
+  HIPCHK(hipGetLastError());
+  return ONION_OK;
+}
+
+int collect_times(onion_server *s) {
+  HIPCHK(hipEventSynchronize(s->ev[5]));
+  float ms[5], core = 0, tot = 0;
+  for (int i = 0; i < 5; ++i) HIPCHK(hipEventElapsedTime(&ms[i], s->ev[i], s->ev[i + 1]));
+  HIPCHK(hipEventElapsedTime(&core, s->ev[6], s->ev[7]));
+  HIPCHK(hipEventElapsedTime(&tot, s->ev[0], s->ev[5]));
+  s->last.expand = ms[0];
+  s->last.convert = ms[1];
+  s->last.first_dim = ms[2];
+  s->last.other_dim = ms[3];
+  s->last.mod_switch = ms[4];
+  s->last.total = tot;
+  s->last.first_dim_core = core;
+  return ONION_OK;
+}
+
+}  // namespace
+
+// =====================================================================================
+// C ABI
+// =====================================================================================
+
+extern "C" {
+
+const char *onion_gpu_last_error(void) { return g_err.c_str(); }
+
+int onion_gpu_device_count(int *count) {
+  if (!count) return fail(ONION_EINVAL, "null argument");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return ONION_OK;
+}
+
+int onion_gpu_create(const onion_params *params, onion_server **out) {
+  if (!params || !out) return fail(ONION_EINVAL, "null argument");
+  const onion_params P = *params;
+  if (P.poly_degree != (uint32_t)kN)
+    return fail(ONION_EINVAL, "the device path serves poly_degree 2048 only");
+  if (P.plain_bits < 2 || P.plain_bits > 30) return fail(ONION_EINVAL, "plain_bits out of range");
+  if (P.gsw_l == 0 || P.gsw_l_key == 0 || P.gsw_l > 16 || P.gsw_l_key > 32)
+    return fail(ONION_EINVAL, "gsw_l / gsw_l_key out of range");
+  if (P.fst_dim == 0 || P.other_dim == 0 || (P.other_dim & (P.other_dim - 1)))
+    return fail(ONION_EINVAL, "fst_dim must be > 0 and other_dim a power of two");
+  if (P.fst_dim > (uint32_t)kMaxFst) return fail(ONION_EINVAL, "fst_dim above 1024");
+
+  onion_server *s = new onion_server();
+  s->P = P;
+  onion_param_info &I = s->I;
+  const uint64_t N = kN;
+  auto big = host::primes_down(2 * N, 60, 2);
+  auto sq = host::primes_down(2 * N, 27, 1);
+  if (big.size() != 2 || sq.size() != 1) {
+    delete s;
+    return fail(ONION_EINVAL, "no NTT-friendly primes");
+  }
+  I.q = big[1];  // CoeffModulus::Create hands out its prime list from the back
+  I.p = big[0];
+  I.small_q = sq[0];
+  I.t = host::smallest_prime_above_pow2((int)P.plain_bits - 1);
+  const uint32_t qbits = 64 - __builtin_clzll(I.q);
+  I.base_log2 = (qbits + P.gsw_l - 1) / P.gsw_l;
+  I.base_log2_key = (qbits + P.gsw_l_key - 1) / P.gsw_l_key;
+  I.ndim = 1 + ceil_log2(P.other_dim);
+  I.useful_cnt = P.fst_dim + P.gsw_l * (I.ndim - 1);
+  I.expan_height = ceil_log2(I.useful_cnt);
+  if (I.expan_height > 11) {
+    delete s;
+    return fail(ONION_EINVAL, "expansion height above log2(N)");
+  }
+  I.num_pt = (uint64_t)P.fst_dim * P.other_dim;
+  I.galois_key_words = I.expan_height * 4 * kN;
+  I.rgsw_key_words = 2 * P.gsw_l_key * 2 * kN;
+  const uint32_t wbits = (uint32_t)std::ceil(std::log2((double)I.small_q));
+  I.response_bytes = (2 * kN * wbits + 7) / 8;
+  s->w = uint64_t(1) << I.expan_height;
+
+  int rc;
+  if (hipGetDevice(&s->device) != hipSuccess) {
+    delete s;
+    return fail(ONION_EHIP, "no HIP device");
+  }
+  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    return fail(ONION_EHIP, "hipStreamCreate failed");
+  }
+  for (auto &e : s->ev) (void)hipEventCreate(&e);
+
+  // NTT tables and constants
+  host::Tables tq = host::make_tables(I.q, kN), tp = host::make_tables(I.p, kN);
+  if ((rc = dalloc(&s->tabs, 8 * N))) { onion_gpu_destroy(s); return rc; }
+  std::vector<uint64_t> h(8 * N);
+  const std::vector<uint64_t> *src[8] = {&tq.rp, &tq.rps, &tq.irp, &tq.irps, &tp.rp, &tp.rps, &tp.irp, &tp.irps};
+  for (int i = 0; i < 8; ++i) memcpy(h.data() + i * N, src[i]->data(), N * 8);
+  if (hipMemcpy(s->tabs, h.data(), 8 * N * 8, hipMemcpyHostToDevice) != hipSuccess) {
+    onion_gpu_destroy(s);
+    return fail(ONION_EHIP, "table upload failed");
+  }
+  s->Tq = NttTab{s->tabs, s->tabs + N, s->tabs + 2 * N, s->tabs + 3 * N, I.q, tq.n_inv, tq.n_inv_s};
+  s->Tp = NttTab{s->tabs + 4 * N, s->tabs + 5 * N, s->tabs + 6 * N, s->tabs + 7 * N, I.p, tp.n_inv, tp.n_inv_s};
+  s->mq.q = I.q;
+  host::barrett_ratio(I.q, s->mq.r_lo, s->mq.r_hi);
+  s->mp.q = I.p;
+  host::barrett_ratio(I.p, s->mp.r_lo, s->mp.r_hi);
+  s->ks.half_p = I.p >> 1;
+  s->ks.half_p_mod_q = s->ks.half_p % I.q;
+  s->ks.p_inv = host::invmod(I.p % I.q, I.q);
+  s->ks.p_inv_s = host::shoup(s->ks.p_inv, I.q);
+  s->lift = LiftConst{I.t, (I.t + 1) / 2, I.q - I.t};
+  for (uint32_t a = 0; a < I.expan_height; ++a) {
+    const uint64_t g = N / (uint64_t(1) << a) + 1;
+    // g is odd, so it is invertible mod 2N
+    uint64_t gi = 1;
+    for (uint64_t x = 1; x < 2 * N; x += 2)
+      if ((x * g) % (2 * N) == 1) { gi = x; break; }
+    s->ginv.push_back((uint32_t)gi);
+  }
+  host::mod_switch_scale(I.q, I.small_q, s->ms_M, s->ms_f);
+
+  // database and scratch
+  const uint64_t ct = 2 * N;
+  const uint32_t nd = I.ndim - 1;
+  const uint64_t conv_digits = (uint64_t)nd * P.gsw_l * 2 * P.gsw_l_key * N;
+  const uint64_t mux_digits = (uint64_t)(P.other_dim / 2) * 2 * P.gsw_l * N;
+  s->digit_words = std::max<uint64_t>(std::max(conv_digits, mux_digits), N);
+  s->io_words = std::max<uint64_t>({(uint64_t)I.useful_cnt * ct, (uint64_t)P.other_dim * ct,
+                                    (uint64_t)std::max(1u, nd) * 2 * P.gsw_l * ct, 64 * ct,
+                                    (uint64_t)P.fst_dim * ct});
+  if ((rc = dalloc(&s->db, I.num_pt * N)) || (rc = dalloc(&s->tree, 2 * s->w * ct)) ||
+      (rc = dalloc(&s->rgsw, (uint64_t)std::max(1u, nd) * 2 * P.gsw_l * ct)) ||
+      (rc = dalloc(&s->fd, (uint64_t)P.other_dim * ct)) || (rc = dalloc(&s->digits, s->digit_words)) ||
+      (rc = dalloc(&s->Qs, (uint64_t)N * P.fst_dim * 4)) || (rc = dalloc(&s->io, s->io_words)) ||
+      (rc = dalloc(&s->pack, (uint64_t)I.response_bytes + 64))) {
+    std::string msg = g_err;
+    onion_gpu_destroy(s);
+    return fail(rc, msg);
+  }
+  s->pt_written.assign(I.num_pt, 0);
+  *out = s;
+  return ONION_OK;
+}
+
+int onion_gpu_destroy(onion_server *s) {
+  if (!s) return ONION_OK;
+  if (s->st) (void)hipStreamSynchronize(s->st);
+  for (auto &kv : s->clients) {
+    dfree(kv.second.gk);
+    dfree(kv.second.rk);
+  }
+  void *bufs[] = {s->tabs, s->db, s->tree, s->rgsw, s->fd, s->digits, s->Qs, s->io, s->pack};
+  for (void *b : bufs) dfree(b);
+  for (auto &e : s->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (s->st) (void)hipStreamDestroy(s->st);
+  delete s;
+  return ONION_OK;
+}
+
+int onion_gpu_param_info(const onion_server *s, onion_param_info *out) {
+  if (!s || !out) return fail(ONION_EINVAL, "null argument");
+  *out = s->I;
+  return ONION_OK;
+}
+
+int onion_gpu_db_generate(onion_server *s, uint64_t seed, int mode) {
+  if (!s) return fail(ONION_EINVAL, "null server");
+  if (mode != 0 && mode != 1) return fail(ONION_EINVAL, "mode must be 0 or 1");
+  if (mode == 1 && s->P.plain_bits != 17)
+    return fail(ONION_EINVAL, "entry mode needs 16 bits per coefficient (plain_bits 17)");
+  std::lock_guard<std::mutex> lk(s->mu);
+  const uint64_t key = stream_key(seed, 0x4442ULL);
+  const uint64_t chunk = 65535;
+  for (uint64_t lo = 0; lo < s->I.num_pt; lo += chunk) {
+    const uint64_t n = std::min(chunk, s->I.num_pt - lo);
+    k_db_generate<<<(uint32_t)n, kNttThreads, 0, s->st>>>(s->db, lo, s->I.num_pt, key, mode, s->lift, s->Tq);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->st));
+  std::fill(s->pt_written.begin(), s->pt_written.end(), 1);
+  s->n_written = s->I.num_pt;
+  return ONION_OK;
+}
+
+int onion_gpu_db_upload_plain(onion_server *s, uint64_t pt_lo, uint64_t n_pt, const uint64_t *coeffs) {
+  if (!s || (!coeffs && n_pt)) return fail(ONION_EINVAL, "null argument");
+  if (pt_lo > s->I.num_pt || n_pt > s->I.num_pt - pt_lo)
+    return fail(ONION_EINVAL, "plaintext range beyond the database");
+  std::lock_guard<std::mutex> lk(s->mu);
+  const uint64_t chunk = std::max<uint64_t>(1, s->io_words / kN);
+  for (uint64_t off = 0; off < n_pt; off += chunk) {
+    const uint64_t n = std::min(chunk, n_pt - off);
+    for (uint64_t i = 0; i < n * kN; ++i)
+      if (coeffs[off * kN + i] >= s->I.t) return fail(ONION_EINVAL, "plaintext coefficient >= t");
+    HIPCHK(hipMemcpyAsync(s->io, coeffs + off * kN, n * kN * 8, hipMemcpyHostToDevice, s->st));
+    k_db_plain<<<(uint32_t)n, kNttThreads, 0, s->st>>>(s->io, s->db, pt_lo + off, s->I.num_pt, s->lift, s->Tq);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->st));
+    for (uint64_t i = 0; i < n; ++i) {
+      uint8_t &f = s->pt_written[pt_lo + off + i];
+      if (!f) { f = 1; ++s->n_written; }
+    }
+  }
+  return ONION_OK;
+}
+
+int onion_gpu_db_read_ntt(onion_server *s, uint64_t pt, uint64_t *out) {
+  if (!s || !out || pt >= s->I.num_pt) return fail(ONION_EINVAL, "bad argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  HIPCHK(hipMemcpy2DAsync(out, 8, s->db + pt, s->I.num_pt * 8, 8, kN, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return ONION_OK;
+}
+
+int onion_gpu_set_galois_key(onion_server *s, uint64_t id, const uint64_t *keys, size_t n) {
+  if (!s || !keys) return fail(ONION_EINVAL, "null argument");
+  if (n != s->I.galois_key_words)
+    return fail(ONION_EINVAL, "Galois key size " + std::to_string(n) + " != " +
+                                  std::to_string(s->I.galois_key_words));
+  std::lock_guard<std::mutex> lk(s->mu);
+  ClientKeys &ck = s->clients[id];
+  if (!ck.gk) {
+    int rc = dalloc(&ck.gk, n);
+    if (rc) return rc;
+  }
+  HIPCHK(hipMemcpyAsync(ck.gk, keys, n * 8, hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return ONION_OK;
+}
+
+int onion_gpu_set_rgsw_key(onion_server *s, uint64_t id, const uint64_t *rows, size_t n) {
+  if (!s || !rows) return fail(ONION_EINVAL, "null argument");
+  if (n != s->I.rgsw_key_words)
+    return fail(ONION_EINVAL, "RGSW key size " + std::to_string(n) + " != " +
+                                  std::to_string(s->I.rgsw_key_words));
+  std::lock_guard<std::mutex> lk(s->mu);
+  ClientKeys &ck = s->clients[id];
+  if (!ck.rk) {
+    int rc = dalloc(&ck.rk, n);
+    if (rc) return rc;
+  }
+  HIPCHK(hipMemcpyAsync(ck.rk, rows, n * 8, hipMemcpyHostToDevice, s->st));
+  // GSWEval::gsw_ntt_negacyclic_harvey: every half-row to NTT form
+  k_ntt_fwd<<<(uint32_t)(n / kN), kNttThreads, 0, s->st>>>(ck.rk, ck.rk, s->Tq);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->st));
+  return ONION_OK;
+}
+
+int onion_gpu_drop_client(onion_server *s, uint64_t id) {
+  if (!s) return fail(ONION_EINVAL, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->clients.find(id);
+  if (it == s->clients.end()) return fail(ONION_ENOKEY, "unknown client " + std::to_string(id));
+  HIPCHK(hipStreamSynchronize(s->st));
+  dfree(it->second.gk);
+  dfree(it->second.rk);
+  s->clients.erase(it);
+  return ONION_OK;
+}
+
+int onion_gpu_answer(onion_server *s, uint64_t id, const uint64_t *query, uint64_t *resp) {
+  if (!s || !query || !resp) return fail(ONION_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  int rc;
+  if ((rc = check_db(s))) return rc;
+  ClientKeys ck;
+  if ((rc = lookup(s, id, ck, s->I.ndim > 1))) return rc;
+  HIPCHK(hipMemcpyAsync(s->tree + 2 * kN, query, 2 * kN * 8, hipMemcpyHostToDevice, s->st));
+  if ((rc = answer_device(s, ck, true))) return rc;
+  HIPCHK(hipMemcpyAsync(resp, s->fd, 2 * kN * 8, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return collect_times(s);
+}
+
+int onion_gpu_answer_batch(onion_server *s, uint32_t n, const uint64_t *ids, const uint64_t *queries,
+                           uint64_t *resps) {
+  if (!s || (n && (!ids || !queries || !resps))) return fail(ONION_EINVAL, "null argument");
+  for (uint32_t i = 0; i < n; ++i) {
+    int rc = onion_gpu_answer(s, ids[i], queries + (size_t)i * 2 * kN, resps + (size_t)i * 2 * kN);
+    if (rc) return rc;
+  }
+  return ONION_OK;
+}
+
+int onion_gpu_pack_response(onion_server *s, const uint64_t *resp, uint8_t *out, size_t cap,
+                            size_t *n_bytes) {
+  if (!s || !resp || !out) return fail(ONION_EINVAL, "null argument");
+  if (cap < s->I.response_bytes) return fail(ONION_EINVAL, "output buffer too small");
+  std::lock_guard<std::mutex> lk(s->mu);
+  const uint32_t wbits = (uint32_t)std::ceil(std::log2((double)s->I.small_q));
+  HIPCHK(hipMemcpyAsync(s->io, resp, 2 * kN * 8, hipMemcpyHostToDevice, s->st));
+  k_pack_response<<<(2 * kN / 8 + 255) / 256, 256, 0, s->st>>>(s->io, s->pack, 2 * kN, wbits);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, s->pack, s->I.response_bytes, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  if (n_bytes) *n_bytes = s->I.response_bytes;
+  return ONION_OK;
+}
+
+int onion_gpu_last_stage_times(const onion_server *s, onion_stage_times *out) {
+  if (!s || !out) return fail(ONION_EINVAL, "null argument");
+  *out = s->last;
+  return ONION_OK;
+}
+
+// ---- taps ---------------------------------------------------------------------------
+
+int onion_gpu_tap_expand(onion_server *s, uint64_t id, const uint64_t *query, uint64_t *leaves) {
+  if (!s || !query || !leaves) return fail(ONION_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  ClientKeys ck;
+  int rc;
+  if ((rc = lookup(s, id, ck, false))) return rc;
+  HIPCHK(hipMemcpyAsync(s->tree + 2 * kN, query, 2 * kN * 8, hipMemcpyHostToDevice, s->st));
+  run_expand(s, ck);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(leaves, s->tree + s->w * 2 * kN, (size_t)s->I.useful_cnt * 2 * kN * 8,
+                        hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return ONION_OK;
+}
+
+int onion_gpu_tap_convert(onion_server *s, uint64_t id, const uint64_t *leaves, uint64_t *rgsw) {
+  if (!s || !leaves || !rgsw) return fail(ONION_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  ClientKeys ck;
+  int rc;
+  if ((rc = lookup(s, id, ck, true))) return rc;
+  const uint32_t nd = s->I.ndim - 1;
+  const size_t n_ct = (size_t)nd * s->P.gsw_l;
+  if (nd == 0) return ONION_OK;
+  HIPCHK(hipMemcpyAsync(s->io, leaves + (size_t)s->P.fst_dim * 2 * kN, n_ct * 2 * kN * 8,
+                        hipMemcpyHostToDevice, s->st));
+  run_convert(s, ck, s->io);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(rgsw, s->rgsw, (size_t)nd * 2 * s->P.gsw_l * 2 * kN * 8, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return ONION_OK;
+}
+
+int onion_gpu_tap_first_dim(onion_server *s, const uint64_t *sel, uint64_t *out) {
+  if (!s || !sel || !out) return fail(ONION_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  int rc;
+  if ((rc = check_db(s))) return rc;
+  HIPCHK(hipMemcpyAsync(s->io, sel, (size_t)s->P.fst_dim * 2 * kN * 8, hipMemcpyHostToDevice, s->st));
+  run_first_dim(s, s->io, false);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, s->fd, (size_t)s->P.other_dim * 2 * kN * 8, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return ONION_OK;
+}
+
+int onion_gpu_tap_mux(onion_server *s, const uint64_t *cts, uint32_t n, const uint64_t *rgsw, uint64_t *out) {
+  if (!s || !cts || !rgsw || !out) return fail(ONION_EINVAL, "null argument");
+  if (n < 2 || (n & 1) || n > s->P.other_dim) return fail(ONION_EINVAL, "n must be even and <= other_dim");
+  std::lock_guard<std::mutex> lk(s->mu);
+  HIPCHK(hipMemcpyAsync(s->fd, cts, (size_t)n * 2 * kN * 8, hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemcpyAsync(s->rgsw, rgsw, (size_t)2 * s->P.gsw_l * 2 * kN * 8, hipMemcpyHostToDevice, s->st));
+  run_mux_level(s, s->fd, n, s->rgsw);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, s->fd, (size_t)(n / 2) * 2 * kN * 8, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return ONION_OK;
+}
+
+int onion_gpu_tap_ntt(onion_server *s, int which, int inverse, uint64_t *polys, uint32_t n) {
+  if (!s || !polys) return fail(ONION_EINVAL, "null argument");
+  if (which != 0 && which != 1) return fail(ONION_EINVAL, "which must be 0 (q) or 1 (p)");
+  if ((size_t)n * kN > s->io_words) return fail(ONION_EINVAL, "too many polynomials");
+  std::lock_guard<std::mutex> lk(s->mu);
+  const NttTab &T = which == 0 ? s->Tq : s->Tp;
+  HIPCHK(hipMemcpyAsync(s->io, polys, (size_t)n * kN * 8, hipMemcpyHostToDevice, s->st));
+  if (inverse)
+    k_ntt_inv<<<n, kNttThreads, 0, s->st>>>(s->io, s->io, T);
+  else
+    k_ntt_fwd<<<n, kNttThreads, 0, s->st>>>(s->io, s->io, T);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(polys, s->io, (size_t)n * kN * 8, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return ONION_OK;
+}
+
+int onion_gpu_tap_galois(onion_server *s, uint64_t id, uint32_t level, const uint64_t *ct, uint64_t *out) {
+  if (!s || !ct || !out) return fail(ONION_EINVAL, "null argument");
+  if (level >= s->I.expan_height) return fail(ONION_EINVAL, "level >= expansion height");
+  std::lock_guard<std::mutex> lk(s->mu);
+  ClientKeys ck;
+  int rc;
+  if ((rc = lookup(s, id, ck, false))) return rc;
+  // place ct at node 2^level; its left child is ct + Subs(ct), so
+  // Subs(ct) = left - ct
+  const uint64_t node = uint64_t(1) << level;
+  HIPCHK(hipMemcpyAsync(s->tree + node * 2 * kN, ct, 2 * kN * 8, hipMemcpyHostToDevice, s->st));
+  k_expand_level<<<1, kNttThreads, 0, s->st>>>(s->tree, ck.gk + (size_t)level * 4 * kN, level,
+                                               s->ginv[level], s->Tq, s->Tp, s->mq, s->mp, s->ks);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, s->tree + 2 * node * 2 * kN, 2 * kN * 8, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  const uint64_t q = s->I.q;
+  for (size_t i = 0; i < 2 * (size_t)kN; ++i) out[i] = out[i] >= ct[i] ? out[i] - ct[i] : out[i] + q - ct[i];
+  return ONION_OK;
+}
+
+int onion_gpu_tap_mod_switch(onion_server *s, const uint64_t *ct, uint64_t *out) {
+  if (!s || !ct || !out) return fail(ONION_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  HIPCHK(hipMemcpyAsync(s->io, ct, 2 * kN * 8, hipMemcpyHostToDevice, s->st));
+  run_mod_switch(s, s->io);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, s->io, 2 * kN * 8, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return ONION_OK;
+}
+
+}  // extern "C"

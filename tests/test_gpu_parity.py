@@ -1,0 +1,197 @@
+"""GPU parity: every stage of the HIP server path against the CPU oracle,
+bit for bit, on the same seeded keys, query and database.
+
+Shapes: the reference's CPU-runnable configuration (fst 256 x other 32,
+N = 2048, t = 65537, l = 5, l_key = 10) and a small odd shape (fst 12 x
+other 4) that exercises the generic first-dimension kernel.
+"""
+import numpy as np
+import pytest
+
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+N = 2048
+DB_SEED = 7
+CLIENT_SEED = 11
+CID = 5
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    from onionpirv2_amd import GpuServer
+    o = Oracle(fst=256, other=32)
+    srv = GpuServer(256, 32)
+    sk = o.keygen(CLIENT_SEED)
+    gk = o.galois_keys(CLIENT_SEED, sk)
+    rk = o.rgsw_key(CLIENT_SEED, sk)
+    srv.set_galois_key(CID, gk)
+    srv.set_rgsw_key(CID, rk)
+    srv.db_generate(DB_SEED, 0)
+    db = o.db_build(DB_SEED, 0)
+    yield dict(o=o, srv=srv, sk=sk, gk=gk, rk=rk, rkn=o.rgsw_key_ntt(rk), db=db)
+    srv.close()
+
+
+def test_derived_parameters(cfg1):
+    o, srv = cfg1["o"], cfg1["srv"]
+    i = srv.info
+    assert (i["q"], i["p"], i["t"], i["small_q"]) == (o.q, o.p, o.t, o.small_q)
+    assert (i["base_log2"], i["base_log2_key"], i["ndim"], i["expan_height"], i["useful_cnt"]) == \
+        (o.base_log2, o.base_log2_key, o.ndim, o.h, o.u)
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_ntt_matches_oracle(cfg1, which):
+    o, srv = cfg1["o"], cfg1["srv"]
+    rng = np.random.default_rng(1 + which)
+    m = o.q if which == 0 else o.p
+    polys = rng.integers(0, m, size=(8, N), dtype=np.uint64)
+    polys[0] = 0
+    polys[1] = m - 1
+    fwd = srv.tap_ntt(polys, which=which)
+    exp = np.stack([o.ntt(p, which) for p in polys]).reshape(-1)
+    assert np.array_equal(fwd, exp)
+    inv = srv.tap_ntt(fwd, which=which, inverse=True)
+    assert np.array_equal(inv, polys.reshape(-1))
+
+
+def test_galois_keyswitch_matches_oracle(cfg1):
+    o, srv = cfg1["o"], cfg1["srv"]
+    ct = o.encrypt(CLIENT_SEED, 900, cfg1["sk"], np.arange(N, dtype=np.uint64) % o.t)
+    import ctypes as C
+    from oracle_lib import ptr, u64
+    for level in range(o.h):
+        g = N // (1 << level) + 1
+        exp = u64(2 * N)
+        o.L.orc_apply_galois(o.c, ptr(ct), g, ptr(cfg1["gk"][level * 4 * N:(level + 1) * 4 * N].copy()), ptr(exp))
+        got = srv.tap_galois(CID, level, ct)
+        assert np.array_equal(got, exp), f"level {level}"
+
+
+def test_expand_matches_oracle(cfg1):
+    o, srv = cfg1["o"], cfg1["srv"]
+    q = o.query(CLIENT_SEED, 1, cfg1["sk"], 4242)
+    exp = o.expand(q, cfg1["gk"])
+    got = srv.tap_expand(CID, q)
+    assert np.array_equal(got, exp)
+
+
+def test_convert_matches_oracle(cfg1):
+    o, srv = cfg1["o"], cfg1["srv"]
+    q = o.query(CLIENT_SEED, 2, cfg1["sk"], 1234)
+    leaves = o.expand(q, cfg1["gk"])
+    exp = o.convert(leaves, cfg1["rkn"])
+    got = srv.tap_convert(CID, leaves)
+    assert np.array_equal(got, exp)
+
+
+def test_first_dim_matches_oracle(cfg1):
+    o, srv = cfg1["o"], cfg1["srv"]
+    rng = np.random.default_rng(3)
+    sel = rng.integers(0, o.q, size=(o.fst, 2 * N), dtype=np.uint64)
+    exp = o.first_dim(cfg1["db"], sel)
+    got = srv.tap_first_dim(sel)
+    assert np.array_equal(got, exp)
+
+
+def test_mux_matches_oracle(cfg1):
+    o, srv = cfg1["o"], cfg1["srv"]
+    rng = np.random.default_rng(4)
+    cts = rng.integers(0, o.q, size=(32, 2 * N), dtype=np.uint64)
+    for bit in (0, 1):
+        g = o.rgsw(CLIENT_SEED, 77 + bit, cfg1["sk"], bit)
+        exp = o.mux_level(cts, g)
+        got = srv.tap_mux(cts, g)
+        assert np.array_equal(got, exp), f"bit {bit}"
+
+
+def test_mod_switch_matches_oracle(cfg1):
+    o, srv = cfg1["o"], cfg1["srv"]
+    rng = np.random.default_rng(5)
+    x = rng.integers(0, o.q, size=2 * N, dtype=np.uint64)
+    x[:8] = [0, 1, o.q - 1, o.q // 2, o.q // 2 + 1, 2**33, 2**33 + 1, 4294967296 * 3]
+    exp = np.array([o.L.orc_mod_switch_one(int(v), o.q, o.small_q) for v in x], dtype=np.uint64)
+    got = srv.tap_mod_switch(x)
+    assert np.array_equal(got, exp)
+
+
+def test_db_generate_matches_oracle(cfg1):
+    o, srv = cfg1["o"], cfg1["srv"]
+    db = cfg1["db"].reshape(N, o.num_pt)
+    for pt in (0, 1, 255, 4097, o.num_pt - 1):
+        assert np.array_equal(srv.db_read_ntt(pt), db[:, pt]), f"pt {pt}"
+
+
+@pytest.mark.parametrize("idx", [0, 4242, 8191])
+def test_answer_bit_exact_and_decrypts(cfg1, idx):
+    o, srv = cfg1["o"], cfg1["srv"]
+    q = o.query(CLIENT_SEED, 100 + idx, cfg1["sk"], idx)
+    exp, _ = o.answer(cfg1["db"], cfg1["gk"], cfg1["rkn"], q)
+    got = srv.answer(CID, q)
+    assert np.array_equal(got, exp)
+    pt = o.decrypt_small(cfg1["sk"], got)
+    assert np.array_equal(pt, o.db_plaintext(DB_SEED, 0, idx))
+    packed = srv.pack_response(got)
+    assert packed == o.pack(exp)
+    t = srv.stage_times()
+    assert t["total"] > 0
+
+
+def test_unknown_client_raises(cfg1):
+    from onionpirv2_amd import OnionKeyError
+    q = np.zeros(2 * N, np.uint64)
+    with pytest.raises(OnionKeyError):
+        cfg1["srv"].answer(999, q)
+
+
+def test_small_shape_generic_kernel_and_upload():
+    """fst = 12 (not a multiple of 16) takes the generic first-dim kernel;
+    the database arrives through the plaintext upload path."""
+    from onionpirv2_amd import GpuServer
+    o = Oracle(fst=12, other=4)
+    rng = np.random.default_rng(9)
+    pts = rng.integers(0, o.t, size=(o.num_pt, N), dtype=np.uint64)
+    pts[0, :4] = o.t - 1
+    with GpuServer(12, 4) as srv:
+        srv.db_upload_plain(0, pts)
+        db = np.zeros(o.num_pt * N, np.uint64)
+        for i in range(o.num_pt):
+            db.reshape(N, o.num_pt)[:, i] = o.ntt(np.where(pts[i] >= (o.t + 1) // 2, pts[i] + o.q - o.t, pts[i]))
+        for i in (0, 5, o.num_pt - 1):
+            assert np.array_equal(srv.db_read_ntt(i), db.reshape(N, o.num_pt)[:, i])
+        sk = o.keygen(21)
+        gk, rk = o.galois_keys(21, sk), o.rgsw_key(21, sk)
+        srv.set_galois_key(1, gk)
+        srv.set_rgsw_key(1, rk)
+        for idx in (0, 7, o.num_pt - 1):
+            q = o.query(21, idx, sk, idx)
+            exp, _ = o.answer(db, gk, o.rgsw_key_ntt(rk), q)
+            got = srv.answer(1, q)
+            assert np.array_equal(got, exp)
+            assert np.array_equal(o.decrypt_small(sk, got), pts[idx])
+
+
+def test_first_dim_wraps_mod_2_128():
+    """All database words and selection values q-1: 512 * (q-1)^2 >= 2^128,
+    so the reference's uint128 accumulator wraps; the device must agree."""
+    from onionpirv2_amd import GpuServer
+    o = Oracle(fst=512, other=2)
+    big = np.full(N, o.q - 1, np.uint64)
+    coeff = o.intt(big)  # selection poly whose NTT is all q-1
+    sel = np.tile(np.concatenate([coeff, coeff]), (o.fst, 1))
+    # database plaintexts whose NTT words are all q-1 are not reachable from
+    # coefficients < t, so compare the raw first-dimension arithmetic on a
+    # database of lifted t-1 plaintexts plus the closed form for all-(q-1).
+    with GpuServer(512, 2) as srv:
+        pts = np.full((o.num_pt, N), o.t - 1, np.uint64)
+        srv.db_upload_plain(0, pts)
+        db = np.zeros((N, o.num_pt), np.uint64)
+        lifted = o.ntt(np.full(N, o.t - 1 + o.q - o.t, np.uint64))
+        db[:, :] = lifted[:, None]
+        wraps = sum(1 for v in lifted if 512 * int(v) * (o.q - 1) >= 2**128)
+        assert wraps > 100  # the uint128 sum overflows in many slots
+        exp = o.first_dim(db.reshape(-1), sel)
+        got = srv.tap_first_dim(sel)
+        assert np.array_equal(got, exp)
