@@ -66,8 +66,11 @@ __global__ __launch_bounds__(kNttThreads) void k_expand_level(uint64_t *tree, co
   }
   __syncthreads();
 
-  uint64_t res[2][kPer];
-#pragma unroll
+  // Component comp of c' is sigma(c0) + KS_0 (comp 0) or KS_1 (comp 1); the
+  // children's polynomial comp is written as soon as it is known.
+  uint64_t *left = tree + (2 * node) * 2 * kN;       // c_{2i}
+  uint64_t *right = tree + (2 * node + 1) * 2 * kN;  // c_{2i+1}
+#pragma unroll 1
   for (int comp = 0; comp < 2; ++comp) {
     const uint64_t *kq = key + (size_t)(2 * comp) * kN, *kp = kq + kN;
 #pragma unroll
@@ -79,42 +82,31 @@ __global__ __launch_bounds__(kNttThreads) void k_expand_level(uint64_t *tree, co
     __syncthreads();
     ntt_inv_lds<kN>(A, Tq);
     ntt_inv_lds<kN>(B, Tp);
-#pragma unroll
+    const uint64_t *cin = comp == 0 ? c0 : c1;
+#pragma unroll 2
     for (int r = 0; r < kPer; ++r) {
       const int e = tid + r * kNttThreads;
       const uint64_t xq = ntt_inv_final(A[e], Tq);
       const uint64_t xp = ntt_inv_final(B[e], Tp);
       uint64_t last = xp + ks.half_p;  // (X_p + floor(p/2)) mod p
       if (last >= p) last -= p;
-      uint64_t lq = last >= q ? last - q : last;  // p < 2q
+      const uint64_t lq = last >= q ? last - q : last;  // p < 2q
       const uint64_t diff = sub_mod(add_mod(xq, ks.half_p_mod_q, q), lq, q);
-      uint64_t v = mul_shoup_lazy(diff, ks.p_inv, ks.p_inv_s, q);
-      res[comp][r] = v >= q ? v - q : v;
+      uint64_t ksv = mul_shoup_lazy(diff, ks.p_inv, ks.p_inv_s, q);
+      if (ksv >= q) ksv -= q;
+      const uint64_t cp = comp == 0 ? add_mod(galois_gather(c0, e, ginv, q), ksv, q) : ksv;
+      const uint64_t a = cin[e];
+      left[(size_t)comp * kN + e] = add_mod(a, cp, q);
+      uint64_t d = sub_mod(a, cp, q);
+      // multiply by x^-k: slot e moves to e - k; wrapped slots are negated
+      int dst = e - (int)k;
+      if (dst < 0) {
+        dst += kN;
+        d = d ? q - d : 0;
+      }
+      right[(size_t)comp * kN + dst] = d;
     }
     __syncthreads();
-  }
-
-  // c' = (sigma(c0) + res0, res1); children
-  uint64_t *left = tree + (2 * node) * 2 * kN;       // c_{2i}
-  uint64_t *right = tree + (2 * node + 1) * 2 * kN;  // c_{2i+1}
-#pragma unroll
-  for (int r = 0; r < kPer; ++r) {
-    const int e = tid + r * kNttThreads;
-    const uint64_t cp0 = add_mod(galois_gather(c0, e, ginv, q), res[0][r], q);
-    const uint64_t cp1 = res[1][r];
-    const uint64_t a0 = c0[e], a1 = c1[e];
-    left[e] = add_mod(a0, cp0, q);
-    left[kN + e] = add_mod(a1, cp1, q);
-    uint64_t d0 = sub_mod(a0, cp0, q), d1 = sub_mod(a1, cp1, q);
-    // multiply by x^-k: slot e moves to e - k; wrapped slots are negated
-    int dst = e - (int)k;
-    if (dst < 0) {
-      dst += kN;
-      d0 = d0 ? q - d0 : 0;
-      d1 = d1 ? q - d1 : 0;
-    }
-    right[dst] = d0;
-    right[kN + dst] = d1;
   }
 }
 

@@ -249,7 +249,7 @@ class Oracle:
         return out
 
     def mux_level(self, cts, rgsw):
-        x = np.array(cts, dtype=np.uint64)
+        x = np.array(cts, dtype=np.uint64).reshape(-1)
         n = x.size // (2 * self.N)
         self.L.orc_mux_level(self.c, ptr(x), n, ptr(np.ascontiguousarray(rgsw)))
         return x[: (n // 2) * 2 * self.N].copy()
