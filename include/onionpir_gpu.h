@@ -77,6 +77,10 @@ const char *onion_gpu_last_error(void);
 /* Number of visible HIP devices (0 when none); no reference counterpart. */
 int onion_gpu_device_count(int *count);
 
+/* PirParams::PirParams (src/pir.cpp:29-95) and getters (src/includes/pir.h:
+ * 29-68): derive moduli, gadget bases and shape without touching a device. */
+int onion_params_derive(const onion_params *params, onion_param_info *out);
+
 /* PirParams::PirParams + PirServer::PirServer (src/pir.cpp:29-95,
  * src/server.cpp:25-39): validate the shape, derive the moduli, allocate the
  * NTT-form database and the per-query scratch on the current HIP device. */

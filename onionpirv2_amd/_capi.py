@@ -56,6 +56,7 @@ _lib = None
 _SIGS = {
     "onion_gpu_last_error": (C.c_char_p, []),
     "onion_gpu_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "onion_params_derive": (C.c_int, [C.POINTER(OnionParams), C.POINTER(OnionParamInfo)]),
     "onion_gpu_create": (C.c_int, [C.POINTER(OnionParams), C.POINTER(C.c_void_p)]),
     "onion_gpu_destroy": (C.c_int, [C.c_void_p]),
     "onion_gpu_param_info": (C.c_int, [C.c_void_p, C.POINTER(OnionParamInfo)]),

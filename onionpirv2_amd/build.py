@@ -39,8 +39,8 @@ def build(verbose: bool = False) -> str:
         if src.endswith(".hip"):
             cmd = [hipcc()] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"), "-c",
                                           os.path.join(CSRC, src), "-o", obj]
-        else:
-            cmd = ["g++", "-O3", "-std=c++17", "-fPIC", "-Wall", "-I", os.path.join(ROOT, "include"),
+        else:  # host-only C++ (hipcc supplies the HIP headers the shared helpers use)
+            cmd = [hipcc(), "-O3", "-std=c++17", "-fPIC", "-Wall", "-I", os.path.join(ROOT, "include"),
                    "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)

@@ -15,6 +15,7 @@
 
 #include "../../include/onionpir_gpu.h"
 #include "host_math.hpp"
+#include "params.hpp"
 #include "k_db.hpp"
 #include "k_expand.hpp"
 #include "k_first_dim.hpp"
@@ -43,12 +44,6 @@ struct ClientKeys {
   uint64_t *gk = nullptr;  // h x 4N, NTT form
   uint64_t *rk = nullptr;  // 2 l_key x 2N, NTT form
 };
-
-uint32_t ceil_log2(uint64_t x) {
-  uint32_t r = 0;
-  while ((uint64_t(1) << r) < x) ++r;
-  return r;
-}
 
 }  // namespace
 
@@ -237,47 +232,26 @@ int onion_gpu_device_count(int *count) {
   return ONION_OK;
 }
 
+int onion_params_derive(const onion_params *params, onion_param_info *out) {
+  if (!params || !out) return fail(ONION_EINVAL, "null argument");
+  std::string err;
+  if (derive_params(*params, *out, err) != ONION_OK) return fail(ONION_EINVAL, err);
+  return ONION_OK;
+}
+
 int onion_gpu_create(const onion_params *params, onion_server **out) {
   if (!params || !out) return fail(ONION_EINVAL, "null argument");
   const onion_params P = *params;
-  if (P.poly_degree != (uint32_t)kN)
-    return fail(ONION_EINVAL, "the device path serves poly_degree 2048 only");
-  if (P.plain_bits < 2 || P.plain_bits > 30) return fail(ONION_EINVAL, "plain_bits out of range");
-  if (P.gsw_l == 0 || P.gsw_l_key == 0 || P.gsw_l > 16 || P.gsw_l_key > 32)
-    return fail(ONION_EINVAL, "gsw_l / gsw_l_key out of range");
-  if (P.fst_dim == 0 || P.other_dim == 0 || (P.other_dim & (P.other_dim - 1)))
-    return fail(ONION_EINVAL, "fst_dim must be > 0 and other_dim a power of two");
+  onion_param_info info;
+  std::string err;
+  if (derive_params(P, info, err, kN) != ONION_OK) return fail(ONION_EINVAL, err);
   if (P.fst_dim > (uint32_t)kMaxFst) return fail(ONION_EINVAL, "fst_dim above 1024");
 
   onion_server *s = new onion_server();
   s->P = P;
-  onion_param_info &I = s->I;
+  s->I = info;
+  const onion_param_info &I = s->I;
   const uint64_t N = kN;
-  auto big = host::primes_down(2 * N, 60, 2);
-  auto sq = host::primes_down(2 * N, 27, 1);
-  if (big.size() != 2 || sq.size() != 1) {
-    delete s;
-    return fail(ONION_EINVAL, "no NTT-friendly primes");
-  }
-  I.q = big[1];  // CoeffModulus::Create hands out its prime list from the back
-  I.p = big[0];
-  I.small_q = sq[0];
-  I.t = host::smallest_prime_above_pow2((int)P.plain_bits - 1);
-  const uint32_t qbits = 64 - __builtin_clzll(I.q);
-  I.base_log2 = (qbits + P.gsw_l - 1) / P.gsw_l;
-  I.base_log2_key = (qbits + P.gsw_l_key - 1) / P.gsw_l_key;
-  I.ndim = 1 + ceil_log2(P.other_dim);
-  I.useful_cnt = P.fst_dim + P.gsw_l * (I.ndim - 1);
-  I.expan_height = ceil_log2(I.useful_cnt);
-  if (I.expan_height > 11) {
-    delete s;
-    return fail(ONION_EINVAL, "expansion height above log2(N)");
-  }
-  I.num_pt = (uint64_t)P.fst_dim * P.other_dim;
-  I.galois_key_words = I.expan_height * 4 * kN;
-  I.rgsw_key_words = 2 * P.gsw_l_key * 2 * kN;
-  const uint32_t wbits = (uint32_t)std::ceil(std::log2((double)I.small_q));
-  I.response_bytes = (2 * kN * wbits + 7) / 8;
   s->w = uint64_t(1) << I.expan_height;
 
   int rc;

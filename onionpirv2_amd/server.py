@@ -18,6 +18,18 @@ def _u64(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.uint64)
 
 
+def _derive_sizes(p: OnionParams) -> dict:
+    info = OnionParamInfo()
+    check(_capi.lib().onion_params_derive(C.byref(p), C.byref(info)))
+    return {f: getattr(info, f) for f, _ in OnionParamInfo._fields_}
+
+
+def derive_params(fst_dim: int, other_dim: int, poly_degree: int = 2048, plain_bits: int = 17,
+                  gsw_l: int = 5, gsw_l_key: int = 10) -> dict:
+    """Moduli, gadget bases and shape of a configuration (no device needed)."""
+    return _derive_sizes(OnionParams(poly_degree, plain_bits, gsw_l, gsw_l_key, fst_dim, other_dim))
+
+
 class GpuServer:
     def __init__(self, fst_dim: int, other_dim: int, poly_degree: int = 2048, plain_bits: int = 17,
                  gsw_l: int = 5, gsw_l_key: int = 10):
