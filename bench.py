@@ -2,7 +2,8 @@
 """Benchmark of the MI355X PIR server answer path.
 
 Workload (the configuration the headline metric is quoted on): ring degree
-2048, t = 65537, l = 5, l_key = 10; first dimension 512 x 1024 further
+2048, plaintext modulus t = 32771 (PlainMod 16, the reference's setting for
+every N = 2048 preset; --plain-bits 17 gives t = 65537), l = 5, l_key = 10; first dimension 512 x 1024 further
 (524,288 plaintexts, 8 GiB NTT-form database resident in HBM); a batch of 64
 queries for distinct seeded indices, each from its own client with its own
 keys.  A step is one batch answered end to end (expand, convert, first
@@ -54,6 +55,9 @@ def main() -> int:
     ap.add_argument("--db-seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-queries", type=int, default=2)
+    ap.add_argument("--plain-bits", type=int, default=16)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (profiler byte-counter scale)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -65,7 +69,7 @@ def main() -> int:
     from onionpirv2_amd.client import GpuClient
 
     fst, other, B = args.fst, args.other, args.batch
-    srv = GpuServer(fst, other)
+    srv = GpuServer(fst, other, plain_bits=args.plain_bits)
     info = srv.info
     num_pt = info["num_pt"]
     db_bytes = num_pt * N * 8
@@ -73,13 +77,16 @@ def main() -> int:
     t0 = time.time()
     srv.db_generate(args.db_seed, 0)
     log(f"[bench] device DB ingest {time.time() - t0:.2f} s")
+    if args.calibrate_copy:
+        n = srv.tap_copy(args.calibrate_copy << 20)
+        log(f"[bench] calibration copy k_copy: {n} bytes read, {n} bytes written")
 
     rng = np.random.default_rng(12345)
     idx = rng.choice(num_pt, size=B, replace=False)
     clients, queries = [], []
     t0 = time.time()
     for i in range(B):
-        c = GpuClient(fst, other, client_id=i, seed=1000 + i)
+        c = GpuClient(fst, other, client_id=i, seed=1000 + i, plain_bits=args.plain_bits)
         srv.set_galois_key(i, c.galois_keys())
         srv.set_rgsw_key(i, c.rgsw_key())
         clients.append(c)
@@ -116,7 +123,7 @@ def main() -> int:
     ok = 0
     for i in range(B):
         pt = clients[i].decrypt(resps[i])
-        ok += int(np.array_equal(pt, dbgen.plaintext(args.db_seed, 0, int(idx[i]))))
+        ok += int(np.array_equal(pt, dbgen.plaintext(args.db_seed, 0, int(idx[i]), t=info["t"])))
     bit_exact = ok == B
     log(f"[bench] records recovered {ok}/{B}")
 
@@ -138,9 +145,10 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic: seeded records uniform in [0,t), seeded clients and indices",
-        "config": {"workload": f"{fst}x{other} plaintexts (N=2048, t=65537, l=5, l_key=10), "
+        "config": {"workload": f"{fst}x{other} plaintexts (N=2048, t={info['t']}, l=5, l_key=10), "
                                f"{db_bytes / 2**30:.0f} GiB NTT-form DB in HBM, batch of {B} queries",
-                   "fst_dim": fst, "other_dim": other, "batch": B, "poly_degree": N},
+                   "fst_dim": fst, "other_dim": other, "batch": B, "poly_degree": N,
+                   "plain_modulus": info["t"]},
         "batched": {"queries_per_s": B * args.steps / total_s, "step_ms": float(np.mean(step_ms)),
                     "p50_ms": pct(step_ms, 50), "p99_ms": pct(step_ms, 99),
                     "db_GBps_scanned": db_bytes * B / (np.mean(step_ms) * 1e-3) / 1e9},
@@ -171,7 +179,7 @@ def cpu_baseline(args, clients, queries, idx, srv):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle, ptr
 
-    o = Oracle(fst=args.fst, other=args.other)
+    o = Oracle(fst=args.fst, other=args.other, plain_bits=args.plain_bits)
     t0 = time.time()
     db = np.zeros(o.num_pt * N, np.uint64)
     workers = min(16, os.cpu_count() or 1)

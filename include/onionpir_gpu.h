@@ -155,6 +155,10 @@ int onion_gpu_tap_galois(onion_server *srv, uint64_t client_id, uint32_t level,
                          const uint64_t *ct, uint64_t *out);
 /* mod_switch_inplace (src/server.cpp:779-801) on 2N values. */
 int onion_gpu_tap_mod_switch(onion_server *srv, const uint64_t *ct, uint64_t *out);
+/* Plain device copy of n_bytes (multiple of 16) from the database to scratch;
+ * no reference counterpart: calibrates profiler byte counters.  *copied gets
+ * the size actually copied (clamped to the scratch size). */
+int onion_gpu_tap_copy(onion_server *srv, uint64_t n_bytes, uint64_t *copied);
 
 #ifdef __cplusplus
 }

@@ -78,6 +78,7 @@ _SIGS = {
     "onion_gpu_tap_ntt": (C.c_int, [C.c_void_p, C.c_int, C.c_int, U64P, C.c_uint32]),
     "onion_gpu_tap_galois": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, U64P, U64P]),
     "onion_gpu_tap_mod_switch": (C.c_int, [C.c_void_p, U64P, U64P]),
+    "onion_gpu_tap_copy": (C.c_int, [C.c_void_p, C.c_uint64, U64P]),
 }
 
 

@@ -53,4 +53,11 @@ __global__ void k_pack_response(const uint64_t *ct, uint8_t *out, uint32_t n_coe
   }
 }
 
+// Plain streaming copy of n 16-byte vectors (calibrates the HBM byte counters).
+__global__ void k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 }  // namespace onion

@@ -581,6 +581,19 @@ int onion_gpu_tap_galois(onion_server *s, uint64_t id, uint32_t level, const uin
   return ONION_OK;
 }
 
+int onion_gpu_tap_copy(onion_server *s, uint64_t n_bytes, uint64_t *copied) {
+  if (!s) return fail(ONION_EINVAL, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  uint64_t n = std::min<uint64_t>({n_bytes, s->digit_words * 8, s->I.num_pt * kN * 8}) / 16;
+  if (n) {
+    k_copy<<<1024, 256, 0, s->st>>>(reinterpret_cast<const uint4 *>(s->db), reinterpret_cast<uint4 *>(s->digits), n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->st));
+  }
+  if (copied) *copied = n * 16;
+  return ONION_OK;
+}
+
 int onion_gpu_tap_mod_switch(onion_server *s, const uint64_t *ct, uint64_t *out) {
   if (!s || !ct || !out) return fail(ONION_EINVAL, "null argument");
   std::lock_guard<std::mutex> lk(s->mu);

@@ -156,6 +156,11 @@ class GpuServer:
         check(self._L.onion_gpu_tap_galois(self._h, client_id, level, u64ptr(c), u64ptr(out)))
         return out
 
+    def tap_copy(self, n_bytes: int) -> int:
+        out = C.c_uint64()
+        check(self._L.onion_gpu_tap_copy(self._h, n_bytes, C.byref(out)))
+        return out.value
+
     def tap_mod_switch(self, ct) -> np.ndarray:
         c = _u64(ct).reshape(-1)
         out = np.zeros(2 * self.N, np.uint64)
