@@ -43,6 +43,18 @@ def pct(xs, p):
     return float(np.percentile(np.asarray(xs), p))
 
 
+def measured_traffic(kernel: str, fst: int, other: int):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of this kernel and shape (profiles/), or None if there is none."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{fst}x{other}_hbm_bytes.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split("=")[1]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +143,7 @@ def main() -> int:
     core_ms = float(np.mean(cores))
     alg_bytes = db_bytes + N * fst * 16 + other * 2 * N * 8
     achieved = alg_bytes / (core_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic("k_first_dim_b1", fst, other)
 
     out = {
         "metric": METRIC,
@@ -160,7 +173,8 @@ def main() -> int:
         "bit_exact": bit_exact,
         "roofline": {"kernel": "k_first_dim_b1", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
-                     "traffic": None, "avg_ms": core_ms, "algorithmic_bytes": alg_bytes},
+                     "traffic": traffic, "traffic_source": traffic_src, "avg_ms": core_ms,
+                     "algorithmic_bytes": alg_bytes},
         "cpu_baseline": None,
     }
 
