@@ -19,16 +19,13 @@ __device__ __forceinline__ uint64_t plain_lift(uint64_t v, const LiftConst &L) {
   return v >= L.thr ? v + L.inc : v;
 }
 
-// NTT of the lifted plaintext in LDS, then db[slot * num_pt + pt] for all slots.
-__device__ __forceinline__ void ntt_and_scatter(uint64_t *s, uint64_t *db, uint64_t pt,
-                                                uint64_t num_pt, const NttTab &T) {
-  __syncthreads();
-  ntt_fwd_lds<kN>(s, T);
+// NTT of the lifted plaintext (natural layout in x), then
+// db[slot * num_pt + pt] for every slot.
+__device__ __forceinline__ void ntt_and_scatter(uint64_t (&x)[8], uint64_t *lds, uint64_t *db,
+                                                uint64_t pt, uint64_t num_pt, const NttTab &T) {
+  ntt_fwd(x, lds, T);
 #pragma unroll
-  for (int r = 0; r < kPer; ++r) {
-    const int e = threadIdx.x + r * kNttThreads;
-    db[(size_t)e * num_pt + pt] = ntt_fwd_final(s[e], T.q);
-  }
+  for (int r = 0; r < 8; ++r) db[(size_t)(8 * threadIdx.x + r) * num_pt + pt] = ntt_fwd_final(x[r], T.q);
 }
 
 // Seeded generation, one workgroup per plaintext pt_lo + blockIdx.x.
@@ -38,11 +35,12 @@ __device__ __forceinline__ void ntt_and_scatter(uint64_t *s, uint64_t *db, uint6
 __global__ __launch_bounds__(kNttThreads) void k_db_generate(uint64_t *db, uint64_t pt_lo,
                                                              uint64_t num_pt, uint64_t key,
                                                              int mode, LiftConst L, NttTab T) {
-  __shared__ uint64_t s[kN];
+  __shared__ uint64_t lds[kNttLds];
   const uint64_t pt = pt_lo + blockIdx.x;
+  uint64_t x[8];
 #pragma unroll
-  for (int r = 0; r < kPer; ++r) {
-    const int e = threadIdx.x + r * kNttThreads;
+  for (int r = 0; r < 8; ++r) {
+    const int e = threadIdx.x + 256 * r;
     uint64_t v;
     if (mode == 0) {
       v = mulhi64(rand64_keyed(key, pt * kN + e), L.t);
@@ -56,24 +54,22 @@ __global__ __launch_bounds__(kNttThreads) void k_db_generate(uint64_t *db, uint6
       }
       v = bytes[0] | (bytes[1] << 8);
     }
-    s[e] = plain_lift(v, L);
+    x[r] = plain_lift(v, L);
   }
-  ntt_and_scatter(s, db, pt, num_pt, T);
+  ntt_and_scatter(x, lds, db, pt, num_pt, T);
 }
 
 // Uploaded plaintexts: coeffs[i*N + e] (< t) for pt = pt_lo + i.
 __global__ __launch_bounds__(kNttThreads) void k_db_plain(const uint64_t *coeffs, uint64_t *db,
                                                           uint64_t pt_lo, uint64_t num_pt,
                                                           LiftConst L, NttTab T) {
-  __shared__ uint64_t s[kN];
+  __shared__ uint64_t lds[kNttLds];
   const uint64_t pt = pt_lo + blockIdx.x;
-  const uint64_t *src = coeffs + (size_t)blockIdx.x * kN;
+  uint64_t x[8];
+  load_natural(x, coeffs + (size_t)blockIdx.x * kN);
 #pragma unroll
-  for (int r = 0; r < kPer; ++r) {
-    const int e = threadIdx.x + r * kNttThreads;
-    s[e] = plain_lift(src[e], L);
-  }
-  ntt_and_scatter(s, db, pt, num_pt, T);
+  for (int r = 0; r < 8; ++r) x[r] = plain_lift(x[r], L);
+  ntt_and_scatter(x, lds, db, pt, num_pt, T);
 }
 
 }  // namespace onion

@@ -8,6 +8,7 @@
 //          switching of sigma(c1) with the special prime p (NTT mod q and mod
 //          p, product with the level's key, INTT, rounding division by p)
 //   c_2i = c_i + c',   c_2i+1 = (c_i - c') * x^-k.
+// Batched: blockIdx.y selects the query; each query has its own tree and key.
 #pragma once
 
 #include "k_ntt.hpp"
@@ -33,38 +34,29 @@ __device__ __forceinline__ uint64_t galois_gather(const uint64_t *in, int j, uin
   return v ? q - v : 0;
 }
 
-// tree:  2w cts, slot s at tree + s*2N (c0 then c1), coefficient form
-// key:   this level's key [comp][limb q,p][N], NTT form
-__global__ __launch_bounds__(kNttThreads) void k_expand_level(uint64_t *tree, const uint64_t *key,
+// trees: query b's tree at trees + b * tree_stride (2w cts, slot s at s*2N)
+// keys:  query b's Galois keys at key_ptrs[b]; this level's key at + level*4N,
+//        laid out [comp][limb q,p][N], NTT form
+__global__ __launch_bounds__(kNttThreads) void k_expand_level(uint64_t *trees, uint64_t tree_stride,
+                                                              const uint64_t *const *key_ptrs,
                                                               uint32_t level, uint32_t ginv,
                                                               NttTab Tq, NttTab Tp, Mod mq, Mod mp,
                                                               KsConst ks) {
-  __shared__ uint64_t A[kN], B[kN];
+  __shared__ uint64_t lds[kNttLds];
   const int tid = threadIdx.x;
   const uint32_t k = 1u << level;
   const uint64_t node = (uint64_t)k + blockIdx.x;
   const uint64_t q = Tq.q, p = Tp.q;
+  uint64_t *tree = trees + (size_t)blockIdx.y * tree_stride;
+  const uint64_t *key = key_ptrs[blockIdx.y] + (size_t)level * 4 * kN;
   const uint64_t *c0 = tree + node * 2 * kN, *c1 = c0 + kN;
 
-  // sigma(c1) into both buffers; q < p so the same residues serve both limbs
+  // sigma(c1), NTT mod q and mod p (q < p: the same residues serve both)
+  uint64_t tq[8], tp[8];
 #pragma unroll
-  for (int r = 0; r < kPer; ++r) {
-    const int e = tid + r * kNttThreads;
-    const uint64_t v = galois_gather(c1, e, ginv, q);
-    A[e] = v;
-    B[e] = v;
-  }
-  __syncthreads();
-  ntt_fwd_lds<kN>(A, Tq);
-  ntt_fwd_lds<kN>(B, Tp);
-  uint64_t tq[kPer], tp[kPer];
-#pragma unroll
-  for (int r = 0; r < kPer; ++r) {
-    const int e = tid + r * kNttThreads;
-    tq[r] = ntt_fwd_final(A[e], q);
-    tp[r] = ntt_fwd_final(B[e], p);
-  }
-  __syncthreads();
+  for (int j = 0; j < 8; ++j) tq[j] = tp[j] = galois_gather(c1, tid + 256 * j, ginv, q);
+  ntt_fwd(tq, lds, Tq);
+  ntt_fwd(tp, lds, Tp);
 
   // Component comp of c' is sigma(c0) + KS_0 (comp 0) or KS_1 (comp 1); the
   // children's polynomial comp is written as soon as it is known.
@@ -73,21 +65,21 @@ __global__ __launch_bounds__(kNttThreads) void k_expand_level(uint64_t *tree, co
 #pragma unroll 1
   for (int comp = 0; comp < 2; ++comp) {
     const uint64_t *kq = key + (size_t)(2 * comp) * kN, *kp = kq + kN;
+    uint64_t aq[8], ap[8], kv[8];
+    load_blocked(kv, kq);
 #pragma unroll
-    for (int r = 0; r < kPer; ++r) {
-      const int e = tid + r * kNttThreads;
-      A[e] = mulmod_barrett(tq[r], kq[e], mq);
-      B[e] = mulmod_barrett(tp[r], kp[e], mp);
-    }
-    __syncthreads();
-    ntt_inv_lds<kN>(A, Tq);
-    ntt_inv_lds<kN>(B, Tp);
+    for (int j = 0; j < 8; ++j) aq[j] = mulmod_barrett(tq[j], kv[j], mq);
+    load_blocked(kv, kp);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) ap[j] = mulmod_barrett(tp[j], kv[j], mp);
+    ntt_inv(aq, lds, Tq);
+    ntt_inv(ap, lds, Tp);
     const uint64_t *cin = comp == 0 ? c0 : c1;
-#pragma unroll 2
-    for (int r = 0; r < kPer; ++r) {
-      const int e = tid + r * kNttThreads;
-      const uint64_t xq = ntt_inv_final(A[e], Tq);
-      const uint64_t xp = ntt_inv_final(B[e], Tp);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int e = tid + 256 * j;
+      const uint64_t xq = ntt_inv_final(aq[j], Tq);
+      const uint64_t xp = ntt_inv_final(ap[j], Tp);
       uint64_t last = xp + ks.half_p;  // (X_p + floor(p/2)) mod p
       if (last >= p) last -= p;
       const uint64_t lq = last >= q ? last - q : last;  // p < 2q
@@ -106,7 +98,6 @@ __global__ __launch_bounds__(kNttThreads) void k_expand_level(uint64_t *tree, co
       }
       right[(size_t)comp * kN + dst] = d;
     }
-    __syncthreads();
   }
 }
 

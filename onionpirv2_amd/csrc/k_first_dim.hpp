@@ -16,9 +16,9 @@
 //
 // Layouts
 //   DB   A[l * num_pt + r * fst + j]            (the reference's db_aligned_)
-//   Qs   [l][j] x {b0_lo, b0_hi, b1_lo, b1_hi}  (uint32 limbs of the two query
-//        polys at slot l; written by k_sel_ntt)
-//   out  [r][c][l]                              (NTT-form ciphertexts)
+//   Qs   [l][j][col][2] uint32 limbs, col = 2*query + poly, padded to `cpad`
+//        columns (written by k_sel_ntt)
+//   out  per query: [r][c][l]                   (NTT-form ciphertexts)
 #pragma once
 
 #include "k_ntt.hpp"
@@ -32,25 +32,24 @@ __device__ __forceinline__ uint64_t mad32(uint32_t a, uint32_t b, uint64_t c) {
   return (uint64_t)a * b + c;
 }
 
-// Selection vector: NTT of poly `c` of leaf j (grid = fst x 2) scattered into
-// the slot-major limb layout Qs.
-__global__ __launch_bounds__(kNttThreads) void k_sel_ntt(const uint64_t *leaves, uint32_t *Qs,
-                                                         uint32_t fst, NttTab T) {
-  __shared__ uint64_t s[kN];
-  const uint32_t j = blockIdx.x / 2, c = blockIdx.x % 2;
-  const uint64_t *src = leaves + (size_t)j * 2 * kN + (size_t)c * kN;
+// Selection vectors: NTT of poly c of leaf j of query b (grid = fst*2 x B),
+// scattered as limbs into Qs[l][j][2b + c].
+__global__ __launch_bounds__(kNttThreads) void k_sel_ntt(const uint64_t *leaves, uint64_t leaf_stride,
+                                                         uint32_t *Qs, uint32_t fst, uint32_t cpad,
+                                                         NttTab T) {
+  __shared__ uint64_t lds[kNttLds];
+  const uint32_t j = blockIdx.x / 2, c = blockIdx.x % 2, b = blockIdx.y;
+  uint64_t x[8];
+  load_natural(x, leaves + (size_t)b * leaf_stride + (size_t)j * 2 * kN + (size_t)c * kN);
+  ntt_fwd(x, lds, T);
 #pragma unroll
-  for (int r = 0; r < kPer; ++r) s[threadIdx.x + r * kNttThreads] = src[threadIdx.x + r * kNttThreads];
-  __syncthreads();
-  ntt_fwd_lds<kN>(s, T);
-#pragma unroll
-  for (int r = 0; r < kPer; ++r) {
-    const int e = threadIdx.x + r * kNttThreads;
-    const uint64_t v = ntt_fwd_final(s[e], T.q);
+  for (int r = 0; r < 8; ++r) {
+    const int e = 8 * threadIdx.x + r;
+    const uint64_t v = ntt_fwd_final(x[r], T.q);
     uint2 lim;
     lim.x = (uint32_t)v & kMask30;
     lim.y = (uint32_t)(v >> 30);
-    *reinterpret_cast<uint2 *>(Qs + ((size_t)e * fst + j) * 4 + 2 * c) = lim;
+    *reinterpret_cast<uint2 *>(Qs + (((size_t)e * fst + j) * cpad + 2 * b + c) * 2) = lim;
   }
 }
 
@@ -78,10 +77,10 @@ __device__ __forceinline__ void split30(uint64_t a, uint32_t &a0, uint32_t &a1) 
   a1 = (uint32_t)(a >> 30);
 }
 
-// Single query (B = 1).  grid.x = slot, grid.y = row chunk of `rows_per_block`
-// rows; 256 threads.  Lane layout per wave: 8 rows x 8 column groups; a lane
-// reads 16 B (two adjacent words) per step, so 8 lanes cover 128 contiguous
-// bytes of one row.  Requires fst % 16 == 0.
+// Single query (B = 1, Qs with cpad = 2).  grid.x = slot, grid.y = row chunk
+// of `rows_per_block` rows; 256 threads.  Lane layout per wave: 8 rows x 8
+// column groups; a lane reads 16 B (two adjacent words) per step, so 8 lanes
+// cover 128 contiguous bytes of one row.  Requires fst % 16 == 0.
 __global__ __launch_bounds__(256) void k_first_dim_b1(const uint64_t *__restrict__ db,
                                                       const uint32_t *__restrict__ Qs,
                                                       uint64_t *__restrict__ out, uint32_t fst,
@@ -105,8 +104,7 @@ __global__ __launch_bounds__(256) void k_first_dim_b1(const uint64_t *__restrict
     const uint4 *row = reinterpret_cast<const uint4 *>(A + (size_t)(active ? r : r_begin) * fst);
     Acc3 x0 = {0, 0, 0}, x1 = {0, 0, 0};
     uint64_t w0lo = 0, w0hi = 0, w1lo = 0, w1hi = 0;
-    uint32_t k = 0;
-    for (; k < steps; k += 4) {
+    for (uint32_t k = 0; k < steps; k += 4) {
       uint4 v[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) v[u] = (k + u < steps) ? row[(k + u) * 8 + cg] : make_uint4(0, 0, 0, 0);
@@ -140,8 +138,7 @@ __global__ __launch_bounds__(256) void k_first_dim_b1(const uint64_t *__restrict
   }
 }
 
-// Generic shape (any fst): one lane per row, scalar loads.  Used for shapes
-// the streaming kernel does not cover.
+// Generic shape (any fst), single query: one lane per row, scalar loads.
 __global__ __launch_bounds__(256) void k_first_dim_generic(const uint64_t *__restrict__ db,
                                                            const uint32_t *__restrict__ Qs,
                                                            uint64_t *__restrict__ out, uint32_t fst,
@@ -167,6 +164,91 @@ __global__ __launch_bounds__(256) void k_first_dim_generic(const uint64_t *__res
     fold(w1lo, w1hi, x1);
     out[((size_t)r * 2 + 0) * kN + slot] = barrett128(w0lo, w0hi, mq);
     out[((size_t)r * 2 + 1) * kN + slot] = barrett128(w1lo, w1hi, mq);
+  }
+}
+
+// Batched: C_l (other x cpad) = A_l (other x fst) * Q_l (fst x cpad) for all
+// columns (2 per query) at once, so each database word leaves HBM once per
+// batch.  Tiles: a block owns one slot and RBLK rows; per 32-wide k chunk the
+// A tile (RBLK x 32, split into limbs) and the Q tile (32 x cpad) are staged in
+// LDS; each thread accumulates TR rows x TC columns (rows rg + i*RG, columns
+// cg + k*CG, so LDS reads are conflict-free or broadcast).
+// Block order is XCD-aware: consecutive blocks of one XCD (block id mod 8)
+// walk the row tiles of one slot, so that slot's Q (fst x cpad words) stays in
+// that XCD's L2 while its tiles run.
+template <int TR, int TC, int CPAD>
+__global__ __launch_bounds__(256) void k_first_dim_bat(const uint64_t *__restrict__ db,
+                                                       const uint32_t *__restrict__ Qs,
+                                                       uint64_t *__restrict__ fd, uint64_t fd_stride,
+                                                       uint32_t fst, uint32_t other, uint64_t num_pt,
+                                                       uint32_t nq, uint32_t n_tiles, Mod mq) {
+  constexpr int CG = CPAD / TC;   // thread columns
+  constexpr int RG = 256 / CG;    // thread rows
+  constexpr int RBLK = RG * TR;   // rows per block
+  constexpr int KC = 32;          // k chunk
+  static_assert(CG * RG == 256, "thread grid must be 256");
+  __shared__ uint2 As[RBLK][KC + 1];
+  __shared__ uint2 Qt[KC][CPAD];
+
+  const uint32_t L = blockIdx.x, xcd = L & 7, seq = L >> 3;
+  const uint32_t slot = (seq / n_tiles) * 8 + xcd;
+  const uint32_t tile = seq % n_tiles;
+  const int tid = threadIdx.x, cg = tid % CG, rg = tid / CG;
+  const uint32_t row0 = tile * RBLK;
+  const uint64_t *A = db + (size_t)slot * num_pt;
+  const uint2 *Q = reinterpret_cast<const uint2 *>(Qs) + (size_t)slot * fst * CPAD;
+
+  Acc3 acc[TR][TC];
+  uint64_t wlo[TR][TC], whi[TR][TC];
+#pragma unroll
+  for (int i = 0; i < TR; ++i)
+#pragma unroll
+    for (int k = 0; k < TC; ++k) {
+      acc[i][k] = Acc3{0, 0, 0};
+      wlo[i][k] = whi[i][k] = 0;
+    }
+
+  for (uint32_t k0 = 0; k0 < fst; k0 += KC) {
+    __syncthreads();
+    for (int idx = tid; idx < RBLK * KC; idx += 256) {
+      const int rr = idx / KC, kk = idx % KC;
+      const uint32_t r = row0 + rr, j = k0 + kk;
+      const uint64_t v = (r < other && j < fst) ? A[(size_t)r * fst + j] : 0;
+      As[rr][kk] = make_uint2((uint32_t)v & kMask30, (uint32_t)(v >> 30));
+    }
+    for (int idx = tid; idx < KC * CPAD; idx += 256) {
+      const int kk = idx / CPAD, col = idx % CPAD;
+      Qt[kk][col] = (k0 + kk < fst) ? Q[(size_t)(k0 + kk) * CPAD + col] : make_uint2(0, 0);
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int kk = 0; kk < KC; ++kk) {
+      uint2 a[TR], b[TC];
+#pragma unroll
+      for (int i = 0; i < TR; ++i) a[i] = As[rg + i * RG][kk];
+#pragma unroll
+      for (int k = 0; k < TC; ++k) b[k] = Qt[kk][cg + k * CG];
+#pragma unroll
+      for (int i = 0; i < TR; ++i)
+#pragma unroll
+        for (int k = 0; k < TC; ++k) acc_word(acc[i][k], a[i].x, a[i].y, b[k].x, b[k].y);
+      if ((kk & 7) == 7) {
+#pragma unroll
+        for (int i = 0; i < TR; ++i)
+#pragma unroll
+          for (int k = 0; k < TC; ++k) fold(wlo[i][k], whi[i][k], acc[i][k]);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < TR; ++i) {
+    const uint32_t r = row0 + rg + i * RG;
+#pragma unroll
+    for (int k = 0; k < TC; ++k) {
+      const uint32_t col = cg + k * CG, b = col >> 1, c = col & 1;
+      if (r < other && b < nq)
+        fd[(size_t)b * fd_stride + ((size_t)r * 2 + c) * kN + slot] = barrett128(wlo[i][k], whi[i][k], mq);
+    }
   }
 }
 

@@ -11,9 +11,11 @@ namespace onion {
 // (M < 2^53); the product x*M is rounded to a 64-bit significand (round half
 // to even, the x87 default), then rounded half away from zero at bit f.
 // Integer-only, so every lane reproduces the host's long double result.
-__global__ void k_mod_switch(uint64_t *ct, uint32_t n, uint64_t M, uint32_t f) {
+// grid.y = query: ct + query * stride holds the 2N values.
+__global__ void k_mod_switch(uint64_t *cts, uint64_t stride, uint32_t n, uint64_t M, uint32_t f) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  uint64_t *ct = cts + (size_t)blockIdx.y * stride;
   const uint64_t x = ct[i];
   if (x == 0) return;
   unsigned __int128 P = (unsigned __int128)x * M;

@@ -1,15 +1,24 @@
-// ntt.hpp -- negacyclic NTT / INTT of one N-point polynomial held in LDS,
-// executed cooperatively by one workgroup.
+// ntt.hpp -- negacyclic NTT / INTT of one 2048-point polynomial, computed by
+// one workgroup of 256 threads with 8 coefficients per thread in registers.
 //
 // Forward: Cooley-Tukey with bit-reversed twiddles psi^bitrev(k), Harvey lazy
-// butterflies (values in [0, 4q)); output slot j = a(psi^(2*bitrev(j)+1)),
+// butterflies (values in [0, 4q)); output slot e = a(psi^(2*bitrev(e)+1)),
 // the order SEAL's ntt_negacyclic_harvey produces.  Inverse: Gentleman-Sande
-// with psi^-bitrev(k); values stay in [0, 2q); the N^-1 scale is applied by
-// the caller when it reads the result out (ntt_inv_final).
+// with psi^-bitrev(k), values in [0, 2q); the N^-1 scale is applied by the
+// caller as it reads the result out (ntt_inv_final).
 //
-// Stages whose butterfly span is >= 64 elements go through LDS with one
-// barrier per stage; the last six stages (span 32..1) run inside a wavefront
-// on registers, exchanging partners with ds_swizzle/permute (__shfl_xor).
+// The 11 radix-2 stages run as four register passes (3 + 3 + 3 + 2 stages).
+// In a pass with top span H a thread owns the 8 coefficients b + j*H/4,
+// j = 0..7, which form one radix-8 butterfly: stage H pairs (j, j+4) under
+// twiddle index m, stage H/2 pairs (j, j+2) under 2m + j/4, stage H/4 pairs
+// (j, j+1) under 4m + j/2.  Between passes the coefficients are exchanged
+// through LDS (index padded by i/16 against bank conflicts); each pass loads
+// its 7 twiddles (and Shoup companions) at once.
+//
+// Register layouts (t = threadIdx.x):
+//   "natural"   x[j] = coefficient t + 256*j   (coalesced global access)
+//   "blocked"   x[j] = slot 8*t + j            (64 contiguous bytes per thread)
+// ntt_fwd: natural -> blocked.  ntt_inv: blocked -> natural.
 #pragma once
 
 #include "arith.hpp"
@@ -17,20 +26,22 @@
 namespace onion {
 
 constexpr int kNttThreads = 256;
+constexpr int kNttLds = 2048 + 2048 / 16;  // padded LDS words per transform
 
-// Butterfly helpers --------------------------------------------------------------
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
+
 __device__ __forceinline__ void ct_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t ws,
                                         uint64_t q, uint64_t two_q) {
   uint64_t u = x;
   if (u >= two_q) u -= two_q;
-  uint64_t v = mul_shoup_lazy(y, w, ws, q);
+  const uint64_t v = mul_shoup_lazy(y, w, ws, q);
   x = u + v;
   y = u + two_q - v;
 }
 
 __device__ __forceinline__ void gs_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t ws,
                                         uint64_t q, uint64_t two_q) {
-  uint64_t u = x, v = y;
+  const uint64_t u = x, v = y;
   uint64_t s = u + v;
   if (s >= two_q) s -= two_q;
   x = s;
@@ -38,98 +49,141 @@ __device__ __forceinline__ void gs_bfly(uint64_t &x, uint64_t &y, uint64_t w, ui
 }
 
 __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int mask) {
-  int lo = __shfl_xor((int)(uint32_t)v, mask, 64);
-  int hi = __shfl_xor((int)(uint32_t)(v >> 32), mask, 64);
+  const int lo = __shfl_xor((int)(uint32_t)v, mask, 64);
+  const int hi = __shfl_xor((int)(uint32_t)(v >> 32), mask, 64);
   return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 
-// Forward NTT of s[0..N) in LDS.  Entry: values < 4q, visible to the whole
-// workgroup (caller synchronises).  Exit: values in [0, 4q) in LDS, synchronised.
-template <int N>
-__device__ __forceinline__ void ntt_fwd_lds(uint64_t *s, const NttTab &T) {
-  static_assert(N >= 128 && (N & (N - 1)) == 0, "N must be a power of two >= 128");
-  const uint64_t q = T.q, two_q = 2 * q;
-  const int tid = threadIdx.x;
-  // LDS stages: half = N/2 .. 64
-  int groups = 1;
-  for (int half = N / 2; half >= 64; half >>= 1, groups <<= 1) {
+// Twiddles of one radix-8 pass rooted at index m: w[0] = tab[m],
+// w[1..2] = tab[2m..2m+1], w[3..6] = tab[4m..4m+3].
+struct Tw8 {
+  uint64_t w[7], s[7];
+};
+
+__device__ __forceinline__ void load_tw8(Tw8 &t, const uint64_t *tab, const uint64_t *tabs, int m) {
+  t.w[0] = tab[m];
+  t.s[0] = tabs[m];
 #pragma unroll
-    for (int r = 0; r < N / 2 / kNttThreads; ++r) {
-      const int b = tid + r * kNttThreads;
-      const int i = b / half, j = b % half;
-      const int x = 2 * i * half + j;
-      ct_bfly(s[x], s[x + half], T.rp[groups + i], T.rps[groups + i], q, two_q);
-    }
-    __syncthreads();
+  for (int i = 0; i < 2; ++i) {
+    t.w[1 + i] = tab[2 * m + i];
+    t.s[1 + i] = tabs[2 * m + i];
   }
-  // Register stages: each wave owns contiguous blocks of 64 elements
-  // (block = 64 consecutive slots, one per lane); stages half = 32 .. 1.
-  const int lane = tid & 63, wave = tid >> 6;
-  constexpr int kBlocks = N / 64;
-  constexpr int kWaves = kNttThreads / 64;
 #pragma unroll
-  for (int r = 0; r < kBlocks / kWaves; ++r) {
-    const int blk = wave + r * kWaves;
-    uint64_t v = s[blk * 64 + lane];
-    int g = groups * (blk * 64) / 64;  // first group index of this block at half=32
-    // at stage `half`, the group covering this lane is g0 + lane/(2*half)
-#pragma unroll
-    for (int half = 32, gmul = 1; half >= 1; half >>= 1, gmul <<= 1) {
-      const int grp = (groups * gmul) + (blk * 64 + lane) / (2 * half);
-      const uint64_t w = T.rp[grp], ws = T.rps[grp];
-      const uint64_t other = shfl_xor64(v, half);
-      const bool upper = (lane & half) != 0;
-      // lower lane holds x, upper lane holds y
-      uint64_t x = upper ? other : v, y = upper ? v : other;
-      ct_bfly(x, y, w, ws, q, two_q);
-      v = upper ? y : x;
-      (void)g;
-    }
-    s[blk * 64 + lane] = v;
+  for (int i = 0; i < 4; ++i) {
+    t.w[3 + i] = tab[4 * m + i];
+    t.s[3 + i] = tabs[4 * m + i];
   }
-  __syncthreads();
 }
 
-// Inverse NTT (without the N^-1 scale) of s[0..N) in LDS.  Entry: values in
-// [0, 2q), synchronised.  Exit: values in [0, 2q), synchronised.
-template <int N>
-__device__ __forceinline__ void ntt_inv_lds(uint64_t *s, const NttTab &T) {
-  const uint64_t q = T.q, two_q = 2 * q;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  constexpr int kBlocks = N / 64;
-  constexpr int kWaves = kNttThreads / 64;
-  // Register stages first: half = 1 .. 32 (span = 2 .. 64)
+// forward radix-8 (three CT stages); `first` = false skips the span-H stage
+__device__ __forceinline__ void fwd8(uint64_t (&x)[8], const Tw8 &t, uint64_t q, bool first) {
+  const uint64_t two_q = 2 * q;
+  if (first) {
 #pragma unroll
-  for (int r = 0; r < kBlocks / kWaves; ++r) {
-    const int blk = wave + r * kWaves;
-    uint64_t v = s[blk * 64 + lane];
-#pragma unroll
-    for (int half = 1; half <= 32; half <<= 1) {
-      const int groups = N / (2 * half);
-      const int grp = groups + (blk * 64 + lane) / (2 * half);
-      const uint64_t w = T.irp[grp], ws = T.irps[grp];
-      const uint64_t other = shfl_xor64(v, half);
-      const bool upper = (lane & half) != 0;
-      uint64_t x = upper ? other : v, y = upper ? v : other;
-      gs_bfly(x, y, w, ws, q, two_q);
-      v = upper ? y : x;
-    }
-    s[blk * 64 + lane] = v;
+    for (int j = 0; j < 4; ++j) ct_bfly(x[j], x[j + 4], t.w[0], t.s[0], q, two_q);
   }
+#pragma unroll
+  for (int j = 0; j < 8; j = (j & 1) ? j + 3 : j + 1)  // j = 0, 1, 4, 5
+    ct_bfly(x[j], x[j + 2], t.w[1 + (j >> 2)], t.s[1 + (j >> 2)], q, two_q);
+#pragma unroll
+  for (int j = 0; j < 8; j += 2) ct_bfly(x[j], x[j + 1], t.w[3 + (j >> 1)], t.s[3 + (j >> 1)], q, two_q);
+}
+
+// inverse radix-8 (three GS stages, reverse order); `last` = false skips the span-H stage
+__device__ __forceinline__ void inv8(uint64_t (&x)[8], const Tw8 &t, uint64_t q, bool last) {
+  const uint64_t two_q = 2 * q;
+#pragma unroll
+  for (int j = 0; j < 8; j += 2) gs_bfly(x[j], x[j + 1], t.w[3 + (j >> 1)], t.s[3 + (j >> 1)], q, two_q);
+#pragma unroll
+  for (int j = 0; j < 8; j = (j & 1) ? j + 3 : j + 1)
+    gs_bfly(x[j], x[j + 2], t.w[1 + (j >> 2)], t.s[1 + (j >> 2)], q, two_q);
+  if (last) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) gs_bfly(x[j], x[j + 4], t.w[0], t.s[0], q, two_q);
+  }
+}
+
+// base coefficient of thread t in the pass whose element stride is 2^s0
+__device__ __forceinline__ int pass_base(int t, int s0) {
+  return (t & ((1 << s0) - 1)) | ((t >> s0) << (s0 + 3));
+}
+
+__device__ __forceinline__ void lds_put(uint64_t *lds, const uint64_t (&x)[8], int b, int stride) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) lds[lds_pad(b + j * stride)] = x[j];
+}
+__device__ __forceinline__ void lds_get(const uint64_t *lds, uint64_t (&x)[8], int b, int stride) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) x[j] = lds[lds_pad(b + j * stride)];
+}
+
+// Forward NTT.  Entry: natural layout, values < 4q.  Exit: blocked layout,
+// values in [0, 4q).  Uses lds[kNttLds]; begins with a barrier so the buffer
+// may have been read just before the call.
+__device__ __forceinline__ void ntt_fwd(uint64_t (&x)[8], uint64_t *lds, const NttTab &T) {
+  const int t = threadIdx.x;
+  const uint64_t q = T.q;
+  Tw8 tw;
+  // pass A: spans 1024, 512, 256 (stride 256, b = t): twiddle root m = 1
+  load_tw8(tw, T.rp, T.rps, 1);
+  fwd8(x, tw, q, true);
+  // pass B: spans 128, 64, 32 (stride 32): m = 8 + t/32
+  const int bB = pass_base(t, 5);
+  load_tw8(tw, T.rp, T.rps, 8 + (t >> 5));
   __syncthreads();
-  // LDS stages: half = 64 .. N/2
-  for (int half = 64; half <= N / 2; half <<= 1) {
-    const int groups = N / (2 * half);
-#pragma unroll
-    for (int r = 0; r < N / 2 / kNttThreads; ++r) {
-      const int b = tid + r * kNttThreads;
-      const int i = b / half, j = b % half;
-      const int x = 2 * i * half + j;
-      gs_bfly(s[x], s[x + half], T.irp[groups + i], T.irps[groups + i], q, two_q);
-    }
-    __syncthreads();
-  }
+  lds_put(lds, x, t, 256);
+  __syncthreads();
+  lds_get(lds, x, bB, 32);
+  fwd8(x, tw, q, true);
+  // pass C: spans 16, 8, 4 (stride 4): m = 64 + t/4
+  const int bC = pass_base(t, 2);
+  load_tw8(tw, T.rp, T.rps, 64 + (t >> 2));
+  __syncthreads();
+  lds_put(lds, x, bB, 32);
+  __syncthreads();
+  lds_get(lds, x, bC, 4);
+  fwd8(x, tw, q, true);
+  // pass D: spans 2, 1 on slots 8t .. 8t+7: m = 256 + t
+  load_tw8(tw, T.rp, T.rps, 256 + t);
+  __syncthreads();
+  lds_put(lds, x, bC, 4);
+  __syncthreads();
+  lds_get(lds, x, 8 * t, 1);
+  fwd8(x, tw, q, false);
+}
+
+// Inverse NTT without the N^-1 scale.  Entry: blocked layout, values < 2q.
+// Exit: natural layout, values in [0, 2q).  Begins with a barrier.
+__device__ __forceinline__ void ntt_inv(uint64_t (&x)[8], uint64_t *lds, const NttTab &T) {
+  const int t = threadIdx.x;
+  const uint64_t q = T.q;
+  Tw8 tw;
+  // pass D: spans 1, 2 on slots 8t .. 8t+7
+  load_tw8(tw, T.irp, T.irps, 256 + t);
+  inv8(x, tw, q, false);
+  // pass C: spans 4, 8, 16 (stride 4)
+  const int bC = pass_base(t, 2);
+  load_tw8(tw, T.irp, T.irps, 64 + (t >> 2));
+  __syncthreads();
+  lds_put(lds, x, 8 * t, 1);
+  __syncthreads();
+  lds_get(lds, x, bC, 4);
+  inv8(x, tw, q, true);
+  // pass B: spans 32, 64, 128 (stride 32)
+  const int bB = pass_base(t, 5);
+  load_tw8(tw, T.irp, T.irps, 8 + (t >> 5));
+  __syncthreads();
+  lds_put(lds, x, bC, 4);
+  __syncthreads();
+  lds_get(lds, x, bB, 32);
+  inv8(x, tw, q, true);
+  // pass A: spans 256, 512, 1024 (stride 256)
+  load_tw8(tw, T.irp, T.irps, 1);
+  __syncthreads();
+  lds_put(lds, x, bB, 32);
+  __syncthreads();
+  lds_get(lds, x, t, 256);
+  inv8(x, tw, q, true);
 }
 
 // Canonical value of a forward-NTT output in [0, 4q).
@@ -142,8 +196,30 @@ __device__ __forceinline__ uint64_t ntt_fwd_final(uint64_t v, uint64_t q) {
 
 // Canonical value of an inverse-NTT output in [0, 2q), scaled by N^-1.
 __device__ __forceinline__ uint64_t ntt_inv_final(uint64_t v, const NttTab &T) {
-  uint64_t r = mul_shoup_lazy(v, T.n_inv, T.n_inv_s, T.q);
+  const uint64_t r = mul_shoup_lazy(v, T.n_inv, T.n_inv_s, T.q);
   return r >= T.q ? r - T.q : r;
+}
+
+// natural-layout global load / blocked-layout global store helpers
+__device__ __forceinline__ void load_natural(uint64_t (&x)[8], const uint64_t *src) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) x[j] = src[threadIdx.x + 256 * j];
+}
+__device__ __forceinline__ void load_blocked(uint64_t (&x)[8], const uint64_t *src) {
+  const uint4 *p = reinterpret_cast<const uint4 *>(src + 8 * threadIdx.x);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint4 v = p[i];
+    x[2 * i] = ((uint64_t)v.y << 32) | v.x;
+    x[2 * i + 1] = ((uint64_t)v.w << 32) | v.z;
+  }
+}
+__device__ __forceinline__ void store_blocked(uint64_t *dst, const uint64_t (&x)[8]) {
+  uint4 *p = reinterpret_cast<uint4 *>(dst + 8 * threadIdx.x);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    p[i] = make_uint4((uint32_t)x[2 * i], (uint32_t)(x[2 * i] >> 32), (uint32_t)x[2 * i + 1],
+                      (uint32_t)(x[2 * i + 1] >> 32));
 }
 
 }  // namespace onion

@@ -2,7 +2,9 @@
 // database, per-client key store, and the answer pipeline
 //   expand -> convert -> first dimension -> other dimensions -> mod switch
 // (PirServer::make_query, src/server.cpp:520-579) as a chain of HIP kernels on
-// one stream.
+// one stream.  Every stage is batched: grid.y indexes the query, each query
+// has its own scratch slice and its own client keys, and the first dimension
+// reads the database once for the whole batch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,13 +17,13 @@
 
 #include "../../include/onionpir_gpu.h"
 #include "host_math.hpp"
-#include "params.hpp"
 #include "k_db.hpp"
 #include "k_expand.hpp"
 #include "k_first_dim.hpp"
 #include "k_gsw.hpp"
 #include "k_ntt.hpp"
 #include "k_tail.hpp"
+#include "params.hpp"
 
 using namespace onion;
 
@@ -44,6 +46,8 @@ struct ClientKeys {
   uint64_t *gk = nullptr;  // h x 4N, NTT form
   uint64_t *rk = nullptr;  // 2 l_key x 2N, NTT form
 };
+
+constexpr uint32_t kMaxBatch = 64;  // queries per pipeline pass
 
 }  // namespace
 
@@ -68,15 +72,16 @@ struct onion_server {
   uint64_t n_written = 0;
   // keys
   std::map<uint64_t, ClientKeys> clients;
-  // scratch
-  uint64_t *tree = nullptr;    // 2w cts
-  uint64_t *rgsw = nullptr;    // (ndim-1) x 2l x 2N
-  uint64_t *fd = nullptr;      // other cts
-  uint64_t *digits = nullptr;  // digit polys
-  uint32_t *Qs = nullptr;      // N x fst x 4 uint32
-  uint64_t *io = nullptr;      // staging for taps (max of the above shapes)
+  // per-query scratch, `cap` queries; strides in words
+  uint32_t cap = 0;
+  uint64_t tree_words = 0, rgsw_words = 0, fd_words = 0, digit_words = 0;
+  uint64_t *trees = nullptr, *rgsw = nullptr, *fd = nullptr, *digits = nullptr;
+  uint32_t *Qs = nullptr;  // N x fst x cpad x 2 uint32 (cpad for `cap` queries)
+  const uint64_t **gk_ptrs = nullptr, **rk_ptrs = nullptr;  // device [cap]
+  std::vector<const uint64_t *> h_gk, h_rk;
+  uint64_t *io = nullptr;  // staging
+  size_t io_words = 0;
   uint8_t *pack = nullptr;
-  size_t io_words = 0, digit_words = 0;
   hipEvent_t ev[8]{};
   onion_stage_times last{};
   std::mutex mu;
@@ -101,61 +106,42 @@ void dfree(void *p) {
   if (p) (void)hipFree(p);
 }
 
-int rows_per_block(uint32_t other) { return other >= 512 ? 512 : (other < 32 ? 32 : (int)other); }
+uint32_t cpad_for(uint32_t nq) {
+  if (nq <= 1) return 2;
+  if (nq <= 16) return 32;
+  if (nq <= 32) return 64;
+  return 128;
+}
 
-// ---- pipeline stages ------------------------------------------------------------
+void free_scratch(onion_server *s) {
+  void *bufs[] = {s->trees, s->rgsw, s->fd, s->digits, s->Qs, s->gk_ptrs, s->rk_ptrs};
+  for (void *b : bufs) dfree(b);
+  s->trees = s->rgsw = s->fd = s->digits = nullptr;
+  s->Qs = nullptr;
+  s->gk_ptrs = s->rk_ptrs = nullptr;
+  s->cap = 0;
+}
 
-// Expansion of the query already stored in tree slot 1.
-void run_expand(onion_server *s, const ClientKeys &ck) {
-  const uint32_t h = s->I.expan_height, u = s->I.useful_cnt;
-  for (uint32_t a = 0; a < h; ++a) {
-    // active nodes at level a: those whose left-most leaf is < u
-    const uint64_t span = s->w >> a;  // leaves under one node
-    const uint64_t n_active = (u + span - 1) / span;
-    k_expand_level<<<(uint32_t)n_active, kNttThreads, 0, s->st>>>(
-        s->tree, ck.gk + (size_t)a * 4 * kN, a, s->ginv[a], s->Tq, s->Tp, s->mq, s->mp, s->ks);
+// Scratch for `cap` queries (kept when large enough).
+int ensure_scratch(onion_server *s, uint32_t cap) {
+  if (s->cap >= cap) return ONION_OK;
+  (void)hipStreamSynchronize(s->st);
+  free_scratch(s);
+  const uint64_t N = kN, ct = 2 * N;
+  int rc;
+  if ((rc = dalloc(&s->trees, cap * s->tree_words)) || (rc = dalloc(&s->rgsw, cap * s->rgsw_words)) ||
+      (rc = dalloc(&s->fd, cap * s->fd_words)) || (rc = dalloc(&s->digits, cap * s->digit_words)) ||
+      (rc = dalloc(&s->Qs, N * s->P.fst_dim * cpad_for(cap) * 2)) || (rc = dalloc(&s->gk_ptrs, cap)) ||
+      (rc = dalloc(&s->rk_ptrs, cap))) {
+    std::string msg = g_err;
+    free_scratch(s);
+    return fail(rc, msg + " (batch scratch for " + std::to_string(cap) + " queries)");
   }
-}
-
-// Conversion of the (ndim-1)*l RGSW leaves (starting at `conv_leaves`).
-void run_convert(onion_server *s, const ClientKeys &ck, const uint64_t *conv_leaves) {
-  const uint32_t nd = s->I.ndim - 1, l = s->P.gsw_l, lk = s->P.gsw_l_key;
-  if (nd == 0) return;
-  const uint32_t n_ct = nd * l;
-  k_rgsw_top_rows<<<n_ct * 2, kNttThreads, 0, s->st>>>(conv_leaves, s->rgsw, l, s->Tq);
-  k_decomp_ntt<<<n_ct * 2 * lk, kNttThreads, 0, s->st>>>(conv_leaves, nullptr, s->digits, lk,
-                                                         s->I.base_log2_key, s->Tq);
-  k_ext_mac_store<<<n_ct * 2, kNttThreads, 0, s->st>>>(s->digits, ck.rk, 2 * lk, s->rgsw, l, s->mq);
-}
-
-// First dimension: fst selection cts (coefficient form) -> s->fd (other cts).
-void run_first_dim(onion_server *s, const uint64_t *sel, bool time_core) {
-  const uint32_t fst = s->P.fst_dim, other = s->P.other_dim;
-  k_sel_ntt<<<fst * 2, kNttThreads, 0, s->st>>>(sel, s->Qs, fst, s->Tq);
-  if (time_core) (void)hipEventRecord(s->ev[6], s->st);
-  if (fst % 16 == 0 && fst <= (uint32_t)kMaxFst) {
-    const uint32_t rpb = rows_per_block(other);
-    dim3 grid(kN, (other + rpb - 1) / rpb);
-    k_first_dim_b1<<<grid, 256, 0, s->st>>>(s->db, s->Qs, s->fd, fst, other, s->I.num_pt, rpb, s->mq);
-  } else {
-    k_first_dim_generic<<<kN, 256, 0, s->st>>>(s->db, s->Qs, s->fd, fst, other, s->I.num_pt, s->mq);
-  }
-  if (time_core) (void)hipEventRecord(s->ev[7], s->st);
-  k_ntt_inv<<<other * 2, kNttThreads, 0, s->st>>>(s->fd, s->fd, s->Tq);
-}
-
-// One multiplexer level over n cts in `cts` with RGSW `g`.
-void run_mux_level(onion_server *s, uint64_t *cts, uint64_t n, const uint64_t *g) {
-  const uint32_t l = s->P.gsw_l;
-  const uint64_t half = n / 2;
-  k_decomp_ntt<<<(uint32_t)(half * 2 * l), kNttThreads, 0, s->st>>>(
-      cts, cts + half * 2 * kN, s->digits, l, s->I.base_log2, s->Tq);
-  k_ext_mac_intt_add<<<(uint32_t)(half * 2), kNttThreads, 0, s->st>>>(s->digits, g, 2 * l, cts,
-                                                                      s->Tq, s->mq);
-}
-
-void run_mod_switch(onion_server *s, uint64_t *ct) {
-  k_mod_switch<<<(2 * kN + 255) / 256, 256, 0, s->st>>>(ct, 2 * kN, s->ms_M, s->ms_f);
+  (void)ct;
+  s->cap = cap;
+  s->h_gk.assign(cap, nullptr);
+  s->h_rk.assign(cap, nullptr);
+  return ONION_OK;
 }
 
 int lookup(onion_server *s, uint64_t id, ClientKeys &out, bool need_rk) {
@@ -168,6 +154,20 @@ int lookup(onion_server *s, uint64_t id, ClientKeys &out, bool need_rk) {
   return ONION_OK;
 }
 
+// Resolve the keys of n clients into the device pointer arrays.
+int bind_keys(onion_server *s, uint32_t n, const uint64_t *ids, bool need_rk) {
+  for (uint32_t i = 0; i < n; ++i) {
+    ClientKeys ck;
+    int rc = lookup(s, ids[i], ck, need_rk);
+    if (rc) return rc;
+    s->h_gk[i] = ck.gk;
+    s->h_rk[i] = ck.rk;
+  }
+  HIPCHK(hipMemcpyAsync(s->gk_ptrs, s->h_gk.data(), n * sizeof(void *), hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemcpyAsync(s->rk_ptrs, s->h_rk.data(), n * sizeof(void *), hipMemcpyHostToDevice, s->st));
+  return ONION_OK;
+}
+
 int check_db(onion_server *s) {
   if (!s->db || s->n_written != s->I.num_pt)
     return fail(ONION_ESTATE, "database incomplete: " + std::to_string(s->n_written) + " of " +
@@ -175,24 +175,97 @@ int check_db(onion_server *s) {
   return ONION_OK;
 }
 
-// Full answer for the query at device pointer tree slot 1; response -> s->fd[0]
-int answer_device(onion_server *s, const ClientKeys &ck, bool timed) {
+// ---- pipeline stages (nq queries; query b's ciphertext at tree b, slot 1) ------------
+
+void run_expand(onion_server *s, uint32_t nq) {
+  const uint32_t h = s->I.expan_height, u = s->I.useful_cnt;
+  for (uint32_t a = 0; a < h; ++a) {
+    // active nodes at level a: those whose left-most leaf is < u
+    const uint64_t span = s->w >> a;  // leaves under one node
+    const uint32_t n_active = (uint32_t)((u + span - 1) / span);
+    k_expand_level<<<dim3(n_active, nq), kNttThreads, 0, s->st>>>(
+        s->trees, s->tree_words, s->gk_ptrs, a, s->ginv[a], s->Tq, s->Tp, s->mq, s->mp, s->ks);
+  }
+}
+
+// Conversion of the (ndim-1)*l RGSW leaves starting at conv (stride per query).
+void run_convert(onion_server *s, uint32_t nq, const uint64_t *conv, uint64_t conv_stride) {
+  const uint32_t nd = s->I.ndim - 1, l = s->P.gsw_l, lk = s->P.gsw_l_key;
+  if (nd == 0) return;
+  const uint32_t n_ct = nd * l;
+  k_rgsw_top_rows<<<dim3(n_ct * 2, nq), kNttThreads, 0, s->st>>>(conv, conv_stride, s->rgsw,
+                                                                 s->rgsw_words, l, s->Tq);
+  k_decomp_ntt<<<dim3(n_ct * 2 * lk, nq), kNttThreads, 0, s->st>>>(
+      conv, nullptr, conv_stride, s->digits, s->digit_words, lk, s->I.base_log2_key, s->Tq);
+  k_ext_mac_store<<<dim3(n_ct * 2, nq), kNttThreads, 0, s->st>>>(
+      s->digits, s->digit_words, s->rk_ptrs, 2 * lk, s->rgsw, s->rgsw_words, l, s->mq);
+}
+
+template <int CPAD>
+void launch_fd_bat(onion_server *s, uint32_t nq) {
+  constexpr int RBLK = (256 / (CPAD / 4)) * 4;
+  const uint32_t n_tiles = (s->P.other_dim + RBLK - 1) / RBLK;
+  k_first_dim_bat<4, 4, CPAD><<<kN * n_tiles, 256, 0, s->st>>>(
+      s->db, s->Qs, s->fd, s->fd_words, s->P.fst_dim, s->P.other_dim, s->I.num_pt, nq, n_tiles, s->mq);
+}
+
+// First dimension: fst selection cts per query (coefficient form, stride
+// sel_stride) -> s->fd (other cts per query, coefficient form).
+void run_first_dim(onion_server *s, uint32_t nq, const uint64_t *sel, uint64_t sel_stride, bool time_core) {
+  const uint32_t fst = s->P.fst_dim, other = s->P.other_dim, cpad = cpad_for(nq);
+  k_sel_ntt<<<dim3(fst * 2, nq), kNttThreads, 0, s->st>>>(sel, sel_stride, s->Qs, fst, cpad, s->Tq);
+  if (time_core) (void)hipEventRecord(s->ev[6], s->st);
+  if (nq == 1) {
+    if (fst % 16 == 0) {
+      const uint32_t rpb = other >= 512 ? 512 : (other < 32 ? 32 : other);
+      k_first_dim_b1<<<dim3(kN, (other + rpb - 1) / rpb), 256, 0, s->st>>>(
+          s->db, s->Qs, s->fd, fst, other, s->I.num_pt, rpb, s->mq);
+    } else {
+      k_first_dim_generic<<<kN, 256, 0, s->st>>>(s->db, s->Qs, s->fd, fst, other, s->I.num_pt, s->mq);
+    }
+  } else if (cpad == 32) {
+    launch_fd_bat<32>(s, nq);
+  } else if (cpad == 64) {
+    launch_fd_bat<64>(s, nq);
+  } else {
+    launch_fd_bat<128>(s, nq);
+  }
+  if (time_core) (void)hipEventRecord(s->ev[7], s->st);
+  k_ntt_inv<<<other * 2 * nq, kNttThreads, 0, s->st>>>(s->fd, s->fd, s->Tq);
+}
+
+// One multiplexer level over n cts per query in fd, with RGSW `g` (per-query stride).
+void run_mux_level(onion_server *s, uint32_t nq, uint64_t n, const uint64_t *g) {
+  const uint32_t l = s->P.gsw_l;
+  const uint64_t half = n / 2;
+  k_decomp_ntt<<<dim3((uint32_t)(half * 2 * l), nq), kNttThreads, 0, s->st>>>(
+      s->fd, s->fd + half * 2 * kN, s->fd_words, s->digits, s->digit_words, l, s->I.base_log2, s->Tq);
+  k_ext_mac_intt_add<<<dim3((uint32_t)(half * 2), nq), kNttThreads, 0, s->st>>>(
+      s->digits, s->digit_words, g, s->rgsw_words, 2 * l, s->fd, s->fd_words, s->Tq, s->mq);
+}
+
+void run_mod_switch(onion_server *s, uint32_t nq, uint64_t *cts, uint64_t stride) {
+  k_mod_switch<<<dim3((2 * kN + 255) / 256, nq), 256, 0, s->st>>>(cts, stride, 2 * kN, s->ms_M, s->ms_f);
+}
+
+// Full answer for nq queries already in tree slot 1; responses in fd[b][0].
+int answer_device(onion_server *s, uint32_t nq, bool timed) {
   const uint32_t fst = s->P.fst_dim, l = s->P.gsw_l;
-  const uint64_t *leaves = s->tree + s->w * 2 * kN;
+  const uint64_t *leaves = s->trees + s->w * 2 * kN;
   if (timed) HIPCHK(hipEventRecord(s->ev[0], s->st));
-  run_expand(s, ck);
+  run_expand(s, nq);
   if (timed) HIPCHK(hipEventRecord(s->ev[1], s->st));
-  run_convert(s, ck, leaves + (size_t)fst * 2 * kN);
+  run_convert(s, nq, leaves + (size_t)fst * 2 * kN, s->tree_words);
   if (timed) HIPCHK(hipEventRecord(s->ev[2], s->st));
-  run_first_dim(s, leaves, timed);
+  run_first_dim(s, nq, leaves, s->tree_words, timed);
   if (timed) HIPCHK(hipEventRecord(s->ev[3], s->st));
   uint64_t n = s->P.other_dim;
   for (uint32_t d = 0; d + 1 < s->I.ndim; ++d) {
-    run_mux_level(s, s->fd, n, s->rgsw + (size_t)d * 2 * l * 2 * kN);
+    run_mux_level(s, nq, n, s->rgsw + (size_t)d * 2 * l * 2 * kN);
     n /= 2;
   }
   if (timed) HIPCHK(hipEventRecord(s->ev[4], s->st));
-  run_mod_switch(s, s->fd);
+  run_mod_switch(s, nq, s->fd, s->fd_words);
   if (timed) HIPCHK(hipEventRecord(s->ev[5], s->st));
   HIPCHK(hipGetLastError());
   return ONION_OK;
@@ -212,6 +285,19 @@ int collect_times(onion_server *s) {
   s->last.total = tot;
   s->last.first_dim_core = core;
   return ONION_OK;
+}
+
+// Answer n (<= cap) queries: upload, run, download.
+int answer_chunk(onion_server *s, uint32_t n, const uint64_t *ids, const uint64_t *queries, uint64_t *resps) {
+  int rc;
+  if ((rc = bind_keys(s, n, ids, s->I.ndim > 1))) return rc;
+  HIPCHK(hipMemcpy2DAsync(s->trees + 2 * kN, s->tree_words * 8, queries, 2 * kN * 8, 2 * kN * 8, n,
+                          hipMemcpyHostToDevice, s->st));
+  if ((rc = answer_device(s, n, true))) return rc;
+  HIPCHK(hipMemcpy2DAsync(resps, 2 * kN * 8, s->fd, s->fd_words * 8, 2 * kN * 8, n, hipMemcpyDeviceToHost,
+                          s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return collect_times(s);
 }
 
 }  // namespace
@@ -296,20 +382,20 @@ int onion_gpu_create(const onion_params *params, onion_server **out) {
   }
   host::mod_switch_scale(I.q, I.small_q, s->ms_M, s->ms_f);
 
-  // database and scratch
+  // per-query scratch sizes
   const uint64_t ct = 2 * N;
   const uint32_t nd = I.ndim - 1;
+  s->tree_words = 2 * s->w * ct;
+  s->rgsw_words = (uint64_t)std::max(1u, nd) * 2 * P.gsw_l * ct;
+  s->fd_words = (uint64_t)P.other_dim * ct;
   const uint64_t conv_digits = (uint64_t)nd * P.gsw_l * 2 * P.gsw_l_key * N;
-  const uint64_t mux_digits = (uint64_t)(P.other_dim / 2) * 2 * P.gsw_l * N;
-  s->digit_words = std::max<uint64_t>(std::max(conv_digits, mux_digits), N);
+  const uint64_t mux_digits = (uint64_t)std::max(1u, P.other_dim / 2) * 2 * P.gsw_l * N;
+  s->digit_words = std::max(conv_digits, mux_digits);
   s->io_words = std::max<uint64_t>({(uint64_t)I.useful_cnt * ct, (uint64_t)P.other_dim * ct,
                                     (uint64_t)std::max(1u, nd) * 2 * P.gsw_l * ct, 64 * ct,
                                     (uint64_t)P.fst_dim * ct});
-  if ((rc = dalloc(&s->db, I.num_pt * N)) || (rc = dalloc(&s->tree, 2 * s->w * ct)) ||
-      (rc = dalloc(&s->rgsw, (uint64_t)std::max(1u, nd) * 2 * P.gsw_l * ct)) ||
-      (rc = dalloc(&s->fd, (uint64_t)P.other_dim * ct)) || (rc = dalloc(&s->digits, s->digit_words)) ||
-      (rc = dalloc(&s->Qs, (uint64_t)N * P.fst_dim * 4)) || (rc = dalloc(&s->io, s->io_words)) ||
-      (rc = dalloc(&s->pack, (uint64_t)I.response_bytes + 64))) {
+  if ((rc = dalloc(&s->db, I.num_pt * N)) || (rc = dalloc(&s->io, s->io_words)) ||
+      (rc = dalloc(&s->pack, (uint64_t)I.response_bytes + 64)) || (rc = ensure_scratch(s, 1))) {
     std::string msg = g_err;
     onion_gpu_destroy(s);
     return fail(rc, msg);
@@ -326,7 +412,8 @@ int onion_gpu_destroy(onion_server *s) {
     dfree(kv.second.gk);
     dfree(kv.second.rk);
   }
-  void *bufs[] = {s->tabs, s->db, s->tree, s->rgsw, s->fd, s->digits, s->Qs, s->io, s->pack};
+  free_scratch(s);
+  void *bufs[] = {s->tabs, s->db, s->io, s->pack};
   for (void *b : bufs) dfree(b);
   for (auto &e : s->ev)
     if (e) (void)hipEventDestroy(e);
@@ -442,21 +529,21 @@ int onion_gpu_answer(onion_server *s, uint64_t id, const uint64_t *query, uint64
   std::lock_guard<std::mutex> lk(s->mu);
   int rc;
   if ((rc = check_db(s))) return rc;
-  ClientKeys ck;
-  if ((rc = lookup(s, id, ck, s->I.ndim > 1))) return rc;
-  HIPCHK(hipMemcpyAsync(s->tree + 2 * kN, query, 2 * kN * 8, hipMemcpyHostToDevice, s->st));
-  if ((rc = answer_device(s, ck, true))) return rc;
-  HIPCHK(hipMemcpyAsync(resp, s->fd, 2 * kN * 8, hipMemcpyDeviceToHost, s->st));
-  HIPCHK(hipStreamSynchronize(s->st));
-  return collect_times(s);
+  return answer_chunk(s, 1, &id, query, resp);
 }
 
 int onion_gpu_answer_batch(onion_server *s, uint32_t n, const uint64_t *ids, const uint64_t *queries,
                            uint64_t *resps) {
   if (!s || (n && (!ids || !queries || !resps))) return fail(ONION_EINVAL, "null argument");
-  for (uint32_t i = 0; i < n; ++i) {
-    int rc = onion_gpu_answer(s, ids[i], queries + (size_t)i * 2 * kN, resps + (size_t)i * 2 * kN);
-    if (rc) return rc;
+  std::lock_guard<std::mutex> lk(s->mu);
+  int rc;
+  if ((rc = check_db(s))) return rc;
+  if (n == 0) return ONION_OK;
+  if ((rc = ensure_scratch(s, std::min(n, kMaxBatch)))) return rc;
+  for (uint32_t off = 0; off < n; off += kMaxBatch) {
+    const uint32_t m = std::min(kMaxBatch, n - off);
+    if ((rc = answer_chunk(s, m, ids + off, queries + (size_t)off * 2 * kN, resps + (size_t)off * 2 * kN)))
+      return rc;
   }
   return ONION_OK;
 }
@@ -466,7 +553,7 @@ int onion_gpu_pack_response(onion_server *s, const uint64_t *resp, uint8_t *out,
   if (!s || !resp || !out) return fail(ONION_EINVAL, "null argument");
   if (cap < s->I.response_bytes) return fail(ONION_EINVAL, "output buffer too small");
   std::lock_guard<std::mutex> lk(s->mu);
-  const uint32_t wbits = (uint32_t)std::ceil(std::log2((double)s->I.small_q));
+  const uint32_t wbits = response_bits(s->I.small_q);
   HIPCHK(hipMemcpyAsync(s->io, resp, 2 * kN * 8, hipMemcpyHostToDevice, s->st));
   k_pack_response<<<(2 * kN / 8 + 255) / 256, 256, 0, s->st>>>(s->io, s->pack, 2 * kN, wbits);
   HIPCHK(hipGetLastError());
@@ -487,13 +574,12 @@ int onion_gpu_last_stage_times(const onion_server *s, onion_stage_times *out) {
 int onion_gpu_tap_expand(onion_server *s, uint64_t id, const uint64_t *query, uint64_t *leaves) {
   if (!s || !query || !leaves) return fail(ONION_EINVAL, "null argument");
   std::lock_guard<std::mutex> lk(s->mu);
-  ClientKeys ck;
   int rc;
-  if ((rc = lookup(s, id, ck, false))) return rc;
-  HIPCHK(hipMemcpyAsync(s->tree + 2 * kN, query, 2 * kN * 8, hipMemcpyHostToDevice, s->st));
-  run_expand(s, ck);
+  if ((rc = bind_keys(s, 1, &id, false))) return rc;
+  HIPCHK(hipMemcpyAsync(s->trees + 2 * kN, query, 2 * kN * 8, hipMemcpyHostToDevice, s->st));
+  run_expand(s, 1);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(leaves, s->tree + s->w * 2 * kN, (size_t)s->I.useful_cnt * 2 * kN * 8,
+  HIPCHK(hipMemcpyAsync(leaves, s->trees + s->w * 2 * kN, (size_t)s->I.useful_cnt * 2 * kN * 8,
                         hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
   return ONION_OK;
@@ -502,15 +588,14 @@ int onion_gpu_tap_expand(onion_server *s, uint64_t id, const uint64_t *query, ui
 int onion_gpu_tap_convert(onion_server *s, uint64_t id, const uint64_t *leaves, uint64_t *rgsw) {
   if (!s || !leaves || !rgsw) return fail(ONION_EINVAL, "null argument");
   std::lock_guard<std::mutex> lk(s->mu);
-  ClientKeys ck;
   int rc;
-  if ((rc = lookup(s, id, ck, true))) return rc;
+  if ((rc = bind_keys(s, 1, &id, true))) return rc;
   const uint32_t nd = s->I.ndim - 1;
   const size_t n_ct = (size_t)nd * s->P.gsw_l;
   if (nd == 0) return ONION_OK;
   HIPCHK(hipMemcpyAsync(s->io, leaves + (size_t)s->P.fst_dim * 2 * kN, n_ct * 2 * kN * 8,
                         hipMemcpyHostToDevice, s->st));
-  run_convert(s, ck, s->io);
+  run_convert(s, 1, s->io, 0);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(rgsw, s->rgsw, (size_t)nd * 2 * s->P.gsw_l * 2 * kN * 8, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
@@ -523,7 +608,7 @@ int onion_gpu_tap_first_dim(onion_server *s, const uint64_t *sel, uint64_t *out)
   int rc;
   if ((rc = check_db(s))) return rc;
   HIPCHK(hipMemcpyAsync(s->io, sel, (size_t)s->P.fst_dim * 2 * kN * 8, hipMemcpyHostToDevice, s->st));
-  run_first_dim(s, s->io, false);
+  run_first_dim(s, 1, s->io, 0, false);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, s->fd, (size_t)s->P.other_dim * 2 * kN * 8, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
@@ -536,7 +621,7 @@ int onion_gpu_tap_mux(onion_server *s, const uint64_t *cts, uint32_t n, const ui
   std::lock_guard<std::mutex> lk(s->mu);
   HIPCHK(hipMemcpyAsync(s->fd, cts, (size_t)n * 2 * kN * 8, hipMemcpyHostToDevice, s->st));
   HIPCHK(hipMemcpyAsync(s->rgsw, rgsw, (size_t)2 * s->P.gsw_l * 2 * kN * 8, hipMemcpyHostToDevice, s->st));
-  run_mux_level(s, s->fd, n, s->rgsw);
+  run_mux_level(s, 1, n, s->rgsw);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, s->fd, (size_t)(n / 2) * 2 * kN * 8, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
@@ -564,17 +649,15 @@ int onion_gpu_tap_galois(onion_server *s, uint64_t id, uint32_t level, const uin
   if (!s || !ct || !out) return fail(ONION_EINVAL, "null argument");
   if (level >= s->I.expan_height) return fail(ONION_EINVAL, "level >= expansion height");
   std::lock_guard<std::mutex> lk(s->mu);
-  ClientKeys ck;
   int rc;
-  if ((rc = lookup(s, id, ck, false))) return rc;
-  // place ct at node 2^level; its left child is ct + Subs(ct), so
-  // Subs(ct) = left - ct
+  if ((rc = bind_keys(s, 1, &id, false))) return rc;
+  // place ct at node 2^level; its left child is ct + Subs(ct), so Subs(ct) = left - ct
   const uint64_t node = uint64_t(1) << level;
-  HIPCHK(hipMemcpyAsync(s->tree + node * 2 * kN, ct, 2 * kN * 8, hipMemcpyHostToDevice, s->st));
-  k_expand_level<<<1, kNttThreads, 0, s->st>>>(s->tree, ck.gk + (size_t)level * 4 * kN, level,
-                                               s->ginv[level], s->Tq, s->Tp, s->mq, s->mp, s->ks);
+  HIPCHK(hipMemcpyAsync(s->trees + node * 2 * kN, ct, 2 * kN * 8, hipMemcpyHostToDevice, s->st));
+  k_expand_level<<<dim3(1, 1), kNttThreads, 0, s->st>>>(s->trees, s->tree_words, s->gk_ptrs, level,
+                                                       s->ginv[level], s->Tq, s->Tp, s->mq, s->mp, s->ks);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(out, s->tree + 2 * node * 2 * kN, 2 * kN * 8, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipMemcpyAsync(out, s->trees + 2 * node * 2 * kN, 2 * kN * 8, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
   const uint64_t q = s->I.q;
   for (size_t i = 0; i < 2 * (size_t)kN; ++i) out[i] = out[i] >= ct[i] ? out[i] - ct[i] : out[i] + q - ct[i];
@@ -584,7 +667,7 @@ int onion_gpu_tap_galois(onion_server *s, uint64_t id, uint32_t level, const uin
 int onion_gpu_tap_copy(onion_server *s, uint64_t n_bytes, uint64_t *copied) {
   if (!s) return fail(ONION_EINVAL, "null server");
   std::lock_guard<std::mutex> lk(s->mu);
-  uint64_t n = std::min<uint64_t>({n_bytes, s->digit_words * 8, s->I.num_pt * kN * 8}) / 16;
+  const uint64_t n = std::min<uint64_t>({n_bytes, s->cap * s->digit_words * 8, s->I.num_pt * kN * 8}) / 16;
   if (n) {
     k_copy<<<1024, 256, 0, s->st>>>(reinterpret_cast<const uint4 *>(s->db), reinterpret_cast<uint4 *>(s->digits), n);
     HIPCHK(hipGetLastError());
@@ -598,7 +681,7 @@ int onion_gpu_tap_mod_switch(onion_server *s, const uint64_t *ct, uint64_t *out)
   if (!s || !ct || !out) return fail(ONION_EINVAL, "null argument");
   std::lock_guard<std::mutex> lk(s->mu);
   HIPCHK(hipMemcpyAsync(s->io, ct, 2 * kN * 8, hipMemcpyHostToDevice, s->st));
-  run_mod_switch(s, s->io);
+  run_mod_switch(s, 1, s->io, 0);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, s->io, 2 * kN * 8, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));

@@ -195,3 +195,35 @@ def test_first_dim_wraps_mod_2_128():
         exp = o.first_dim(db.reshape(-1), sel)
         got = srv.tap_first_dim(sel)
         assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("nq", [3, 17, 40])
+def test_answer_batch_matches_single_and_oracle(cfg1, nq):
+    """A batch of nq queries from nq different clients (each with its own keys)
+    gives, query by query, the single-query response; three of them are also
+    checked against the oracle.  nq = 3 / 17 / 40 take the 32 / 64 / 128-column
+    variants of the batched first-dimension kernel."""
+    o, srv = cfg1["o"], cfg1["srv"]
+    rng = np.random.default_rng(nq)
+    idx = rng.choice(o.num_pt, size=nq, replace=False)
+    ids, qs, keys = [], [], []
+    for i in range(nq):
+        cid = 1000 + i
+        seed = 500 + i
+        sk = o.keygen(seed)
+        gk, rk = o.galois_keys(seed, sk), o.rgsw_key(seed, sk)
+        srv.set_galois_key(cid, gk)
+        srv.set_rgsw_key(cid, rk)
+        ids.append(cid)
+        qs.append(o.query(seed, i, sk, int(idx[i])))
+        keys.append((sk, gk, rk))
+    got = srv.answer_batch(ids, np.stack(qs))
+    for i in range(nq):
+        assert np.array_equal(got[i], srv.answer(ids[i], qs[i])), f"query {i}"
+    for i in (0, nq // 2, nq - 1):
+        sk, gk, rk = keys[i]
+        exp, _ = o.answer(cfg1["db"], gk, o.rgsw_key_ntt(rk), qs[i])
+        assert np.array_equal(got[i], exp)
+        assert np.array_equal(o.decrypt_small(sk, got[i]), o.db_plaintext(DB_SEED, 0, int(idx[i])))
+    for cid in ids:
+        srv.drop_client(cid)
