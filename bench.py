@@ -123,12 +123,13 @@ def main() -> int:
     # ---- batched path (the timed steps) -----------------------------------------------
     for _ in range(args.warmup):
         srv.answer_batch(ids, queries)
-    step_ms = []
+    step_ms, bstages = [], []
     t_all = time.perf_counter()
     for _ in range(args.steps):
         t = time.perf_counter()
         resps = srv.answer_batch(ids, queries)
         step_ms.append((time.perf_counter() - t) * 1e3)
+        bstages.append(srv.stage_times())
     total_s = time.perf_counter() - t_all
 
     # ---- correctness: every client decrypts its exact record ----------------------------
@@ -164,7 +165,9 @@ def main() -> int:
                    "plain_modulus": info["t"]},
         "batched": {"queries_per_s": B * args.steps / total_s, "step_ms": float(np.mean(step_ms)),
                     "p50_ms": pct(step_ms, 50), "p99_ms": pct(step_ms, 99),
-                    "db_GBps_scanned": db_bytes * B / (np.mean(step_ms) * 1e-3) / 1e9},
+                    "db_GBps_scanned": db_bytes * B / (np.mean(step_ms) * 1e-3) / 1e9,
+                    "stage_ms": {k: float(np.mean([s[k] for s in bstages])) for k in bstages[0]}
+                    if bstages else {}},
         "single_query": {"queries_per_s": 1e3 / float(np.mean(single_lat)),
                          "p50_ms": pct(single_lat, 50), "p99_ms": pct(single_lat, 99),
                          "db_GBps_scanned": db_bytes / (float(np.mean(single_lat)) * 1e-3) / 1e9,
