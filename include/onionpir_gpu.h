@@ -70,6 +70,21 @@ typedef struct onion_stage_times {
   double first_dim_core; /* the database-streaming kernel alone                   */
 } onion_stage_times;
 
+/* Launch-shape choices of the device path (no reference counterpart: the
+ * reference fixes its loop shapes at compile time).  Every choice gives
+ * bit-identical results; only speed differs. */
+typedef struct onion_tuning {
+  uint32_t fuse_min;     /* external products (conversion, mux levels) run one
+                            workgroup per ciphertext when a stage has at least
+                            this many ciphertexts across the batch; below it
+                            every digit NTT gets its own workgroup            */
+  uint32_t fd_rows_b1;   /* rows per workgroup of the single-query first-
+                            dimension kernel (multiple of 32, 32..1024)       */
+  uint32_t fd_tile_rows; /* thread rows of the batched first-dimension tile
+                            (TR, 2 or 4)                                      */
+  uint32_t reserved;
+} onion_tuning;
+
 typedef struct onion_server onion_server;
 
 const char *onion_gpu_last_error(void);
@@ -130,6 +145,9 @@ int onion_gpu_pack_response(onion_server *srv, const uint64_t *resp, uint8_t *ou
                             size_t out_cap, size_t *n_bytes);
 
 int onion_gpu_last_stage_times(const onion_server *srv, onion_stage_times *out);
+
+int onion_gpu_get_tuning(const onion_server *srv, onion_tuning *out);
+int onion_gpu_set_tuning(onion_server *srv, const onion_tuning *tuning);
 
 /* ---- stage taps: each runs one stage of the device path on host-given
  * inputs, for parity tests against the CPU oracle -------------------------- */

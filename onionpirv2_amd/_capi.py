@@ -37,6 +37,11 @@ class OnionStageTimes(C.Structure):
                 ("expand", "convert", "first_dim", "other_dim", "mod_switch", "total", "first_dim_core")]
 
 
+class OnionTuning(C.Structure):
+    _fields_ = [("fuse_min", C.c_uint32), ("fd_rows_b1", C.c_uint32), ("fd_tile_rows", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
 class OnionError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
@@ -71,6 +76,8 @@ _SIGS = {
     "onion_gpu_pack_response": (C.c_int, [C.c_void_p, U64P, C.c_char_p, C.c_size_t,
                                           C.POINTER(C.c_size_t)]),
     "onion_gpu_last_stage_times": (C.c_int, [C.c_void_p, C.POINTER(OnionStageTimes)]),
+    "onion_gpu_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(OnionTuning)]),
+    "onion_gpu_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(OnionTuning)]),
     "onion_gpu_tap_expand": (C.c_int, [C.c_void_p, C.c_uint64, U64P, U64P]),
     "onion_gpu_tap_convert": (C.c_int, [C.c_void_p, C.c_uint64, U64P, U64P]),
     "onion_gpu_tap_first_dim": (C.c_int, [C.c_void_p, U64P, U64P]),

@@ -72,6 +72,28 @@ __device__ __forceinline__ void fold(uint64_t &lo, uint64_t &hi, Acc3 &x) {
   x.s0 = x.s1 = x.s2 = 0;
 }
 
+// Four independent partial sums (the two cross products apart), so each of
+// the four v_mad_u64_u32 per product accumulates in place; 16 terms fit.
+struct Acc4 {
+  uint64_t s0, s1a, s1b, s2;
+};
+
+__device__ __forceinline__ void acc_word4(Acc4 &x, uint32_t a0, uint32_t a1, uint32_t b0,
+                                          uint32_t b1) {
+  x.s0 = mad32(a0, b0, x.s0);
+  x.s1a = mad32(a0, b1, x.s1a);
+  x.s1b = mad32(a1, b0, x.s1b);
+  x.s2 = mad32(a1, b1, x.s2);
+}
+
+__device__ __forceinline__ void fold4(uint64_t &lo, uint64_t &hi, Acc4 &x) {
+  add128(lo, hi, x.s0, 0);
+  add128(lo, hi, x.s1a << 30, x.s1a >> 34);
+  add128(lo, hi, x.s1b << 30, x.s1b >> 34);
+  add128(lo, hi, x.s2 << 60, x.s2 >> 4);
+  x.s0 = x.s1a = x.s1b = x.s2 = 0;
+}
+
 __device__ __forceinline__ void split30(uint64_t a, uint32_t &a0, uint32_t &a1) {
   a0 = (uint32_t)a & kMask30;
   a1 = (uint32_t)(a >> 30);
@@ -198,13 +220,13 @@ __global__ __launch_bounds__(256) void k_first_dim_bat(const uint64_t *__restric
   const uint64_t *A = db + (size_t)slot * num_pt;
   const uint2 *Q = reinterpret_cast<const uint2 *>(Qs) + (size_t)slot * fst * CPAD;
 
-  Acc3 acc[TR][TC];
+  Acc4 acc[TR][TC];
   uint64_t wlo[TR][TC], whi[TR][TC];
 #pragma unroll
   for (int i = 0; i < TR; ++i)
 #pragma unroll
     for (int k = 0; k < TC; ++k) {
-      acc[i][k] = Acc3{0, 0, 0};
+      acc[i][k] = Acc4{0, 0, 0, 0};
       wlo[i][k] = whi[i][k] = 0;
     }
 
@@ -221,7 +243,7 @@ __global__ __launch_bounds__(256) void k_first_dim_bat(const uint64_t *__restric
       Qt[kk][col] = (k0 + kk < fst) ? Q[(size_t)(k0 + kk) * CPAD + col] : make_uint2(0, 0);
     }
     __syncthreads();
-#pragma unroll 2
+#pragma unroll 4
     for (int kk = 0; kk < KC; ++kk) {
       uint2 a[TR], b[TC];
 #pragma unroll
@@ -231,12 +253,12 @@ __global__ __launch_bounds__(256) void k_first_dim_bat(const uint64_t *__restric
 #pragma unroll
       for (int i = 0; i < TR; ++i)
 #pragma unroll
-        for (int k = 0; k < TC; ++k) acc_word(acc[i][k], a[i].x, a[i].y, b[k].x, b[k].y);
-      if ((kk & 7) == 7) {
+        for (int k = 0; k < TC; ++k) acc_word4(acc[i][k], a[i].x, a[i].y, b[k].x, b[k].y);
+      if ((kk & 15) == 15) {
 #pragma unroll
         for (int i = 0; i < TR; ++i)
 #pragma unroll
-          for (int k = 0; k < TC; ++k) fold(wlo[i][k], whi[i][k], acc[i][k]);
+          for (int k = 0; k < TC; ++k) fold4(wlo[i][k], whi[i][k], acc[i][k]);
       }
     }
   }

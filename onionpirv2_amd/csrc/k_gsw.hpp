@@ -111,6 +111,97 @@ __global__ __launch_bounds__(kNttThreads) void k_ext_mac_intt_add(const uint64_t
   }
 }
 
+// ---- fused variant: one workgroup per (ciphertext, query) ---------------------------
+// All 2l digit NTTs of one ciphertext run back to back in the workgroup and
+// both output components accumulate in registers, so digits never leave the
+// CU.  Chosen when a level has enough ciphertexts to fill the device.
+// Result: res[comp][r] = blocked slot 8t + r of sum_j D_j * G_j[comp] mod q.
+__device__ __forceinline__ void ext_fused_core(const uint64_t *xs, const uint64_t *ys, const uint64_t *G,
+                                               uint32_t l, uint32_t base_log2, uint64_t *lds,
+                                               const NttTab &T, const Mod &mq, uint64_t (&res)[2][8]) {
+  const uint64_t mask = (1ULL << base_log2) - 1;
+  uint64_t lo[2][8], hi[2][8];
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int r = 0; r < 8; ++r) lo[c][r] = hi[c][r] = 0;
+#pragma unroll 1
+  for (uint32_t poly = 0; poly < 2; ++poly) {
+    uint64_t v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int e = threadIdx.x + 256 * r;
+      const uint64_t xa = xs[(size_t)poly * kN + e];
+      v[r] = ys ? sub_mod(ys[(size_t)poly * kN + e], xa, T.q) : xa;
+    }
+#pragma unroll 1
+    for (uint32_t kk = 0; kk < l; ++kk) {
+      const uint32_t sh = (l - 1 - kk) * base_log2;
+      uint64_t d[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) d[r] = (v[r] >> sh) & mask;
+      ntt_fwd(d, lds, T);
+      const uint64_t *g = G + (size_t)(poly * l + kk) * 2 * kN;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        uint64_t gv[8];
+        load_blocked(gv, g + (size_t)c * kN);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          uint64_t plo, phi;
+          mul128(ntt_fwd_final(d[r], T.q), gv[r], plo, phi);
+          add128(lo[c][r], hi[c][r], plo, phi);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int r = 0; r < 8; ++r) res[c][r] = barrett128(lo[c][r], hi[c][r], mq);
+}
+
+// Mux fold, fused: x_c += INTT(ext(RGSW, y_c - x_c)) for ciphertext blockIdx.x
+// of query blockIdx.y (x = cts + c*2N, y = cts + (c + half)*2N).
+__global__ __launch_bounds__(kNttThreads) void k_mux_fused(uint64_t *cts, uint64_t cts_stride, uint64_t half,
+                                                           const uint64_t *G, uint64_t g_stride, uint32_t l,
+                                                           uint32_t base_log2, NttTab T, Mod mq) {
+  __shared__ uint64_t lds[kNttLds];
+  uint64_t *x = cts + (size_t)blockIdx.y * cts_stride + (size_t)blockIdx.x * 2 * kN;
+  const uint64_t *y = x + half * 2 * kN;
+  uint64_t res[2][8];
+  ext_fused_core(x, y, G + (size_t)blockIdx.y * g_stride, l, base_log2, lds, T, mq, res);
+#pragma unroll 1
+  for (int c = 0; c < 2; ++c) {
+    uint64_t v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = res[c][r];
+    ntt_inv(v, lds, T);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int e = threadIdx.x + 256 * r;
+      x[(size_t)c * kN + e] = add_mod(x[(size_t)c * kN + e], ntt_inv_final(v[r], T), T.q);
+    }
+  }
+}
+
+// Conversion bottom rows, fused: row l + c%l of RGSW c/l = ext(RGSW(s) key, leaf c).
+__global__ __launch_bounds__(kNttThreads) void k_convert_fused(const uint64_t *leaves, uint64_t leaf_stride,
+                                                               const uint64_t *const *g_ptrs, uint32_t lk,
+                                                               uint32_t base_log2_key, uint64_t *out,
+                                                               uint64_t out_stride, uint32_t l_out, NttTab T,
+                                                               Mod mq) {
+  __shared__ uint64_t lds[kNttLds];
+  const uint64_t ct = blockIdx.x;
+  const uint64_t *src = leaves + (size_t)blockIdx.y * leaf_stride + ct * 2 * kN;
+  uint64_t res[2][8];
+  ext_fused_core(src, nullptr, g_ptrs[blockIdx.y], lk, base_log2_key, lds, T, mq, res);
+  uint64_t *dst = out + (size_t)blockIdx.y * out_stride + (ct / l_out) * (2 * l_out) * 2 * kN +
+                  (l_out + ct % l_out) * 2 * kN;
+  store_blocked(dst, res[0]);
+  store_blocked(dst + kN, res[1]);
+}
+
 // Conversion rows 0..l-1: NTT of both polys of leaf c into
 //   out + (c / l) * 2l * 2N + (c % l) * 2N + poly * N   (per query)
 __global__ __launch_bounds__(kNttThreads) void k_rgsw_top_rows(const uint64_t *leaves, uint64_t leaf_stride,

@@ -84,6 +84,7 @@ struct onion_server {
   uint8_t *pack = nullptr;
   hipEvent_t ev[8]{};
   onion_stage_times last{};
+  onion_tuning tune{1024, 512, 4, 0};
   std::mutex mu;
 };
 
@@ -195,18 +196,32 @@ void run_convert(onion_server *s, uint32_t nq, const uint64_t *conv, uint64_t co
   const uint32_t n_ct = nd * l;
   k_rgsw_top_rows<<<dim3(n_ct * 2, nq), kNttThreads, 0, s->st>>>(conv, conv_stride, s->rgsw,
                                                                  s->rgsw_words, l, s->Tq);
+  if ((uint64_t)n_ct * nq >= s->tune.fuse_min) {
+    k_convert_fused<<<dim3(n_ct, nq), kNttThreads, 0, s->st>>>(conv, conv_stride, s->rk_ptrs, lk,
+                                                               s->I.base_log2_key, s->rgsw, s->rgsw_words, l,
+                                                               s->Tq, s->mq);
+    return;
+  }
   k_decomp_ntt<<<dim3(n_ct * 2 * lk, nq), kNttThreads, 0, s->st>>>(
       conv, nullptr, conv_stride, s->digits, s->digit_words, lk, s->I.base_log2_key, s->Tq);
   k_ext_mac_store<<<dim3(n_ct * 2, nq), kNttThreads, 0, s->st>>>(
       s->digits, s->digit_words, s->rk_ptrs, 2 * lk, s->rgsw, s->rgsw_words, l, s->mq);
 }
 
+template <int TR, int CPAD>
+void launch_fd_bat_tr(onion_server *s, uint32_t nq) {
+  constexpr int RBLK = (256 / (CPAD / 4)) * TR;
+  const uint32_t n_tiles = (s->P.other_dim + RBLK - 1) / RBLK;
+  k_first_dim_bat<TR, 4, CPAD><<<kN * n_tiles, 256, 0, s->st>>>(
+      s->db, s->Qs, s->fd, s->fd_words, s->P.fst_dim, s->P.other_dim, s->I.num_pt, nq, n_tiles, s->mq);
+}
+
 template <int CPAD>
 void launch_fd_bat(onion_server *s, uint32_t nq) {
-  constexpr int RBLK = (256 / (CPAD / 4)) * 4;
-  const uint32_t n_tiles = (s->P.other_dim + RBLK - 1) / RBLK;
-  k_first_dim_bat<4, 4, CPAD><<<kN * n_tiles, 256, 0, s->st>>>(
-      s->db, s->Qs, s->fd, s->fd_words, s->P.fst_dim, s->P.other_dim, s->I.num_pt, nq, n_tiles, s->mq);
+  if (s->tune.fd_tile_rows == 2)
+    launch_fd_bat_tr<2, CPAD>(s, nq);
+  else
+    launch_fd_bat_tr<4, CPAD>(s, nq);
 }
 
 // First dimension: fst selection cts per query (coefficient form, stride
@@ -217,7 +232,7 @@ void run_first_dim(onion_server *s, uint32_t nq, const uint64_t *sel, uint64_t s
   if (time_core) (void)hipEventRecord(s->ev[6], s->st);
   if (nq == 1) {
     if (fst % 16 == 0) {
-      const uint32_t rpb = other >= 512 ? 512 : (other < 32 ? 32 : other);
+      const uint32_t rpb = std::max(32u, std::min(s->tune.fd_rows_b1, (other + 31) / 32 * 32));
       k_first_dim_b1<<<dim3(kN, (other + rpb - 1) / rpb), 256, 0, s->st>>>(
           s->db, s->Qs, s->fd, fst, other, s->I.num_pt, rpb, s->mq);
     } else {
@@ -238,6 +253,11 @@ void run_first_dim(onion_server *s, uint32_t nq, const uint64_t *sel, uint64_t s
 void run_mux_level(onion_server *s, uint32_t nq, uint64_t n, const uint64_t *g) {
   const uint32_t l = s->P.gsw_l;
   const uint64_t half = n / 2;
+  if (half * nq >= s->tune.fuse_min) {
+    k_mux_fused<<<dim3((uint32_t)half, nq), kNttThreads, 0, s->st>>>(s->fd, s->fd_words, half, g, s->rgsw_words,
+                                                                    l, s->I.base_log2, s->Tq, s->mq);
+    return;
+  }
   k_decomp_ntt<<<dim3((uint32_t)(half * 2 * l), nq), kNttThreads, 0, s->st>>>(
       s->fd, s->fd + half * 2 * kN, s->fd_words, s->digits, s->digit_words, l, s->I.base_log2, s->Tq);
   k_ext_mac_intt_add<<<dim3((uint32_t)(half * 2), nq), kNttThreads, 0, s->st>>>(
@@ -560,6 +580,23 @@ int onion_gpu_pack_response(onion_server *s, const uint64_t *resp, uint8_t *out,
   HIPCHK(hipMemcpyAsync(out, s->pack, s->I.response_bytes, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
   if (n_bytes) *n_bytes = s->I.response_bytes;
+  return ONION_OK;
+}
+
+int onion_gpu_get_tuning(const onion_server *s, onion_tuning *out) {
+  if (!s || !out) return fail(ONION_EINVAL, "null argument");
+  *out = s->tune;
+  return ONION_OK;
+}
+
+int onion_gpu_set_tuning(onion_server *s, const onion_tuning *t) {
+  if (!s || !t) return fail(ONION_EINVAL, "null argument");
+  if (t->fd_rows_b1 < 32 || t->fd_rows_b1 > 1024 || t->fd_rows_b1 % 32)
+    return fail(ONION_EINVAL, "fd_rows_b1 must be a multiple of 32 in [32, 1024]");
+  if (t->fd_tile_rows != 2 && t->fd_tile_rows != 4) return fail(ONION_EINVAL, "fd_tile_rows must be 2 or 4");
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->tune = *t;
+  s->tune.fuse_min = std::max(1u, t->fuse_min);
   return ONION_OK;
 }
 

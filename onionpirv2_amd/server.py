@@ -112,6 +112,17 @@ class GpuServer:
         check(self._L.onion_gpu_pack_response(self._h, u64ptr(r), buf, cap, C.byref(n)))
         return buf.raw[: n.value]
 
+    def get_tuning(self) -> dict:
+        t = _capi.OnionTuning()
+        check(self._L.onion_gpu_get_tuning(self._h, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in _capi.OnionTuning._fields_}
+
+    def set_tuning(self, **kw) -> None:
+        cur = self.get_tuning()
+        cur.update(kw)
+        t = _capi.OnionTuning(*[cur[f] for f, _ in _capi.OnionTuning._fields_])
+        check(self._L.onion_gpu_set_tuning(self._h, C.byref(t)))
+
     def stage_times(self) -> dict:
         t = OnionStageTimes()
         check(self._L.onion_gpu_last_stage_times(self._h, C.byref(t)))

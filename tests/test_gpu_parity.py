@@ -227,3 +227,28 @@ def test_answer_batch_matches_single_and_oracle(cfg1, nq):
         assert np.array_equal(o.decrypt_small(sk, got[i]), o.db_plaintext(DB_SEED, 0, int(idx[i])))
     for cid in ids:
         srv.drop_client(cid)
+
+
+@pytest.mark.parametrize("tuning", [dict(fuse_min=1), dict(fuse_min=1 << 30),
+                                    dict(fd_rows_b1=64), dict(fd_tile_rows=2)])
+def test_tuning_choices_are_bit_identical(cfg1, tuning):
+    """Every launch-shape choice (fused vs split external products, first-dim
+    tile shapes) gives the oracle's bits, for a single query and a batch."""
+    o, srv = cfg1["o"], cfg1["srv"]
+    saved = srv.get_tuning()
+    try:
+        srv.set_tuning(**tuning)
+        rng = np.random.default_rng(77)
+        cts = rng.integers(0, o.q, size=(32, 2 * N), dtype=np.uint64)
+        g = o.rgsw(CLIENT_SEED, 91, cfg1["sk"], 1)
+        assert np.array_equal(srv.tap_mux(cts, g), o.mux_level(cts, g))
+        q = o.query(CLIENT_SEED, 300, cfg1["sk"], 777)
+        leaves = o.expand(q, cfg1["gk"])
+        assert np.array_equal(srv.tap_convert(CID, leaves), o.convert(leaves, cfg1["rkn"]))
+        exp, _ = o.answer(cfg1["db"], cfg1["gk"], cfg1["rkn"], q)
+        assert np.array_equal(srv.answer(CID, q), exp)
+        got = srv.answer_batch([CID] * 5, np.stack([q] * 5))
+        for i in range(5):
+            assert np.array_equal(got[i], exp)
+    finally:
+        srv.set_tuning(**saved)
