@@ -2,13 +2,18 @@
 // (PirServer::fast_expand_qry, src/server.cpp:382-428).
 //
 // The tree lives in a device buffer of 2w ciphertexts (heap slots, root = 1).
-// At level a (k = 2^a) the active nodes are i = k .. k + n_active - 1; one
-// workgroup handles one node end to end:
-//   c'   = Subs(c_i, N/k + 1)  -- Galois permutation of c0 and c1, then key
-//          switching of sigma(c1) with the special prime p (NTT mod q and mod
-//          p, product with the level's key, INTT, rounding division by p)
-//   c_2i = c_i + c',   c_2i+1 = (c_i - c') * x^-k.
-// Batched: blockIdx.y selects the query; each query has its own tree and key.
+// At level a (k = 2^a) the active nodes are i = k .. k + n_active - 1.  Node i
+// computes c' = Subs(c_i, N/k + 1): the Galois permutation of c0 and c1, then
+// key switching of sigma(c1) with the special prime p, and the children
+// c_2i = c_i + c',  c_2i+1 = (c_i - c') * x^-k.  Two launches per level:
+//   k_expand_ks      -- one workgroup per (node, component, limb q|p):
+//                       NTT(sigma(c1)) * key[component][limb], INTT -> X
+//   k_expand_combine -- one workgroup per node: the rounding division by p
+//                       (SEAL switch_key_inplace), sigma(c0), the children.
+// Splitting the key switch four ways keeps each workgroup at two transforms
+// deep, so a level's latency is two transforms, not six.
+// Batched: blockIdx.y selects the query; each query has its own tree, key and
+// X scratch.
 #pragma once
 
 #include "k_ntt.hpp"
@@ -34,60 +39,63 @@ __device__ __forceinline__ uint64_t galois_gather(const uint64_t *in, int j, uin
   return v ? q - v : 0;
 }
 
-// trees: query b's tree at trees + b * tree_stride (2w cts, slot s at s*2N)
+// trees: query b's tree at trees + b*tree_stride (2w cts, slot s at s*2N)
 // keys:  query b's Galois keys at key_ptrs[b]; this level's key at + level*4N,
 //        laid out [comp][limb q,p][N], NTT form
-__global__ __launch_bounds__(kNttThreads) void k_expand_level(uint64_t *trees, uint64_t tree_stride,
-                                                              const uint64_t *const *key_ptrs,
-                                                              uint32_t level, uint32_t ginv,
-                                                              NttTab Tq, NttTab Tp, Mod mq, Mod mp,
-                                                              KsConst ks) {
+// xbuf:  query b's X at xbuf + b*x_stride, [node - first][comp][limb][N]
+__global__ __launch_bounds__(kNttThreads) void k_expand_ks(const uint64_t *trees, uint64_t tree_stride,
+                                                           const uint64_t *const *key_ptrs, uint32_t level,
+                                                           uint32_t ginv, uint64_t *xbuf, uint64_t x_stride,
+                                                           NttTab Tq, NttTab Tp, Mod mq, Mod mp) {
   __shared__ uint64_t lds[kNttLds];
-  const int tid = threadIdx.x;
-  const uint32_t k = 1u << level;
-  const uint64_t node = (uint64_t)k + blockIdx.x;
-  const uint64_t q = Tq.q, p = Tp.q;
-  uint64_t *tree = trees + (size_t)blockIdx.y * tree_stride;
-  const uint64_t *key = key_ptrs[blockIdx.y] + (size_t)level * 4 * kN;
-  const uint64_t *c0 = tree + node * 2 * kN, *c1 = c0 + kN;
-
-  // sigma(c1), NTT mod q and mod p (q < p: the same residues serve both)
-  uint64_t tq[8], tp[8];
+  const uint32_t nl = blockIdx.x >> 2, comp = (blockIdx.x >> 1) & 1, limb = blockIdx.x & 1;
+  const uint64_t node = (1ull << level) + nl;
+  const NttTab &T = limb ? Tp : Tq;
+  const Mod &m = limb ? mp : mq;
+  const uint64_t *c1 = trees + (size_t)blockIdx.y * tree_stride + node * 2 * kN + kN;
+  const uint64_t *key = key_ptrs[blockIdx.y] + (size_t)level * 4 * kN + (size_t)(2 * comp + limb) * kN;
+  uint64_t x[8];
+  // sigma(c1); its residues < q < p serve both limbs unchanged
 #pragma unroll
-  for (int j = 0; j < 8; ++j) tq[j] = tp[j] = galois_gather(c1, tid + 256 * j, ginv, q);
-  ntt_fwd(tq, lds, Tq);
-  ntt_fwd(tp, lds, Tp);
+  for (int j = 0; j < 8; ++j) x[j] = galois_gather(c1, threadIdx.x + 256 * j, ginv, Tq.q);
+  ntt_fwd(x, lds, T);
+  uint64_t kv[8];
+  load_blocked(kv, key);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) x[j] = mulmod_barrett(x[j], kv[j], m);
+  ntt_inv(x, lds, T);
+  uint64_t *dst = xbuf + (size_t)blockIdx.y * x_stride + ((size_t)(nl * 2 + comp) * 2 + limb) * kN;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) dst[threadIdx.x + 256 * j] = ntt_inv_final(x[j], T);
+}
 
-  // Component comp of c' is sigma(c0) + KS_0 (comp 0) or KS_1 (comp 1); the
-  // children's polynomial comp is written as soon as it is known.
+__global__ __launch_bounds__(256) void k_expand_combine(uint64_t *trees, uint64_t tree_stride,
+                                                        const uint64_t *xbuf, uint64_t x_stride,
+                                                        uint32_t level, uint32_t ginv, uint64_t q,
+                                                        uint64_t p, KsConst ks) {
+  const uint32_t nl = blockIdx.x, k = 1u << level;
+  const uint64_t node = (uint64_t)k + nl;
+  uint64_t *tree = trees + (size_t)blockIdx.y * tree_stride;
+  const uint64_t *c0 = tree + node * 2 * kN;
   uint64_t *left = tree + (2 * node) * 2 * kN;       // c_{2i}
   uint64_t *right = tree + (2 * node + 1) * 2 * kN;  // c_{2i+1}
-#pragma unroll 1
+  const uint64_t *X = xbuf + (size_t)blockIdx.y * x_stride + (size_t)nl * 4 * kN;
   for (int comp = 0; comp < 2; ++comp) {
-    const uint64_t *kq = key + (size_t)(2 * comp) * kN, *kp = kq + kN;
-    uint64_t aq[8], ap[8], kv[8];
-    load_blocked(kv, kq);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) aq[j] = mulmod_barrett(tq[j], kv[j], mq);
-    load_blocked(kv, kp);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) ap[j] = mulmod_barrett(tp[j], kv[j], mp);
-    ntt_inv(aq, lds, Tq);
-    ntt_inv(ap, lds, Tp);
-    const uint64_t *cin = comp == 0 ? c0 : c1;
-#pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const int e = tid + 256 * j;
-      const uint64_t xq = ntt_inv_final(aq[j], Tq);
-      const uint64_t xp = ntt_inv_final(ap[j], Tp);
-      uint64_t last = xp + ks.half_p;  // (X_p + floor(p/2)) mod p
+      const int e = threadIdx.x + 256 * j;
+      const uint64_t xq = X[(size_t)(2 * comp) * kN + e];
+      const uint64_t xp = X[(size_t)(2 * comp + 1) * kN + e];
+      // SEAL switch_key_inplace: floor((X + floor(p/2)) / p) mod q, with
+      // X = CRT(X_q, X_p):  (X_q - ((X_p + floor(p/2)) mod p) + floor(p/2)) * p^-1
+      uint64_t last = xp + ks.half_p;
       if (last >= p) last -= p;
       const uint64_t lq = last >= q ? last - q : last;  // p < 2q
       const uint64_t diff = sub_mod(add_mod(xq, ks.half_p_mod_q, q), lq, q);
       uint64_t ksv = mul_shoup_lazy(diff, ks.p_inv, ks.p_inv_s, q);
       if (ksv >= q) ksv -= q;
+      // c'_0 = sigma(c0) + KS_0, c'_1 = KS_1 (c1 was cleared before the key switch)
       const uint64_t cp = comp == 0 ? add_mod(galois_gather(c0, e, ginv, q), ksv, q) : ksv;
-      const uint64_t a = cin[e];
+      const uint64_t a = c0[(size_t)comp * kN + e];
       left[(size_t)comp * kN + e] = add_mod(a, cp, q);
       uint64_t d = sub_mod(a, cp, q);
       // multiply by x^-k: slot e moves to e - k; wrapped slots are negated

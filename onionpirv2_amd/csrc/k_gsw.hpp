@@ -143,13 +143,15 @@ __device__ __forceinline__ void ext_fused_core(const uint64_t *xs, const uint64_
       ntt_fwd(d, lds, T);
       const uint64_t *g = G + (size_t)(poly * l + kk) * 2 * kN;
 #pragma unroll
+      for (int r = 0; r < 8; ++r) d[r] = ntt_fwd_final(d[r], T.q);
+#pragma unroll
       for (int c = 0; c < 2; ++c) {
         uint64_t gv[8];
         load_blocked(gv, g + (size_t)c * kN);
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
           uint64_t plo, phi;
-          mul128(ntt_fwd_final(d[r], T.q), gv[r], plo, phi);
+          mul128(d[r], gv[r], plo, phi);
           add128(lo[c][r], hi[c][r], plo, phi);
         }
       }
@@ -167,15 +169,18 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused(uint64_t *cts, uint64
                                                            const uint64_t *G, uint64_t g_stride, uint32_t l,
                                                            uint32_t base_log2, NttTab T, Mod mq) {
   __shared__ uint64_t lds[kNttLds];
+  __shared__ uint64_t park[kN];  // component 1 waits here during component 0's INTT
   uint64_t *x = cts + (size_t)blockIdx.y * cts_stride + (size_t)blockIdx.x * 2 * kN;
   const uint64_t *y = x + half * 2 * kN;
   uint64_t res[2][8];
   ext_fused_core(x, y, G + (size_t)blockIdx.y * g_stride, l, base_log2, lds, T, mq, res);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) park[8 * threadIdx.x + r] = res[1][r];
 #pragma unroll 1
   for (int c = 0; c < 2; ++c) {
     uint64_t v[8];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = res[c][r];
+    for (int r = 0; r < 8; ++r) v[r] = c == 0 ? res[0][r] : park[8 * threadIdx.x + r];
     ntt_inv(v, lds, T);
 #pragma unroll
     for (int r = 0; r < 8; ++r) {

@@ -61,6 +61,10 @@ struct Tw8 {
 };
 
 __device__ __forceinline__ void load_tw8(Tw8 &t, const uint64_t *tab, const uint64_t *tabs, int m) {
+  // keep each pass's twiddle loads in that pass: without this fence the
+  // scheduler hoists all four passes' loads to the top and the live twiddles
+  // alone take ~110 VGPRs in kernels that run several transforms
+  __builtin_amdgcn_sched_barrier(0);
   t.w[0] = tab[m];
   t.s[0] = tabs[m];
 #pragma unroll

@@ -178,14 +178,21 @@ int check_db(onion_server *s) {
 
 // ---- pipeline stages (nq queries; query b's ciphertext at tree b, slot 1) ------------
 
+// Nodes 2^a .. 2^a + n_active - 1 of every query's tree; X scratch in digits.
+void expand_level(onion_server *s, uint32_t nq, uint32_t a, uint32_t n_active) {
+  k_expand_ks<<<dim3(n_active * 4, nq), kNttThreads, 0, s->st>>>(
+      s->trees, s->tree_words, s->gk_ptrs, a, s->ginv[a], s->digits, s->digit_words, s->Tq, s->Tp, s->mq, s->mp);
+  k_expand_combine<<<dim3(n_active, nq), 256, 0, s->st>>>(s->trees, s->tree_words, s->digits, s->digit_words, a,
+                                                          s->ginv[a], s->I.q, s->I.p, s->ks);
+}
+
 void run_expand(onion_server *s, uint32_t nq) {
   const uint32_t h = s->I.expan_height, u = s->I.useful_cnt;
   for (uint32_t a = 0; a < h; ++a) {
     // active nodes at level a: those whose left-most leaf is < u
     const uint64_t span = s->w >> a;  // leaves under one node
     const uint32_t n_active = (uint32_t)((u + span - 1) / span);
-    k_expand_level<<<dim3(n_active, nq), kNttThreads, 0, s->st>>>(
-        s->trees, s->tree_words, s->gk_ptrs, a, s->ginv[a], s->Tq, s->Tp, s->mq, s->mp, s->ks);
+    expand_level(s, nq, a, n_active);
   }
 }
 
@@ -410,7 +417,8 @@ int onion_gpu_create(const onion_params *params, onion_server **out) {
   s->fd_words = (uint64_t)P.other_dim * ct;
   const uint64_t conv_digits = (uint64_t)nd * P.gsw_l * 2 * P.gsw_l_key * N;
   const uint64_t mux_digits = (uint64_t)std::max(1u, P.other_dim / 2) * 2 * P.gsw_l * N;
-  s->digit_words = std::max(conv_digits, mux_digits);
+  const uint64_t expand_x = (uint64_t)((I.useful_cnt + 1) / 2) * 4 * N;  // widest level's X
+  s->digit_words = std::max({conv_digits, mux_digits, expand_x});
   s->io_words = std::max<uint64_t>({(uint64_t)I.useful_cnt * ct, (uint64_t)P.other_dim * ct,
                                     (uint64_t)std::max(1u, nd) * 2 * P.gsw_l * ct, 64 * ct,
                                     (uint64_t)P.fst_dim * ct});
@@ -691,8 +699,7 @@ int onion_gpu_tap_galois(onion_server *s, uint64_t id, uint32_t level, const uin
   // place ct at node 2^level; its left child is ct + Subs(ct), so Subs(ct) = left - ct
   const uint64_t node = uint64_t(1) << level;
   HIPCHK(hipMemcpyAsync(s->trees + node * 2 * kN, ct, 2 * kN * 8, hipMemcpyHostToDevice, s->st));
-  k_expand_level<<<dim3(1, 1), kNttThreads, 0, s->st>>>(s->trees, s->tree_words, s->gk_ptrs, level,
-                                                       s->ginv[level], s->Tq, s->Tp, s->mq, s->mp, s->ks);
+  expand_level(s, 1, level, 1);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, s->trees + 2 * node * 2 * kN, 2 * kN * 8, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
