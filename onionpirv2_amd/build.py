@@ -54,7 +54,24 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    build_cpp_tests(verbose)
     return LIB
+
+
+def build_cpp_tests(verbose: bool = False) -> None:
+    """C++ programs under tests/cpp that use the drop-in API (libonionpir.so)."""
+    src_dir = os.path.join(ROOT, "tests", "cpp")
+    out_dir = os.path.join(src_dir, "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    for src in sorted(os.listdir(src_dir)):
+        if not src.endswith(".cpp"):
+            continue
+        exe = os.path.join(out_dir, src[:-4])
+        cmd = [hipcc(), "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), os.path.join(src_dir, src),
+               "-o", exe, "-L", LIB_DIR, "-lonionpir", "-Wl,-rpath,$ORIGIN/../../../onionpirv2_amd/_lib"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
 
 
 if __name__ == "__main__":

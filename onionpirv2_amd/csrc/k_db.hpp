@@ -28,31 +28,36 @@ __device__ __forceinline__ void ntt_and_scatter(uint64_t (&x)[8], uint64_t *lds,
   for (int r = 0; r < 8; ++r) db[(size_t)(8 * threadIdx.x + r) * num_pt + pt] = ntt_fwd_final(x[r], T.q);
 }
 
+// Byte b of the seeded entry of plaintext pt: an 8-byte big-endian entry
+// index (utils::writeIdxToEntry, src/utils.cpp:98-104), then random bytes.
+__device__ __forceinline__ uint32_t entry_byte(uint64_t key, uint64_t pt, uint32_t b) {
+  return b < 8 ? (uint32_t)((pt >> (8 * (7 - b))) & 0xFF)
+               : (uint32_t)((rand64_keyed(key, pt * kN + b / 8) >> (8 * (b % 8))) & 0xFF);
+}
+
 // Seeded generation, one workgroup per plaintext pt_lo + blockIdx.x.
 // mode 0: coefficient e = floor(r * t / 2^64), r = rand64(key, pt*N + e)
-// mode 1: entry bytes (8-byte big-endian entry index, then random bytes
-//         r = rand64(key, pt*N + b/8) byte b%8), 16 bits per coefficient.
+// mode 1: the entry bytes above packed bpc bits per coefficient, LSB first
+//         (push_database_chunk, src/server.cpp:635-701); bpc <= 24.
 __global__ __launch_bounds__(kNttThreads) void k_db_generate(uint64_t *db, uint64_t pt_lo,
                                                              uint64_t num_pt, uint64_t key,
-                                                             int mode, LiftConst L, NttTab T) {
+                                                             int mode, uint32_t bpc, LiftConst L,
+                                                             NttTab T) {
   __shared__ uint64_t lds[kNttLds];
   const uint64_t pt = pt_lo + blockIdx.x;
+  const uint32_t entry_bytes = bpc * kN / 8;
   uint64_t x[8];
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
-    const int e = threadIdx.x + 256 * r;
+    const uint32_t e = threadIdx.x + 256 * r;
     uint64_t v;
     if (mode == 0) {
       v = mulhi64(rand64_keyed(key, pt * kN + e), L.t);
     } else {
-      uint32_t bytes[2];
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const uint32_t b = 2 * e + h;
-        bytes[h] = b < 8 ? (uint32_t)((pt >> (8 * (7 - b))) & 0xFF)
-                         : (uint32_t)((rand64_keyed(key, pt * kN + b / 8) >> (8 * (b % 8))) & 0xFF);
-      }
-      v = bytes[0] | (bytes[1] << 8);
+      const uint32_t bit = e * bpc, b0 = bit / 8;
+      uint32_t window = 0;
+      for (uint32_t h = 0; h < 4 && b0 + h < entry_bytes; ++h) window |= entry_byte(key, pt, b0 + h) << (8 * h);
+      v = (window >> (bit % 8)) & ((1u << bpc) - 1);
     }
     x[r] = plain_lift(v, L);
   }

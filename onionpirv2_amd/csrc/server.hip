@@ -459,14 +459,15 @@ int onion_gpu_param_info(const onion_server *s, onion_param_info *out) {
 int onion_gpu_db_generate(onion_server *s, uint64_t seed, int mode) {
   if (!s) return fail(ONION_EINVAL, "null server");
   if (mode != 0 && mode != 1) return fail(ONION_EINVAL, "mode must be 0 or 1");
-  if (mode == 1 && s->P.plain_bits != 17)
-    return fail(ONION_EINVAL, "entry mode needs 16 bits per coefficient (plain_bits 17)");
+  if (mode == 1 && s->P.plain_bits > 25)
+    return fail(ONION_EINVAL, "entry mode packs at most 24 bits per coefficient");
   std::lock_guard<std::mutex> lk(s->mu);
   const uint64_t key = stream_key(seed, 0x4442ULL);
   const uint64_t chunk = 65535;
   for (uint64_t lo = 0; lo < s->I.num_pt; lo += chunk) {
     const uint64_t n = std::min(chunk, s->I.num_pt - lo);
-    k_db_generate<<<(uint32_t)n, kNttThreads, 0, s->st>>>(s->db, lo, s->I.num_pt, key, mode, s->lift, s->Tq);
+    k_db_generate<<<(uint32_t)n, kNttThreads, 0, s->st>>>(s->db, lo, s->I.num_pt, key, mode, s->P.plain_bits - 1,
+                                                          s->lift, s->Tq);
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->st));

@@ -4,7 +4,7 @@ used to check retrieved records without reading the database back.
 mode 0: coefficient e of plaintext pt = floor(r * t / 2^64),
         r = rand64(seed, stream 0x4442, pt*N + e)
 mode 1: entry bytes -- 8-byte big-endian index, then bytes of
-        rand64(seed, 0x4442, pt*N + b/8); 16 bits per coefficient.
+        rand64(seed, 0x4442, pt*N + b/8); bit_count(t)-1 bits per coefficient.
 """
 from __future__ import annotations
 
@@ -51,8 +51,11 @@ def plaintext(seed: int, mode: int, pt: int, t: int = 65537, N: int = 2048) -> n
     if mode == 0:
         r = rand64(key, np.arange(N, dtype=np.uint64) + np.uint64(pt * N))
         return np.array([(int(v) * t) >> 64 for v in r], dtype=np.uint64)
-    nbytes = 2 * N
-    r = rand64(key, np.arange(nbytes // 8, dtype=np.uint64) + np.uint64(pt * N))
-    b = r.view(np.uint8).copy()
+    bpc = t.bit_length() - 1
+    nbytes = bpc * N // 8
+    r = rand64(key, np.arange((nbytes + 7) // 8, dtype=np.uint64) + np.uint64(pt * N))
+    b = r.view(np.uint8)[:nbytes].copy()
     b[:8] = np.frombuffer(int(pt).to_bytes(8, "big"), dtype=np.uint8)
-    return (b[0::2].astype(np.uint64) | (b[1::2].astype(np.uint64) << np.uint64(8)))
+    bits = int.from_bytes(b.tobytes(), "little")
+    mask = (1 << bpc) - 1
+    return np.array([(bits >> (bpc * i)) & mask for i in range(N)], dtype=np.uint64)
