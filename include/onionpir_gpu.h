@@ -82,7 +82,9 @@ typedef struct onion_tuning {
                             dimension kernel (multiple of 32, 32..1024)       */
   uint32_t fd_tile_rows; /* thread rows of the batched first-dimension tile
                             (TR, 2 or 4)                                      */
-  uint32_t reserved;
+  uint32_t fd_b1_variant; /* single-query first-dimension loads: bit 0 = 8
+                             (else 4) 16-byte loads in flight per lane, bit 1
+                             = nontemporal loads                             */
 } onion_tuning;
 
 typedef struct onion_server onion_server;

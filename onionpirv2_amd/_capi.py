@@ -39,7 +39,7 @@ class OnionStageTimes(C.Structure):
 
 class OnionTuning(C.Structure):
     _fields_ = [("fuse_min", C.c_uint32), ("fd_rows_b1", C.c_uint32), ("fd_tile_rows", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("fd_b1_variant", C.c_uint32)]
 
 
 class OnionError(RuntimeError):

@@ -99,15 +99,31 @@ __device__ __forceinline__ void split30(uint64_t a, uint32_t &a0, uint32_t &a1) 
   a1 = (uint32_t)(a >> 30);
 }
 
+// 16-byte streaming load; NT marks it nontemporal (the database is read once
+// per query, so it should not displace the L2-resident selection vector).
+template <bool NT>
+__device__ __forceinline__ uint4 stream_load(const uint4 *p) {
+  if constexpr (NT) {
+    const uint64_t *w = reinterpret_cast<const uint64_t *>(p);
+    const uint64_t lo = __builtin_nontemporal_load(w), hi = __builtin_nontemporal_load(w + 1);
+    return make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+  } else {
+    return *p;
+  }
+}
+
 // Single query (B = 1, Qs with cpad = 2).  grid.x = slot, grid.y = row chunk
 // of `rows_per_block` rows; 256 threads.  Lane layout per wave: 8 rows x 8
 // column groups; a lane reads 16 B (two adjacent words) per step, so 8 lanes
-// cover 128 contiguous bytes of one row.  Requires fst % 16 == 0.
+// cover 128 contiguous bytes of one row; UNROLL steps are in flight per lane.
+// Requires fst % 16 == 0.
+template <int UNROLL, bool NT>
 __global__ __launch_bounds__(256) void k_first_dim_b1(const uint64_t *__restrict__ db,
                                                       const uint32_t *__restrict__ Qs,
                                                       uint64_t *__restrict__ out, uint32_t fst,
                                                       uint32_t other, uint64_t num_pt,
                                                       uint32_t rows_per_block, Mod mq) {
+  static_assert(UNROLL == 4 || UNROLL == 8, "fold cadence assumes 4 or 8 steps");
   __shared__ uint4 qs[kMaxFst];  // per column j: {b0_lo, b0_hi, b1_lo, b1_hi}
   const uint32_t slot = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -126,12 +142,13 @@ __global__ __launch_bounds__(256) void k_first_dim_b1(const uint64_t *__restrict
     const uint4 *row = reinterpret_cast<const uint4 *>(A + (size_t)(active ? r : r_begin) * fst);
     Acc3 x0 = {0, 0, 0}, x1 = {0, 0, 0};
     uint64_t w0lo = 0, w0hi = 0, w1lo = 0, w1hi = 0;
-    for (uint32_t k = 0; k < steps; k += 4) {
-      uint4 v[4];
+    for (uint32_t k = 0; k < steps; k += UNROLL) {
+      uint4 v[UNROLL];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) v[u] = (k + u < steps) ? row[(k + u) * 8 + cg] : make_uint4(0, 0, 0, 0);
+      for (int u = 0; u < UNROLL; ++u)
+        v[u] = (k + u < steps) ? stream_load<NT>(row + (k + u) * 8 + cg) : make_uint4(0, 0, 0, 0);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < UNROLL; ++u) {
         const uint32_t j = (k + u) * 16 + cg * 2;
         const uint4 qa = qs[min(j, fst - 1)], qb = qs[min(j + 1, fst - 1)];
         uint32_t a0, a1;
@@ -141,9 +158,12 @@ __global__ __launch_bounds__(256) void k_first_dim_b1(const uint64_t *__restrict
         split30(((uint64_t)v[u].w << 32) | v[u].z, a0, a1);
         acc_word(x0, a0, a1, qb.x, qb.y);
         acc_word(x1, a0, a1, qb.z, qb.w);
+        // s1 takes two products per word and a step is two words: fold every 4 steps
+        if ((u & 3) == 3) {
+          fold(w0lo, w0hi, x0);
+          fold(w1lo, w1hi, x1);
+        }
       }
-      fold(w0lo, w0hi, x0);
-      fold(w1lo, w1hi, x1);
     }
     // reduce the 8 column groups of this row (lanes differing in bits 0..2)
 #pragma unroll
@@ -235,7 +255,7 @@ __global__ __launch_bounds__(256) void k_first_dim_bat(const uint64_t *__restric
     for (int idx = tid; idx < RBLK * KC; idx += 256) {
       const int rr = idx / KC, kk = idx % KC;
       const uint32_t r = row0 + rr, j = k0 + kk;
-      const uint64_t v = (r < other && j < fst) ? A[(size_t)r * fst + j] : 0;
+      const uint64_t v = (r < other && j < fst) ? __builtin_nontemporal_load(A + (size_t)r * fst + j) : 0;
       As[rr][kk] = make_uint2((uint32_t)v & kMask30, (uint32_t)(v >> 30));
     }
     for (int idx = tid; idx < KC * CPAD; idx += 256) {

@@ -230,7 +230,9 @@ def test_answer_batch_matches_single_and_oracle(cfg1, nq):
 
 
 @pytest.mark.parametrize("tuning", [dict(fuse_min=1), dict(fuse_min=1 << 30),
-                                    dict(fd_rows_b1=64), dict(fd_tile_rows=2)])
+                                    dict(fd_rows_b1=64), dict(fd_tile_rows=2),
+                                    dict(fd_b1_variant=1), dict(fd_b1_variant=2),
+                                    dict(fd_b1_variant=3)])
 def test_tuning_choices_are_bit_identical(cfg1, tuning):
     """Every launch-shape choice (fused vs split external products, first-dim
     tile shapes) gives the oracle's bits, for a single query and a batch."""
