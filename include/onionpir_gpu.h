@@ -85,6 +85,8 @@ typedef struct onion_tuning {
   uint32_t fd_b1_variant; /* single-query first-dimension loads: bit 0 = 8
                              (else 4) 16-byte loads in flight per lane, bit 1
                              = nontemporal loads                             */
+  uint32_t use_graph;     /* 1: each lane captures its per-batch pipeline in a
+                             hipGraph once per batch size and replays it      */
 } onion_tuning;
 
 typedef struct onion_server onion_server;
@@ -150,6 +152,21 @@ int onion_gpu_last_stage_times(const onion_server *srv, onion_stage_times *out);
 
 int onion_gpu_get_tuning(const onion_server *srv, onion_tuning *out);
 int onion_gpu_set_tuning(onion_server *srv, const onion_tuning *tuning);
+
+/* Concurrent pipelines (no reference counterpart: the reference server is
+ * single-threaded).  A server owns n_lanes lanes; each lane has its own HIP
+ * stream and scratch, so batches on different lanes overlap on the device.
+ * Lane 0 serves every other entry point of this header.  Key and database
+ * changes wait for batches in flight on any lane. */
+int onion_gpu_set_lanes(onion_server *srv, uint32_t n_lanes);
+int onion_gpu_answer_batch_lane(onion_server *srv, uint32_t lane, uint32_t n, const uint64_t *client_ids,
+                                const uint64_t *queries, uint64_t *resps, onion_stage_times *times);
+/* Device bytes of per-query scratch a lane allocates for each query of its
+ * largest batch (the unit of admission by device-memory budget). */
+int onion_gpu_scratch_bytes_per_query(const onion_server *srv, uint64_t *bytes);
+/* Allocate lane `lane`'s scratch for batches of up to n_queries (capped at 64)
+ * now rather than on first use; fails with ONION_ENOMEM when it does not fit. */
+int onion_gpu_reserve(onion_server *srv, uint32_t lane, uint32_t n_queries);
 
 /* ---- stage taps: each runs one stage of the device path on host-given
  * inputs, for parity tests against the CPU oracle -------------------------- */

@@ -39,7 +39,7 @@ class OnionStageTimes(C.Structure):
 
 class OnionTuning(C.Structure):
     _fields_ = [("fuse_min", C.c_uint32), ("fd_rows_b1", C.c_uint32), ("fd_tile_rows", C.c_uint32),
-                ("fd_b1_variant", C.c_uint32)]
+                ("fd_b1_variant", C.c_uint32), ("use_graph", C.c_uint32)]
 
 
 class OnionError(RuntimeError):
@@ -78,6 +78,11 @@ _SIGS = {
     "onion_gpu_last_stage_times": (C.c_int, [C.c_void_p, C.POINTER(OnionStageTimes)]),
     "onion_gpu_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(OnionTuning)]),
     "onion_gpu_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(OnionTuning)]),
+    "onion_gpu_set_lanes": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "onion_gpu_answer_batch_lane": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, U64P, U64P, U64P,
+                                              C.POINTER(OnionStageTimes)]),
+    "onion_gpu_scratch_bytes_per_query": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "onion_gpu_reserve": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
     "onion_gpu_tap_expand": (C.c_int, [C.c_void_p, C.c_uint64, U64P, U64P]),
     "onion_gpu_tap_convert": (C.c_int, [C.c_void_p, C.c_uint64, U64P, U64P]),
     "onion_gpu_tap_first_dim": (C.c_int, [C.c_void_p, U64P, U64P]),

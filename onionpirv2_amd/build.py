@@ -16,7 +16,7 @@ LIB_DIR = os.path.join(PKG, "_lib")
 LIB = os.path.join(LIB_DIR, "libonionpir.so")
 
 HIP_SOURCES = ["server.hip"]
-CXX_SOURCES = ["client.cpp", "facade.cpp"]
+CXX_SOURCES = ["client.cpp", "facade.cpp", "serve.cpp"]
 
 # packed-fp32 instructions are disabled for every device compilation
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -1,17 +1,27 @@
 // server.hip -- the C ABI of include/onionpir_gpu.h: device-resident
 // database, per-client key store, and the answer pipeline
 //   expand -> convert -> first dimension -> other dimensions -> mod switch
-// (PirServer::make_query, src/server.cpp:520-579) as a chain of HIP kernels on
-// one stream.  Every stage is batched: grid.y indexes the query, each query
-// has its own scratch slice and its own client keys, and the first dimension
-// reads the database once for the whole batch.
+// (PirServer::make_query, src/server.cpp:520-579) as a chain of HIP kernels.
+// Every stage is batched: grid.y indexes the query, each query has its own
+// scratch slice and its own client keys, and the first dimension reads the
+// database once for the whole batch.
+//
+// Concurrency: a server owns one or more lanes.  A lane is a pipeline -- its
+// own HIP stream, events, per-query scratch and (optionally) the hipGraphs of
+// its pipeline, one per batch size -- so batches on different lanes overlap on
+// the device while they share the database, the tables and the key store.
+// A writer-preferring gate orders the two kinds of call: batches hold it
+// shared, database / key / tuning / lane changes hold it exclusively.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 
@@ -48,14 +58,62 @@ struct ClientKeys {
 };
 
 constexpr uint32_t kMaxBatch = 64;  // queries per pipeline pass
+constexpr uint32_t kMaxLanes = 16;
+
+// Shared/exclusive gate in which a waiting writer goes ahead of readers that
+// arrive after it, so a stream of overlapping batches cannot starve a key
+// upload.
+class Gate {
+  std::mutex m_;
+  std::condition_variable cv_;
+  int readers_ = 0, writers_waiting_ = 0;
+  bool writer_ = false;
+
+ public:
+  void lock_shared() {
+    std::unique_lock<std::mutex> g(m_);
+    cv_.wait(g, [&] { return !writer_ && writers_waiting_ == 0; });
+    ++readers_;
+  }
+  void unlock_shared() {
+    std::lock_guard<std::mutex> g(m_);
+    if (--readers_ == 0) cv_.notify_all();
+  }
+  void lock() {
+    std::unique_lock<std::mutex> g(m_);
+    ++writers_waiting_;
+    cv_.wait(g, [&] { return !writer_ && readers_ == 0; });
+    --writers_waiting_;
+    writer_ = true;
+  }
+  void unlock() {
+    std::lock_guard<std::mutex> g(m_);
+    writer_ = false;
+    cv_.notify_all();
+  }
+};
 
 }  // namespace
+
+// One pipeline: stream, events, per-query scratch for `cap` queries, the
+// device arrays of per-query key pointers, and the graphs captured for it.
+struct Lane {
+  hipStream_t st = nullptr;
+  hipEvent_t ev[8]{};
+  uint32_t cap = 0;
+  uint64_t *trees = nullptr, *rgsw = nullptr, *fd = nullptr, *digits = nullptr;
+  uint32_t *Qs = nullptr;  // N x fst x cpad x 2 uint32 (cpad for `cap` queries)
+  const uint64_t **gk_ptrs = nullptr, **rk_ptrs = nullptr;  // device [cap]
+  std::vector<const uint64_t *> h_gk, h_rk;
+  std::map<uint32_t, hipGraphExec_t> graphs;  // by batch size
+  uint64_t graph_epoch = 0;                   // tuning epoch the graphs were captured under
+  std::mutex mu;                              // one batch at a time per lane
+};
 
 struct onion_server {
   onion_params P{};
   onion_param_info I{};
   int device = 0;
-  hipStream_t st = nullptr;
   uint64_t w = 0;  // 2^h
   // device NTT tables: q then p, each rp rps irp irps (4N words)
   uint64_t *tabs = nullptr;
@@ -72,25 +130,25 @@ struct onion_server {
   uint64_t n_written = 0;
   // keys
   std::map<uint64_t, ClientKeys> clients;
-  // per-query scratch, `cap` queries; strides in words
-  uint32_t cap = 0;
+  // per-query scratch strides in words
   uint64_t tree_words = 0, rgsw_words = 0, fd_words = 0, digit_words = 0;
-  uint64_t *trees = nullptr, *rgsw = nullptr, *fd = nullptr, *digits = nullptr;
-  uint32_t *Qs = nullptr;  // N x fst x cpad x 2 uint32 (cpad for `cap` queries)
-  const uint64_t **gk_ptrs = nullptr, **rk_ptrs = nullptr;  // device [cap]
-  std::vector<const uint64_t *> h_gk, h_rk;
-  uint64_t *io = nullptr;  // staging
+  std::vector<std::unique_ptr<Lane>> lanes;  // lane 0 also serves taps and ingest
+  uint64_t *io = nullptr;                    // staging for taps and ingest (lane 0)
   size_t io_words = 0;
   uint8_t *pack = nullptr;
-  hipEvent_t ev[8]{};
-  onion_stage_times last{};
   // measured on MI355X at 512x1024: 8 nontemporal 16-byte loads in flight per
   // lane and 256-row blocks stream the database at 94% of a plain read kernel
-  onion_tuning tune{1024, 256, 4, 3};
-  std::mutex mu;
+  onion_tuning tune{1024, 256, 4, 3, 0};
+  uint64_t tune_epoch = 1;
+  Gate gate;
+  mutable std::mutex last_mu;
+  onion_stage_times last{};
 };
 
 namespace {
+
+using Shared = std::shared_lock<Gate>;
+using Exclusive = std::unique_lock<Gate>;
 
 template <typename T>
 int dalloc(T **p, size_t count) {
@@ -116,38 +174,71 @@ uint32_t cpad_for(uint32_t nq) {
   return 128;
 }
 
-void free_scratch(onion_server *s) {
-  void *bufs[] = {s->trees, s->rgsw, s->fd, s->digits, s->Qs, s->gk_ptrs, s->rk_ptrs};
-  for (void *b : bufs) dfree(b);
-  s->trees = s->rgsw = s->fd = s->digits = nullptr;
-  s->Qs = nullptr;
-  s->gk_ptrs = s->rk_ptrs = nullptr;
-  s->cap = 0;
+void drop_graphs(Lane *L) {
+  for (auto &kv : L->graphs) (void)hipGraphExecDestroy(kv.second);
+  L->graphs.clear();
 }
 
-// Scratch for `cap` queries (kept when large enough).
-int ensure_scratch(onion_server *s, uint32_t cap) {
-  if (s->cap >= cap) return ONION_OK;
-  (void)hipStreamSynchronize(s->st);
-  free_scratch(s);
-  const uint64_t N = kN, ct = 2 * N;
+void free_scratch(Lane *L) {
+  drop_graphs(L);
+  void *bufs[] = {L->trees, L->rgsw, L->fd, L->digits, L->Qs, L->gk_ptrs, L->rk_ptrs};
+  for (void *b : bufs) dfree(b);
+  L->trees = L->rgsw = L->fd = L->digits = nullptr;
+  L->Qs = nullptr;
+  L->gk_ptrs = L->rk_ptrs = nullptr;
+  L->cap = 0;
+}
+
+// Scratch of lane L for `cap` queries (kept when large enough).
+int ensure_scratch(const onion_server *s, Lane *L, uint32_t cap) {
+  if (L->cap >= cap) return ONION_OK;
+  (void)hipStreamSynchronize(L->st);
+  free_scratch(L);
   int rc;
-  if ((rc = dalloc(&s->trees, cap * s->tree_words)) || (rc = dalloc(&s->rgsw, cap * s->rgsw_words)) ||
-      (rc = dalloc(&s->fd, cap * s->fd_words)) || (rc = dalloc(&s->digits, cap * s->digit_words)) ||
-      (rc = dalloc(&s->Qs, N * s->P.fst_dim * cpad_for(cap) * 2)) || (rc = dalloc(&s->gk_ptrs, cap)) ||
-      (rc = dalloc(&s->rk_ptrs, cap))) {
+  if ((rc = dalloc(&L->trees, cap * s->tree_words)) || (rc = dalloc(&L->rgsw, cap * s->rgsw_words)) ||
+      (rc = dalloc(&L->fd, cap * s->fd_words)) || (rc = dalloc(&L->digits, cap * s->digit_words)) ||
+      (rc = dalloc(&L->Qs, (size_t)kN * s->P.fst_dim * cpad_for(cap) * 2)) || (rc = dalloc(&L->gk_ptrs, cap)) ||
+      (rc = dalloc(&L->rk_ptrs, cap))) {
     std::string msg = g_err;
-    free_scratch(s);
+    free_scratch(L);
     return fail(rc, msg + " (batch scratch for " + std::to_string(cap) + " queries)");
   }
-  (void)ct;
-  s->cap = cap;
-  s->h_gk.assign(cap, nullptr);
-  s->h_rk.assign(cap, nullptr);
+  L->cap = cap;
+  L->h_gk.assign(cap, nullptr);
+  L->h_rk.assign(cap, nullptr);
   return ONION_OK;
 }
 
-int lookup(onion_server *s, uint64_t id, ClientKeys &out, bool need_rk) {
+void destroy_lane(Lane *L) {
+  if (L->st) (void)hipStreamSynchronize(L->st);
+  free_scratch(L);
+  for (auto &e : L->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (L->st) (void)hipStreamDestroy(L->st);
+  L->st = nullptr;
+}
+
+int make_lane(std::unique_ptr<Lane> &out) {
+  auto L = std::make_unique<Lane>();
+  bool ok = hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking) == hipSuccess;
+  for (auto &e : L->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+  if (!ok) {
+    destroy_lane(L.get());
+    return fail(ONION_EHIP, "lane stream or event creation failed");
+  }
+  out = std::move(L);
+  return ONION_OK;
+}
+
+// Device bytes of lane scratch per query of a kMaxBatch-query lane (the
+// selection tile Qs is shared by the batch and counted pro rata).
+uint64_t scratch_bytes_per_query(const onion_server *s) {
+  const uint64_t words = s->tree_words + s->rgsw_words + s->fd_words + s->digit_words;
+  const uint64_t qs = (uint64_t)kN * s->P.fst_dim * cpad_for(kMaxBatch) * 2 * 4;
+  return words * 8 + 2 * sizeof(void *) + (qs + kMaxBatch - 1) / kMaxBatch;
+}
+
+int lookup(const onion_server *s, uint64_t id, ClientKeys &out, bool need_rk) {
   auto it = s->clients.find(id);
   if (it == s->clients.end() || !it->second.gk)
     return fail(ONION_ENOKEY, "no Galois keys for client " + std::to_string(id));
@@ -157,21 +248,21 @@ int lookup(onion_server *s, uint64_t id, ClientKeys &out, bool need_rk) {
   return ONION_OK;
 }
 
-// Resolve the keys of n clients into the device pointer arrays.
-int bind_keys(onion_server *s, uint32_t n, const uint64_t *ids, bool need_rk) {
+// Resolve the keys of n clients into lane L's device pointer arrays.
+int bind_keys(const onion_server *s, Lane *L, uint32_t n, const uint64_t *ids, bool need_rk) {
   for (uint32_t i = 0; i < n; ++i) {
     ClientKeys ck;
     int rc = lookup(s, ids[i], ck, need_rk);
     if (rc) return rc;
-    s->h_gk[i] = ck.gk;
-    s->h_rk[i] = ck.rk;
+    L->h_gk[i] = ck.gk;
+    L->h_rk[i] = ck.rk;
   }
-  HIPCHK(hipMemcpyAsync(s->gk_ptrs, s->h_gk.data(), n * sizeof(void *), hipMemcpyHostToDevice, s->st));
-  HIPCHK(hipMemcpyAsync(s->rk_ptrs, s->h_rk.data(), n * sizeof(void *), hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemcpyAsync(L->gk_ptrs, L->h_gk.data(), n * sizeof(void *), hipMemcpyHostToDevice, L->st));
+  HIPCHK(hipMemcpyAsync(L->rk_ptrs, L->h_rk.data(), n * sizeof(void *), hipMemcpyHostToDevice, L->st));
   return ONION_OK;
 }
 
-int check_db(onion_server *s) {
+int check_db(const onion_server *s) {
   if (!s->db || s->n_written != s->I.num_pt)
     return fail(ONION_ESTATE, "database incomplete: " + std::to_string(s->n_written) + " of " +
                                   std::to_string(s->I.num_pt) + " plaintexts written");
@@ -181,157 +272,233 @@ int check_db(onion_server *s) {
 // ---- pipeline stages (nq queries; query b's ciphertext at tree b, slot 1) ------------
 
 // Nodes 2^a .. 2^a + n_active - 1 of every query's tree; X scratch in digits.
-void expand_level(onion_server *s, uint32_t nq, uint32_t a, uint32_t n_active) {
-  k_expand_ks<<<dim3(n_active * 4, nq), kNttThreads, 0, s->st>>>(
-      s->trees, s->tree_words, s->gk_ptrs, a, s->ginv[a], s->digits, s->digit_words, s->Tq, s->Tp, s->mq, s->mp);
-  k_expand_combine<<<dim3(n_active, nq), 256, 0, s->st>>>(s->trees, s->tree_words, s->digits, s->digit_words, a,
+void expand_level(const onion_server *s, Lane *L, uint32_t nq, uint32_t a, uint32_t n_active) {
+  k_expand_ks<<<dim3(n_active * 4, nq), kNttThreads, 0, L->st>>>(
+      L->trees, s->tree_words, L->gk_ptrs, a, s->ginv[a], L->digits, s->digit_words, s->Tq, s->Tp, s->mq, s->mp);
+  k_expand_combine<<<dim3(n_active, nq), 256, 0, L->st>>>(L->trees, s->tree_words, L->digits, s->digit_words, a,
                                                           s->ginv[a], s->I.q, s->I.p, s->ks);
 }
 
-void run_expand(onion_server *s, uint32_t nq) {
+void run_expand(const onion_server *s, Lane *L, uint32_t nq) {
   const uint32_t h = s->I.expan_height, u = s->I.useful_cnt;
   for (uint32_t a = 0; a < h; ++a) {
     // active nodes at level a: those whose left-most leaf is < u
     const uint64_t span = s->w >> a;  // leaves under one node
     const uint32_t n_active = (uint32_t)((u + span - 1) / span);
-    expand_level(s, nq, a, n_active);
+    expand_level(s, L, nq, a, n_active);
   }
 }
 
 // Conversion of the (ndim-1)*l RGSW leaves starting at conv (stride per query).
-void run_convert(onion_server *s, uint32_t nq, const uint64_t *conv, uint64_t conv_stride) {
+void run_convert(const onion_server *s, Lane *L, uint32_t nq, const uint64_t *conv, uint64_t conv_stride) {
   const uint32_t nd = s->I.ndim - 1, l = s->P.gsw_l, lk = s->P.gsw_l_key;
   if (nd == 0) return;
   const uint32_t n_ct = nd * l;
-  k_rgsw_top_rows<<<dim3(n_ct * 2, nq), kNttThreads, 0, s->st>>>(conv, conv_stride, s->rgsw,
-                                                                 s->rgsw_words, l, s->Tq);
+  k_rgsw_top_rows<<<dim3(n_ct * 2, nq), kNttThreads, 0, L->st>>>(conv, conv_stride, L->rgsw, s->rgsw_words, l,
+                                                                 s->Tq);
   if ((uint64_t)n_ct * nq >= s->tune.fuse_min) {
-    k_convert_fused<<<dim3(n_ct, nq), kNttThreads, 0, s->st>>>(conv, conv_stride, s->rk_ptrs, lk,
-                                                               s->I.base_log2_key, s->rgsw, s->rgsw_words, l,
+    k_convert_fused<<<dim3(n_ct, nq), kNttThreads, 0, L->st>>>(conv, conv_stride, L->rk_ptrs, lk,
+                                                               s->I.base_log2_key, L->rgsw, s->rgsw_words, l,
                                                                s->Tq, s->mq);
     return;
   }
-  k_decomp_ntt<<<dim3(n_ct * 2 * lk, nq), kNttThreads, 0, s->st>>>(
-      conv, nullptr, conv_stride, s->digits, s->digit_words, lk, s->I.base_log2_key, s->Tq);
-  k_ext_mac_store<<<dim3(n_ct * 2, nq), kNttThreads, 0, s->st>>>(
-      s->digits, s->digit_words, s->rk_ptrs, 2 * lk, s->rgsw, s->rgsw_words, l, s->mq);
+  k_decomp_ntt<<<dim3(n_ct * 2 * lk, nq), kNttThreads, 0, L->st>>>(conv, nullptr, conv_stride, L->digits,
+                                                                   s->digit_words, lk, s->I.base_log2_key, s->Tq);
+  k_ext_mac_store<<<dim3(n_ct * 2, nq), kNttThreads, 0, L->st>>>(L->digits, s->digit_words, L->rk_ptrs, 2 * lk,
+                                                                 L->rgsw, s->rgsw_words, l, s->mq);
 }
 
 template <int TR, int CPAD>
-void launch_fd_bat_tr(onion_server *s, uint32_t nq) {
+void launch_fd_bat_tr(const onion_server *s, Lane *L, uint32_t nq) {
   constexpr int RBLK = (256 / (CPAD / 4)) * TR;
   const uint32_t n_tiles = (s->P.other_dim + RBLK - 1) / RBLK;
-  k_first_dim_bat<TR, 4, CPAD><<<kN * n_tiles, 256, 0, s->st>>>(
-      s->db, s->Qs, s->fd, s->fd_words, s->P.fst_dim, s->P.other_dim, s->I.num_pt, nq, n_tiles, s->mq);
+  k_first_dim_bat<TR, 4, CPAD><<<kN * n_tiles, 256, 0, L->st>>>(
+      s->db, L->Qs, L->fd, s->fd_words, s->P.fst_dim, s->P.other_dim, s->I.num_pt, nq, n_tiles, s->mq);
 }
 
 template <int CPAD>
-void launch_fd_bat(onion_server *s, uint32_t nq) {
+void launch_fd_bat(const onion_server *s, Lane *L, uint32_t nq) {
   if (s->tune.fd_tile_rows == 2)
-    launch_fd_bat_tr<2, CPAD>(s, nq);
+    launch_fd_bat_tr<2, CPAD>(s, L, nq);
   else
-    launch_fd_bat_tr<4, CPAD>(s, nq);
+    launch_fd_bat_tr<4, CPAD>(s, L, nq);
 }
 
 // First dimension: fst selection cts per query (coefficient form, stride
-// sel_stride) -> s->fd (other cts per query, coefficient form).
-void run_first_dim(onion_server *s, uint32_t nq, const uint64_t *sel, uint64_t sel_stride, bool time_core) {
+// sel_stride) -> L->fd (other cts per query, coefficient form).
+void run_first_dim(const onion_server *s, Lane *L, uint32_t nq, const uint64_t *sel, uint64_t sel_stride,
+                   bool time_core) {
   const uint32_t fst = s->P.fst_dim, other = s->P.other_dim, cpad = cpad_for(nq);
-  k_sel_ntt<<<dim3(fst * 2, nq), kNttThreads, 0, s->st>>>(sel, sel_stride, s->Qs, fst, cpad, s->Tq);
-  if (time_core) (void)hipEventRecord(s->ev[6], s->st);
+  const uint64_t num_pt = s->I.num_pt;
+  k_sel_ntt<<<dim3(fst * 2, nq), kNttThreads, 0, L->st>>>(sel, sel_stride, L->Qs, fst, cpad, s->Tq);
+  if (time_core) (void)hipEventRecord(L->ev[6], L->st);
   if (nq == 1) {
     if (fst % 16 == 0) {
       const uint32_t rpb = std::max(32u, std::min(s->tune.fd_rows_b1, (other + 31) / 32 * 32));
       const dim3 grid(kN, (other + rpb - 1) / rpb);
       switch (s->tune.fd_b1_variant & 3) {
-        case 0: k_first_dim_b1<4, false><<<grid, 256, 0, s->st>>>(s->db, s->Qs, s->fd, fst, other, s->I.num_pt, rpb, s->mq); break;
-        case 1: k_first_dim_b1<8, false><<<grid, 256, 0, s->st>>>(s->db, s->Qs, s->fd, fst, other, s->I.num_pt, rpb, s->mq); break;
-        case 2: k_first_dim_b1<4, true><<<grid, 256, 0, s->st>>>(s->db, s->Qs, s->fd, fst, other, s->I.num_pt, rpb, s->mq); break;
-        default: k_first_dim_b1<8, true><<<grid, 256, 0, s->st>>>(s->db, s->Qs, s->fd, fst, other, s->I.num_pt, rpb, s->mq); break;
+        case 0: k_first_dim_b1<4, false><<<grid, 256, 0, L->st>>>(s->db, L->Qs, L->fd, fst, other, num_pt, rpb, s->mq); break;
+        case 1: k_first_dim_b1<8, false><<<grid, 256, 0, L->st>>>(s->db, L->Qs, L->fd, fst, other, num_pt, rpb, s->mq); break;
+        case 2: k_first_dim_b1<4, true><<<grid, 256, 0, L->st>>>(s->db, L->Qs, L->fd, fst, other, num_pt, rpb, s->mq); break;
+        default: k_first_dim_b1<8, true><<<grid, 256, 0, L->st>>>(s->db, L->Qs, L->fd, fst, other, num_pt, rpb, s->mq); break;
       }
     } else {
-      k_first_dim_generic<<<kN, 256, 0, s->st>>>(s->db, s->Qs, s->fd, fst, other, s->I.num_pt, s->mq);
+      k_first_dim_generic<<<kN, 256, 0, L->st>>>(s->db, L->Qs, L->fd, fst, other, num_pt, s->mq);
     }
   } else if (cpad == 32) {
-    launch_fd_bat<32>(s, nq);
+    launch_fd_bat<32>(s, L, nq);
   } else if (cpad == 64) {
-    launch_fd_bat<64>(s, nq);
+    launch_fd_bat<64>(s, L, nq);
   } else {
-    launch_fd_bat<128>(s, nq);
+    launch_fd_bat<128>(s, L, nq);
   }
-  if (time_core) (void)hipEventRecord(s->ev[7], s->st);
-  k_ntt_inv<<<other * 2 * nq, kNttThreads, 0, s->st>>>(s->fd, s->fd, s->Tq);
+  if (time_core) (void)hipEventRecord(L->ev[7], L->st);
+  k_ntt_inv<<<other * 2 * nq, kNttThreads, 0, L->st>>>(L->fd, L->fd, s->Tq);
 }
 
 // One multiplexer level over n cts per query in fd, with RGSW `g` (per-query stride).
-void run_mux_level(onion_server *s, uint32_t nq, uint64_t n, const uint64_t *g) {
+void run_mux_level(const onion_server *s, Lane *L, uint32_t nq, uint64_t n, const uint64_t *g) {
   const uint32_t l = s->P.gsw_l;
   const uint64_t half = n / 2;
   if (half * nq >= s->tune.fuse_min) {
-    k_mux_fused<<<dim3((uint32_t)half, nq), kNttThreads, 0, s->st>>>(s->fd, s->fd_words, half, g, s->rgsw_words,
+    k_mux_fused<<<dim3((uint32_t)half, nq), kNttThreads, 0, L->st>>>(L->fd, s->fd_words, half, g, s->rgsw_words,
                                                                     l, s->I.base_log2, s->Tq, s->mq);
     return;
   }
-  k_decomp_ntt<<<dim3((uint32_t)(half * 2 * l), nq), kNttThreads, 0, s->st>>>(
-      s->fd, s->fd + half * 2 * kN, s->fd_words, s->digits, s->digit_words, l, s->I.base_log2, s->Tq);
-  k_ext_mac_intt_add<<<dim3((uint32_t)(half * 2), nq), kNttThreads, 0, s->st>>>(
-      s->digits, s->digit_words, g, s->rgsw_words, 2 * l, s->fd, s->fd_words, s->Tq, s->mq);
+  k_decomp_ntt<<<dim3((uint32_t)(half * 2 * l), nq), kNttThreads, 0, L->st>>>(
+      L->fd, L->fd + half * 2 * kN, s->fd_words, L->digits, s->digit_words, l, s->I.base_log2, s->Tq);
+  k_ext_mac_intt_add<<<dim3((uint32_t)(half * 2), nq), kNttThreads, 0, L->st>>>(
+      L->digits, s->digit_words, g, s->rgsw_words, 2 * l, L->fd, s->fd_words, s->Tq, s->mq);
 }
 
-void run_mod_switch(onion_server *s, uint32_t nq, uint64_t *cts, uint64_t stride) {
-  k_mod_switch<<<dim3((2 * kN + 255) / 256, nq), 256, 0, s->st>>>(cts, stride, 2 * kN, s->ms_M, s->ms_f);
+void run_mod_switch(const onion_server *s, Lane *L, uint32_t nq, uint64_t *cts, uint64_t stride) {
+  k_mod_switch<<<dim3((2 * kN + 255) / 256, nq), 256, 0, L->st>>>(cts, stride, 2 * kN, s->ms_M, s->ms_f);
 }
 
 // Full answer for nq queries already in tree slot 1; responses in fd[b][0].
-int answer_device(onion_server *s, uint32_t nq, bool timed) {
+// With `timed`, events ev[0..5] bracket the stages and ev[6..7] the
+// database-streaming kernel (never inside a graph capture).
+void answer_device(const onion_server *s, Lane *L, uint32_t nq, bool timed) {
   const uint32_t fst = s->P.fst_dim, l = s->P.gsw_l;
-  const uint64_t *leaves = s->trees + s->w * 2 * kN;
-  if (timed) HIPCHK(hipEventRecord(s->ev[0], s->st));
-  run_expand(s, nq);
-  if (timed) HIPCHK(hipEventRecord(s->ev[1], s->st));
-  run_convert(s, nq, leaves + (size_t)fst * 2 * kN, s->tree_words);
-  if (timed) HIPCHK(hipEventRecord(s->ev[2], s->st));
-  run_first_dim(s, nq, leaves, s->tree_words, timed);
-  if (timed) HIPCHK(hipEventRecord(s->ev[3], s->st));
+  const uint64_t *leaves = L->trees + s->w * 2 * kN;
+  if (timed) (void)hipEventRecord(L->ev[0], L->st);
+  run_expand(s, L, nq);
+  if (timed) (void)hipEventRecord(L->ev[1], L->st);
+  run_convert(s, L, nq, leaves + (size_t)fst * 2 * kN, s->tree_words);
+  if (timed) (void)hipEventRecord(L->ev[2], L->st);
+  run_first_dim(s, L, nq, leaves, s->tree_words, timed);
+  if (timed) (void)hipEventRecord(L->ev[3], L->st);
   uint64_t n = s->P.other_dim;
   for (uint32_t d = 0; d + 1 < s->I.ndim; ++d) {
-    run_mux_level(s, nq, n, s->rgsw + (size_t)d * 2 * l * 2 * kN);
+    run_mux_level(s, L, nq, n, L->rgsw + (size_t)d * 2 * l * 2 * kN);
     n /= 2;
   }
-  if (timed) HIPCHK(hipEventRecord(s->ev[4], s->st));
-  run_mod_switch(s, nq, s->fd, s->fd_words);
-  if (timed) HIPCHK(hipEventRecord(s->ev[5], s->st));
-  HIPCHK(hipGetLastError());
+  if (timed) (void)hipEventRecord(L->ev[4], L->st);
+  run_mod_switch(s, L, nq, L->fd, s->fd_words);
+  if (timed) (void)hipEventRecord(L->ev[5], L->st);
+}
+
+// The lane's graph of the pipeline for nq queries, captured on first use.
+int graph_for(const onion_server *s, Lane *L, uint32_t nq, hipGraphExec_t *out) {
+  if (L->graph_epoch != s->tune_epoch) {
+    drop_graphs(L);
+    L->graph_epoch = s->tune_epoch;
+  }
+  auto it = L->graphs.find(nq);
+  if (it != L->graphs.end()) {
+    *out = it->second;
+    return ONION_OK;
+  }
+  HIPCHK(hipStreamBeginCapture(L->st, hipStreamCaptureModeThreadLocal));
+  answer_device(s, L, nq, false);
+  hipGraph_t g = nullptr;
+  const hipError_t launch_err = hipGetLastError();
+  const hipError_t end_err = hipStreamEndCapture(L->st, &g);
+  if (launch_err != hipSuccess || end_err != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(ONION_EHIP, std::string("pipeline capture failed: ") +
+                                hipGetErrorString(launch_err != hipSuccess ? launch_err : end_err));
+  }
+  hipGraphExec_t ex = nullptr;
+  const hipError_t inst_err = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (inst_err != hipSuccess) return fail(ONION_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(inst_err));
+  L->graphs[nq] = ex;
+  *out = ex;
   return ONION_OK;
 }
 
-int collect_times(onion_server *s) {
-  HIPCHK(hipEventSynchronize(s->ev[5]));
-  float ms[5], core = 0, tot = 0;
-  for (int i = 0; i < 5; ++i) HIPCHK(hipEventElapsedTime(&ms[i], s->ev[i], s->ev[i + 1]));
-  HIPCHK(hipEventElapsedTime(&core, s->ev[6], s->ev[7]));
-  HIPCHK(hipEventElapsedTime(&tot, s->ev[0], s->ev[5]));
-  s->last.expand = ms[0];
-  s->last.convert = ms[1];
-  s->last.first_dim = ms[2];
-  s->last.other_dim = ms[3];
-  s->last.mod_switch = ms[4];
-  s->last.total = tot;
-  s->last.first_dim_core = core;
+int collect_times(onion_server *s, Lane *L, bool staged, onion_stage_times *out) {
+  HIPCHK(hipEventSynchronize(L->ev[5]));
+  onion_stage_times t{};
+  float tot = 0;
+  HIPCHK(hipEventElapsedTime(&tot, L->ev[0], L->ev[5]));
+  t.total = tot;
+  if (staged) {
+    float ms[5], core = 0;
+    for (int i = 0; i < 5; ++i) HIPCHK(hipEventElapsedTime(&ms[i], L->ev[i], L->ev[i + 1]));
+    HIPCHK(hipEventElapsedTime(&core, L->ev[6], L->ev[7]));
+    t.expand = ms[0];
+    t.convert = ms[1];
+    t.first_dim = ms[2];
+    t.other_dim = ms[3];
+    t.mod_switch = ms[4];
+    t.first_dim_core = core;
+  }
+  if (out) *out = t;
+  std::lock_guard<std::mutex> g(s->last_mu);
+  s->last = t;
   return ONION_OK;
 }
 
-// Answer n (<= cap) queries: upload, run, download.
-int answer_chunk(onion_server *s, uint32_t n, const uint64_t *ids, const uint64_t *queries, uint64_t *resps) {
+// Answer n (<= L->cap) queries on lane L: upload, run, download.
+int answer_chunk(onion_server *s, Lane *L, uint32_t n, const uint64_t *ids, const uint64_t *queries,
+                 uint64_t *resps, onion_stage_times *times) {
   int rc;
-  if ((rc = bind_keys(s, n, ids, s->I.ndim > 1))) return rc;
-  HIPCHK(hipMemcpy2DAsync(s->trees + 2 * kN, s->tree_words * 8, queries, 2 * kN * 8, 2 * kN * 8, n,
-                          hipMemcpyHostToDevice, s->st));
-  if ((rc = answer_device(s, n, true))) return rc;
-  HIPCHK(hipMemcpy2DAsync(resps, 2 * kN * 8, s->fd, s->fd_words * 8, 2 * kN * 8, n, hipMemcpyDeviceToHost,
-                          s->st));
-  HIPCHK(hipStreamSynchronize(s->st));
-  return collect_times(s);
+  if ((rc = bind_keys(s, L, n, ids, s->I.ndim > 1))) return rc;
+  HIPCHK(hipMemcpy2DAsync(L->trees + 2 * kN, s->tree_words * 8, queries, 2 * kN * 8, 2 * kN * 8, n,
+                          hipMemcpyHostToDevice, L->st));
+  const bool graph = s->tune.use_graph != 0;
+  if (graph) {
+    hipGraphExec_t ex;
+    if ((rc = graph_for(s, L, n, &ex))) return rc;
+    HIPCHK(hipEventRecord(L->ev[0], L->st));
+    HIPCHK(hipGraphLaunch(ex, L->st));
+    HIPCHK(hipEventRecord(L->ev[5], L->st));
+  } else {
+    answer_device(s, L, n, true);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipMemcpy2DAsync(resps, 2 * kN * 8, L->fd, s->fd_words * 8, 2 * kN * 8, n, hipMemcpyDeviceToHost,
+                          L->st));
+  HIPCHK(hipStreamSynchronize(L->st));
+  return collect_times(s, L, !graph, times);
+}
+
+// n queries on lane L in chunks of kMaxBatch; the caller holds the gate shared
+// and L->mu.
+int answer_on_lane(onion_server *s, Lane *L, uint32_t n, const uint64_t *ids, const uint64_t *queries,
+                   uint64_t *resps, onion_stage_times *times) {
+  int rc;
+  if ((rc = check_db(s))) return rc;
+  if (n == 0) return ONION_OK;
+  if ((rc = ensure_scratch(s, L, std::min(n, kMaxBatch)))) return rc;
+  onion_stage_times sum{};
+  for (uint32_t off = 0; off < n; off += kMaxBatch) {
+    const uint32_t m = std::min(kMaxBatch, n - off);
+    onion_stage_times t{};
+    if ((rc = answer_chunk(s, L, m, ids + off, queries + (size_t)off * 2 * kN, resps + (size_t)off * 2 * kN, &t)))
+      return rc;
+    sum.expand += t.expand;
+    sum.convert += t.convert;
+    sum.first_dim += t.first_dim;
+    sum.other_dim += t.other_dim;
+    sum.mod_switch += t.mod_switch;
+    sum.total += t.total;
+    sum.first_dim_core += t.first_dim_core;
+  }
+  if (times) *times = sum;
+  return ONION_OK;
 }
 
 }  // namespace
@@ -379,11 +546,12 @@ int onion_gpu_create(const onion_params *params, onion_server **out) {
     delete s;
     return fail(ONION_EHIP, "no HIP device");
   }
-  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) {
-    delete s;
-    return fail(ONION_EHIP, "hipStreamCreate failed");
+  s->lanes.emplace_back();
+  if ((rc = make_lane(s->lanes[0]))) {
+    std::string msg = g_err;
+    onion_gpu_destroy(s);
+    return fail(rc, msg);
   }
-  for (auto &e : s->ev) (void)hipEventCreate(&e);
 
   // NTT tables and constants
   host::Tables tq = host::make_tables(I.q, kN), tp = host::make_tables(I.p, kN);
@@ -430,7 +598,7 @@ int onion_gpu_create(const onion_params *params, onion_server **out) {
                                     (uint64_t)std::max(1u, nd) * 2 * P.gsw_l * ct, 64 * ct,
                                     (uint64_t)P.fst_dim * ct});
   if ((rc = dalloc(&s->db, I.num_pt * N)) || (rc = dalloc(&s->io, s->io_words)) ||
-      (rc = dalloc(&s->pack, (uint64_t)I.response_bytes + 64)) || (rc = ensure_scratch(s, 1))) {
+      (rc = dalloc(&s->pack, (uint64_t)I.response_bytes + 64)) || (rc = ensure_scratch(s, s->lanes[0].get(), 1))) {
     std::string msg = g_err;
     onion_gpu_destroy(s);
     return fail(rc, msg);
@@ -442,17 +610,18 @@ int onion_gpu_create(const onion_params *params, onion_server **out) {
 
 int onion_gpu_destroy(onion_server *s) {
   if (!s) return ONION_OK;
-  if (s->st) (void)hipStreamSynchronize(s->st);
-  for (auto &kv : s->clients) {
-    dfree(kv.second.gk);
-    dfree(kv.second.rk);
+  {
+    Exclusive x(s->gate);
+    for (auto &L : s->lanes)
+      if (L) destroy_lane(L.get());
+    s->lanes.clear();
+    for (auto &kv : s->clients) {
+      dfree(kv.second.gk);
+      dfree(kv.second.rk);
+    }
+    void *bufs[] = {s->tabs, s->db, s->io, s->pack};
+    for (void *b : bufs) dfree(b);
   }
-  free_scratch(s);
-  void *bufs[] = {s->tabs, s->db, s->io, s->pack};
-  for (void *b : bufs) dfree(b);
-  for (auto &e : s->ev)
-    if (e) (void)hipEventDestroy(e);
-  if (s->st) (void)hipStreamDestroy(s->st);
   delete s;
   return ONION_OK;
 }
@@ -463,21 +632,55 @@ int onion_gpu_param_info(const onion_server *s, onion_param_info *out) {
   return ONION_OK;
 }
 
+int onion_gpu_set_lanes(onion_server *s, uint32_t n) {
+  if (!s) return fail(ONION_EINVAL, "null server");
+  if (n < 1 || n > kMaxLanes) return fail(ONION_EINVAL, "n_lanes must be in [1, 16]");
+  Exclusive x(s->gate);
+  while (s->lanes.size() > n) {
+    destroy_lane(s->lanes.back().get());
+    s->lanes.pop_back();
+  }
+  while (s->lanes.size() < n) {
+    std::unique_ptr<Lane> L;
+    int rc = make_lane(L);
+    if (rc) return rc;
+    s->lanes.push_back(std::move(L));
+  }
+  return ONION_OK;
+}
+
+int onion_gpu_scratch_bytes_per_query(const onion_server *s, uint64_t *bytes) {
+  if (!s || !bytes) return fail(ONION_EINVAL, "null argument");
+  *bytes = scratch_bytes_per_query(s);
+  return ONION_OK;
+}
+
+int onion_gpu_reserve(onion_server *s, uint32_t lane, uint32_t n) {
+  if (!s) return fail(ONION_EINVAL, "null server");
+  Shared g(s->gate);
+  if (lane >= s->lanes.size())
+    return fail(ONION_EINVAL, "lane " + std::to_string(lane) + " of " + std::to_string(s->lanes.size()));
+  Lane *L = s->lanes[lane].get();
+  std::lock_guard<std::mutex> lk(L->mu);
+  return ensure_scratch(s, L, std::max(1u, std::min(n, kMaxBatch)));
+}
+
 int onion_gpu_db_generate(onion_server *s, uint64_t seed, int mode) {
   if (!s) return fail(ONION_EINVAL, "null server");
   if (mode != 0 && mode != 1) return fail(ONION_EINVAL, "mode must be 0 or 1");
   if (mode == 1 && s->P.plain_bits > 25)
     return fail(ONION_EINVAL, "entry mode packs at most 24 bits per coefficient");
-  std::lock_guard<std::mutex> lk(s->mu);
+  Exclusive x(s->gate);
+  hipStream_t st = s->lanes[0]->st;
   const uint64_t key = stream_key(seed, 0x4442ULL);
   const uint64_t chunk = 65535;
   for (uint64_t lo = 0; lo < s->I.num_pt; lo += chunk) {
     const uint64_t n = std::min(chunk, s->I.num_pt - lo);
-    k_db_generate<<<(uint32_t)n, kNttThreads, 0, s->st>>>(s->db, lo, s->I.num_pt, key, mode, s->P.plain_bits - 1,
-                                                          s->lift, s->Tq);
+    k_db_generate<<<(uint32_t)n, kNttThreads, 0, st>>>(s->db, lo, s->I.num_pt, key, mode, s->P.plain_bits - 1,
+                                                       s->lift, s->Tq);
   }
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(s->st));
+  HIPCHK(hipStreamSynchronize(st));
   std::fill(s->pt_written.begin(), s->pt_written.end(), 1);
   s->n_written = s->I.num_pt;
   return ONION_OK;
@@ -487,16 +690,17 @@ int onion_gpu_db_upload_plain(onion_server *s, uint64_t pt_lo, uint64_t n_pt, co
   if (!s || (!coeffs && n_pt)) return fail(ONION_EINVAL, "null argument");
   if (pt_lo > s->I.num_pt || n_pt > s->I.num_pt - pt_lo)
     return fail(ONION_EINVAL, "plaintext range beyond the database");
-  std::lock_guard<std::mutex> lk(s->mu);
+  for (uint64_t i = 0; i < n_pt * kN; ++i)
+    if (coeffs[i] >= s->I.t) return fail(ONION_EINVAL, "plaintext coefficient >= t");
+  Exclusive x(s->gate);
+  hipStream_t st = s->lanes[0]->st;
   const uint64_t chunk = std::max<uint64_t>(1, s->io_words / kN);
   for (uint64_t off = 0; off < n_pt; off += chunk) {
     const uint64_t n = std::min(chunk, n_pt - off);
-    for (uint64_t i = 0; i < n * kN; ++i)
-      if (coeffs[off * kN + i] >= s->I.t) return fail(ONION_EINVAL, "plaintext coefficient >= t");
-    HIPCHK(hipMemcpyAsync(s->io, coeffs + off * kN, n * kN * 8, hipMemcpyHostToDevice, s->st));
-    k_db_plain<<<(uint32_t)n, kNttThreads, 0, s->st>>>(s->io, s->db, pt_lo + off, s->I.num_pt, s->lift, s->Tq);
+    HIPCHK(hipMemcpyAsync(s->io, coeffs + off * kN, n * kN * 8, hipMemcpyHostToDevice, st));
+    k_db_plain<<<(uint32_t)n, kNttThreads, 0, st>>>(s->io, s->db, pt_lo + off, s->I.num_pt, s->lift, s->Tq);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(s->st));
+    HIPCHK(hipStreamSynchronize(st));
     for (uint64_t i = 0; i < n; ++i) {
       uint8_t &f = s->pt_written[pt_lo + off + i];
       if (!f) { f = 1; ++s->n_written; }
@@ -507,9 +711,11 @@ int onion_gpu_db_upload_plain(onion_server *s, uint64_t pt_lo, uint64_t n_pt, co
 
 int onion_gpu_db_read_ntt(onion_server *s, uint64_t pt, uint64_t *out) {
   if (!s || !out || pt >= s->I.num_pt) return fail(ONION_EINVAL, "bad argument");
-  std::lock_guard<std::mutex> lk(s->mu);
-  HIPCHK(hipMemcpy2DAsync(out, 8, s->db + pt, s->I.num_pt * 8, 8, kN, hipMemcpyDeviceToHost, s->st));
-  HIPCHK(hipStreamSynchronize(s->st));
+  Shared g(s->gate);
+  Lane *L = s->lanes[0].get();
+  std::lock_guard<std::mutex> lk(L->mu);
+  HIPCHK(hipMemcpy2DAsync(out, 8, s->db + pt, s->I.num_pt * 8, 8, kN, hipMemcpyDeviceToHost, L->st));
+  HIPCHK(hipStreamSynchronize(L->st));
   return ONION_OK;
 }
 
@@ -518,14 +724,18 @@ int onion_gpu_set_galois_key(onion_server *s, uint64_t id, const uint64_t *keys,
   if (n != s->I.galois_key_words)
     return fail(ONION_EINVAL, "Galois key size " + std::to_string(n) + " != " +
                                   std::to_string(s->I.galois_key_words));
-  std::lock_guard<std::mutex> lk(s->mu);
+  Exclusive x(s->gate);
+  hipStream_t st = s->lanes[0]->st;
   ClientKeys &ck = s->clients[id];
   if (!ck.gk) {
     int rc = dalloc(&ck.gk, n);
-    if (rc) return rc;
+    if (rc) {
+      if (!ck.rk) s->clients.erase(id);
+      return rc;
+    }
   }
-  HIPCHK(hipMemcpyAsync(ck.gk, keys, n * 8, hipMemcpyHostToDevice, s->st));
-  HIPCHK(hipStreamSynchronize(s->st));
+  HIPCHK(hipMemcpyAsync(ck.gk, keys, n * 8, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
   return ONION_OK;
 }
 
@@ -534,26 +744,29 @@ int onion_gpu_set_rgsw_key(onion_server *s, uint64_t id, const uint64_t *rows, s
   if (n != s->I.rgsw_key_words)
     return fail(ONION_EINVAL, "RGSW key size " + std::to_string(n) + " != " +
                                   std::to_string(s->I.rgsw_key_words));
-  std::lock_guard<std::mutex> lk(s->mu);
+  Exclusive x(s->gate);
+  hipStream_t st = s->lanes[0]->st;
   ClientKeys &ck = s->clients[id];
   if (!ck.rk) {
     int rc = dalloc(&ck.rk, n);
-    if (rc) return rc;
+    if (rc) {
+      if (!ck.gk) s->clients.erase(id);
+      return rc;
+    }
   }
-  HIPCHK(hipMemcpyAsync(ck.rk, rows, n * 8, hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemcpyAsync(ck.rk, rows, n * 8, hipMemcpyHostToDevice, st));
   // GSWEval::gsw_ntt_negacyclic_harvey: every half-row to NTT form
-  k_ntt_fwd<<<(uint32_t)(n / kN), kNttThreads, 0, s->st>>>(ck.rk, ck.rk, s->Tq);
+  k_ntt_fwd<<<(uint32_t)(n / kN), kNttThreads, 0, st>>>(ck.rk, ck.rk, s->Tq);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(s->st));
+  HIPCHK(hipStreamSynchronize(st));
   return ONION_OK;
 }
 
 int onion_gpu_drop_client(onion_server *s, uint64_t id) {
   if (!s) return fail(ONION_EINVAL, "null server");
-  std::lock_guard<std::mutex> lk(s->mu);
+  Exclusive x(s->gate);
   auto it = s->clients.find(id);
   if (it == s->clients.end()) return fail(ONION_ENOKEY, "unknown client " + std::to_string(id));
-  HIPCHK(hipStreamSynchronize(s->st));
   dfree(it->second.gk);
   dfree(it->second.rk);
   s->clients.erase(it);
@@ -562,39 +775,45 @@ int onion_gpu_drop_client(onion_server *s, uint64_t id) {
 
 int onion_gpu_answer(onion_server *s, uint64_t id, const uint64_t *query, uint64_t *resp) {
   if (!s || !query || !resp) return fail(ONION_EINVAL, "null argument");
-  std::lock_guard<std::mutex> lk(s->mu);
-  int rc;
-  if ((rc = check_db(s))) return rc;
-  return answer_chunk(s, 1, &id, query, resp);
+  Shared g(s->gate);
+  Lane *L = s->lanes[0].get();
+  std::lock_guard<std::mutex> lk(L->mu);
+  return answer_on_lane(s, L, 1, &id, query, resp, nullptr);
 }
 
 int onion_gpu_answer_batch(onion_server *s, uint32_t n, const uint64_t *ids, const uint64_t *queries,
                            uint64_t *resps) {
   if (!s || (n && (!ids || !queries || !resps))) return fail(ONION_EINVAL, "null argument");
-  std::lock_guard<std::mutex> lk(s->mu);
-  int rc;
-  if ((rc = check_db(s))) return rc;
-  if (n == 0) return ONION_OK;
-  if ((rc = ensure_scratch(s, std::min(n, kMaxBatch)))) return rc;
-  for (uint32_t off = 0; off < n; off += kMaxBatch) {
-    const uint32_t m = std::min(kMaxBatch, n - off);
-    if ((rc = answer_chunk(s, m, ids + off, queries + (size_t)off * 2 * kN, resps + (size_t)off * 2 * kN)))
-      return rc;
-  }
-  return ONION_OK;
+  Shared g(s->gate);
+  Lane *L = s->lanes[0].get();
+  std::lock_guard<std::mutex> lk(L->mu);
+  return answer_on_lane(s, L, n, ids, queries, resps, nullptr);
+}
+
+int onion_gpu_answer_batch_lane(onion_server *s, uint32_t lane, uint32_t n, const uint64_t *ids,
+                                const uint64_t *queries, uint64_t *resps, onion_stage_times *times) {
+  if (!s || (n && (!ids || !queries || !resps))) return fail(ONION_EINVAL, "null argument");
+  Shared g(s->gate);
+  if (lane >= s->lanes.size())
+    return fail(ONION_EINVAL, "lane " + std::to_string(lane) + " of " + std::to_string(s->lanes.size()));
+  Lane *L = s->lanes[lane].get();
+  std::lock_guard<std::mutex> lk(L->mu);
+  return answer_on_lane(s, L, n, ids, queries, resps, times);
 }
 
 int onion_gpu_pack_response(onion_server *s, const uint64_t *resp, uint8_t *out, size_t cap,
                             size_t *n_bytes) {
   if (!s || !resp || !out) return fail(ONION_EINVAL, "null argument");
   if (cap < s->I.response_bytes) return fail(ONION_EINVAL, "output buffer too small");
-  std::lock_guard<std::mutex> lk(s->mu);
+  Shared g(s->gate);
+  Lane *L = s->lanes[0].get();
+  std::lock_guard<std::mutex> lk(L->mu);
   const uint32_t wbits = response_bits(s->I.small_q);
-  HIPCHK(hipMemcpyAsync(s->io, resp, 2 * kN * 8, hipMemcpyHostToDevice, s->st));
-  k_pack_response<<<(2 * kN / 8 + 255) / 256, 256, 0, s->st>>>(s->io, s->pack, 2 * kN, wbits);
+  HIPCHK(hipMemcpyAsync(s->io, resp, 2 * kN * 8, hipMemcpyHostToDevice, L->st));
+  k_pack_response<<<(2 * kN / 8 + 255) / 256, 256, 0, L->st>>>(s->io, s->pack, 2 * kN, wbits);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(out, s->pack, s->I.response_bytes, hipMemcpyDeviceToHost, s->st));
-  HIPCHK(hipStreamSynchronize(s->st));
+  HIPCHK(hipMemcpyAsync(out, s->pack, s->I.response_bytes, hipMemcpyDeviceToHost, L->st));
+  HIPCHK(hipStreamSynchronize(L->st));
   if (n_bytes) *n_bytes = s->I.response_bytes;
   return ONION_OK;
 }
@@ -611,74 +830,87 @@ int onion_gpu_set_tuning(onion_server *s, const onion_tuning *t) {
     return fail(ONION_EINVAL, "fd_rows_b1 must be a multiple of 32 in [32, 1024]");
   if (t->fd_tile_rows != 2 && t->fd_tile_rows != 4) return fail(ONION_EINVAL, "fd_tile_rows must be 2 or 4");
   if (t->fd_b1_variant > 3) return fail(ONION_EINVAL, "fd_b1_variant must be 0..3");
-  std::lock_guard<std::mutex> lk(s->mu);
+  if (t->use_graph > 1) return fail(ONION_EINVAL, "use_graph must be 0 or 1");
+  Exclusive x(s->gate);
   s->tune = *t;
   s->tune.fuse_min = std::max(1u, t->fuse_min);
+  ++s->tune_epoch;
   return ONION_OK;
 }
 
 int onion_gpu_last_stage_times(const onion_server *s, onion_stage_times *out) {
   if (!s || !out) return fail(ONION_EINVAL, "null argument");
+  std::lock_guard<std::mutex> g(s->last_mu);
   *out = s->last;
   return ONION_OK;
 }
 
-// ---- taps ---------------------------------------------------------------------------
+// ---- taps (lane 0) ------------------------------------------------------------------
 
 int onion_gpu_tap_expand(onion_server *s, uint64_t id, const uint64_t *query, uint64_t *leaves) {
   if (!s || !query || !leaves) return fail(ONION_EINVAL, "null argument");
-  std::lock_guard<std::mutex> lk(s->mu);
+  Shared g(s->gate);
+  Lane *L = s->lanes[0].get();
+  std::lock_guard<std::mutex> lk(L->mu);
   int rc;
-  if ((rc = bind_keys(s, 1, &id, false))) return rc;
-  HIPCHK(hipMemcpyAsync(s->trees + 2 * kN, query, 2 * kN * 8, hipMemcpyHostToDevice, s->st));
-  run_expand(s, 1);
+  if ((rc = ensure_scratch(s, L, 1)) || (rc = bind_keys(s, L, 1, &id, false))) return rc;
+  HIPCHK(hipMemcpyAsync(L->trees + 2 * kN, query, 2 * kN * 8, hipMemcpyHostToDevice, L->st));
+  run_expand(s, L, 1);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(leaves, s->trees + s->w * 2 * kN, (size_t)s->I.useful_cnt * 2 * kN * 8,
-                        hipMemcpyDeviceToHost, s->st));
-  HIPCHK(hipStreamSynchronize(s->st));
+  HIPCHK(hipMemcpyAsync(leaves, L->trees + s->w * 2 * kN, (size_t)s->I.useful_cnt * 2 * kN * 8,
+                        hipMemcpyDeviceToHost, L->st));
+  HIPCHK(hipStreamSynchronize(L->st));
   return ONION_OK;
 }
 
 int onion_gpu_tap_convert(onion_server *s, uint64_t id, const uint64_t *leaves, uint64_t *rgsw) {
   if (!s || !leaves || !rgsw) return fail(ONION_EINVAL, "null argument");
-  std::lock_guard<std::mutex> lk(s->mu);
+  Shared g(s->gate);
+  Lane *L = s->lanes[0].get();
+  std::lock_guard<std::mutex> lk(L->mu);
   int rc;
-  if ((rc = bind_keys(s, 1, &id, true))) return rc;
+  if ((rc = ensure_scratch(s, L, 1)) || (rc = bind_keys(s, L, 1, &id, true))) return rc;
   const uint32_t nd = s->I.ndim - 1;
   const size_t n_ct = (size_t)nd * s->P.gsw_l;
   if (nd == 0) return ONION_OK;
   HIPCHK(hipMemcpyAsync(s->io, leaves + (size_t)s->P.fst_dim * 2 * kN, n_ct * 2 * kN * 8,
-                        hipMemcpyHostToDevice, s->st));
-  run_convert(s, 1, s->io, 0);
+                        hipMemcpyHostToDevice, L->st));
+  run_convert(s, L, 1, s->io, 0);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(rgsw, s->rgsw, (size_t)nd * 2 * s->P.gsw_l * 2 * kN * 8, hipMemcpyDeviceToHost, s->st));
-  HIPCHK(hipStreamSynchronize(s->st));
+  HIPCHK(hipMemcpyAsync(rgsw, L->rgsw, (size_t)nd * 2 * s->P.gsw_l * 2 * kN * 8, hipMemcpyDeviceToHost, L->st));
+  HIPCHK(hipStreamSynchronize(L->st));
   return ONION_OK;
 }
 
 int onion_gpu_tap_first_dim(onion_server *s, const uint64_t *sel, uint64_t *out) {
   if (!s || !sel || !out) return fail(ONION_EINVAL, "null argument");
-  std::lock_guard<std::mutex> lk(s->mu);
+  Shared g(s->gate);
+  Lane *L = s->lanes[0].get();
+  std::lock_guard<std::mutex> lk(L->mu);
   int rc;
-  if ((rc = check_db(s))) return rc;
-  HIPCHK(hipMemcpyAsync(s->io, sel, (size_t)s->P.fst_dim * 2 * kN * 8, hipMemcpyHostToDevice, s->st));
-  run_first_dim(s, 1, s->io, 0, false);
+  if ((rc = check_db(s)) || (rc = ensure_scratch(s, L, 1))) return rc;
+  HIPCHK(hipMemcpyAsync(s->io, sel, (size_t)s->P.fst_dim * 2 * kN * 8, hipMemcpyHostToDevice, L->st));
+  run_first_dim(s, L, 1, s->io, 0, false);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(out, s->fd, (size_t)s->P.other_dim * 2 * kN * 8, hipMemcpyDeviceToHost, s->st));
-  HIPCHK(hipStreamSynchronize(s->st));
+  HIPCHK(hipMemcpyAsync(out, L->fd, (size_t)s->P.other_dim * 2 * kN * 8, hipMemcpyDeviceToHost, L->st));
+  HIPCHK(hipStreamSynchronize(L->st));
   return ONION_OK;
 }
 
 int onion_gpu_tap_mux(onion_server *s, const uint64_t *cts, uint32_t n, const uint64_t *rgsw, uint64_t *out) {
   if (!s || !cts || !rgsw || !out) return fail(ONION_EINVAL, "null argument");
   if (n < 2 || (n & 1) || n > s->P.other_dim) return fail(ONION_EINVAL, "n must be even and <= other_dim");
-  std::lock_guard<std::mutex> lk(s->mu);
-  HIPCHK(hipMemcpyAsync(s->fd, cts, (size_t)n * 2 * kN * 8, hipMemcpyHostToDevice, s->st));
-  HIPCHK(hipMemcpyAsync(s->rgsw, rgsw, (size_t)2 * s->P.gsw_l * 2 * kN * 8, hipMemcpyHostToDevice, s->st));
-  run_mux_level(s, 1, n, s->rgsw);
+  Shared g(s->gate);
+  Lane *L = s->lanes[0].get();
+  std::lock_guard<std::mutex> lk(L->mu);
+  int rc;
+  if ((rc = ensure_scratch(s, L, 1))) return rc;
+  HIPCHK(hipMemcpyAsync(L->fd, cts, (size_t)n * 2 * kN * 8, hipMemcpyHostToDevice, L->st));
+  HIPCHK(hipMemcpyAsync(L->rgsw, rgsw, (size_t)2 * s->P.gsw_l * 2 * kN * 8, hipMemcpyHostToDevice, L->st));
+  run_mux_level(s, L, 1, n, L->rgsw);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(out, s->fd, (size_t)(n / 2) * 2 * kN * 8, hipMemcpyDeviceToHost, s->st));
-  HIPCHK(hipStreamSynchronize(s->st));
+  HIPCHK(hipMemcpyAsync(out, L->fd, (size_t)(n / 2) * 2 * kN * 8, hipMemcpyDeviceToHost, L->st));
+  HIPCHK(hipStreamSynchronize(L->st));
   return ONION_OK;
 }
 
@@ -686,32 +918,36 @@ int onion_gpu_tap_ntt(onion_server *s, int which, int inverse, uint64_t *polys, 
   if (!s || !polys) return fail(ONION_EINVAL, "null argument");
   if (which != 0 && which != 1) return fail(ONION_EINVAL, "which must be 0 (q) or 1 (p)");
   if ((size_t)n * kN > s->io_words) return fail(ONION_EINVAL, "too many polynomials");
-  std::lock_guard<std::mutex> lk(s->mu);
+  Shared g(s->gate);
+  Lane *L = s->lanes[0].get();
+  std::lock_guard<std::mutex> lk(L->mu);
   const NttTab &T = which == 0 ? s->Tq : s->Tp;
-  HIPCHK(hipMemcpyAsync(s->io, polys, (size_t)n * kN * 8, hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipMemcpyAsync(s->io, polys, (size_t)n * kN * 8, hipMemcpyHostToDevice, L->st));
   if (inverse)
-    k_ntt_inv<<<n, kNttThreads, 0, s->st>>>(s->io, s->io, T);
+    k_ntt_inv<<<n, kNttThreads, 0, L->st>>>(s->io, s->io, T);
   else
-    k_ntt_fwd<<<n, kNttThreads, 0, s->st>>>(s->io, s->io, T);
+    k_ntt_fwd<<<n, kNttThreads, 0, L->st>>>(s->io, s->io, T);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(polys, s->io, (size_t)n * kN * 8, hipMemcpyDeviceToHost, s->st));
-  HIPCHK(hipStreamSynchronize(s->st));
+  HIPCHK(hipMemcpyAsync(polys, s->io, (size_t)n * kN * 8, hipMemcpyDeviceToHost, L->st));
+  HIPCHK(hipStreamSynchronize(L->st));
   return ONION_OK;
 }
 
 int onion_gpu_tap_galois(onion_server *s, uint64_t id, uint32_t level, const uint64_t *ct, uint64_t *out) {
   if (!s || !ct || !out) return fail(ONION_EINVAL, "null argument");
   if (level >= s->I.expan_height) return fail(ONION_EINVAL, "level >= expansion height");
-  std::lock_guard<std::mutex> lk(s->mu);
+  Shared g(s->gate);
+  Lane *L = s->lanes[0].get();
+  std::lock_guard<std::mutex> lk(L->mu);
   int rc;
-  if ((rc = bind_keys(s, 1, &id, false))) return rc;
+  if ((rc = ensure_scratch(s, L, 1)) || (rc = bind_keys(s, L, 1, &id, false))) return rc;
   // place ct at node 2^level; its left child is ct + Subs(ct), so Subs(ct) = left - ct
   const uint64_t node = uint64_t(1) << level;
-  HIPCHK(hipMemcpyAsync(s->trees + node * 2 * kN, ct, 2 * kN * 8, hipMemcpyHostToDevice, s->st));
-  expand_level(s, 1, level, 1);
+  HIPCHK(hipMemcpyAsync(L->trees + node * 2 * kN, ct, 2 * kN * 8, hipMemcpyHostToDevice, L->st));
+  expand_level(s, L, 1, level, 1);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(out, s->trees + 2 * node * 2 * kN, 2 * kN * 8, hipMemcpyDeviceToHost, s->st));
-  HIPCHK(hipStreamSynchronize(s->st));
+  HIPCHK(hipMemcpyAsync(out, L->trees + 2 * node * 2 * kN, 2 * kN * 8, hipMemcpyDeviceToHost, L->st));
+  HIPCHK(hipStreamSynchronize(L->st));
   const uint64_t q = s->I.q;
   for (size_t i = 0; i < 2 * (size_t)kN; ++i) out[i] = out[i] >= ct[i] ? out[i] - ct[i] : out[i] + q - ct[i];
   return ONION_OK;
@@ -719,12 +955,16 @@ int onion_gpu_tap_galois(onion_server *s, uint64_t id, uint32_t level, const uin
 
 int onion_gpu_tap_copy(onion_server *s, uint64_t n_bytes, uint64_t *copied) {
   if (!s) return fail(ONION_EINVAL, "null server");
-  std::lock_guard<std::mutex> lk(s->mu);
-  const uint64_t n = std::min<uint64_t>({n_bytes, s->cap * s->digit_words * 8, s->I.num_pt * kN * 8}) / 16;
+  Shared g(s->gate);
+  Lane *L = s->lanes[0].get();
+  std::lock_guard<std::mutex> lk(L->mu);
+  int rc;
+  if ((rc = ensure_scratch(s, L, 1))) return rc;
+  const uint64_t n = std::min<uint64_t>({n_bytes, L->cap * s->digit_words * 8, s->I.num_pt * kN * 8}) / 16;
   if (n) {
-    k_copy<<<1024, 256, 0, s->st>>>(reinterpret_cast<const uint4 *>(s->db), reinterpret_cast<uint4 *>(s->digits), n);
+    k_copy<<<1024, 256, 0, L->st>>>(reinterpret_cast<const uint4 *>(s->db), reinterpret_cast<uint4 *>(L->digits), n);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(s->st));
+    HIPCHK(hipStreamSynchronize(L->st));
   }
   if (copied) *copied = n * 16;
   return ONION_OK;
@@ -732,12 +972,14 @@ int onion_gpu_tap_copy(onion_server *s, uint64_t n_bytes, uint64_t *copied) {
 
 int onion_gpu_tap_mod_switch(onion_server *s, const uint64_t *ct, uint64_t *out) {
   if (!s || !ct || !out) return fail(ONION_EINVAL, "null argument");
-  std::lock_guard<std::mutex> lk(s->mu);
-  HIPCHK(hipMemcpyAsync(s->io, ct, 2 * kN * 8, hipMemcpyHostToDevice, s->st));
-  run_mod_switch(s, 1, s->io, 0);
+  Shared g(s->gate);
+  Lane *L = s->lanes[0].get();
+  std::lock_guard<std::mutex> lk(L->mu);
+  HIPCHK(hipMemcpyAsync(s->io, ct, 2 * kN * 8, hipMemcpyHostToDevice, L->st));
+  run_mod_switch(s, L, 1, s->io, 0);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(out, s->io, 2 * kN * 8, hipMemcpyDeviceToHost, s->st));
-  HIPCHK(hipStreamSynchronize(s->st));
+  HIPCHK(hipMemcpyAsync(out, s->io, 2 * kN * 8, hipMemcpyDeviceToHost, L->st));
+  HIPCHK(hipStreamSynchronize(L->st));
   return ONION_OK;
 }
 

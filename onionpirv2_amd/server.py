@@ -104,6 +104,24 @@ class GpuServer:
         check(self._L.onion_gpu_answer_batch(self._h, ids.size, u64ptr(ids), u64ptr(q), u64ptr(out)))
         return out.reshape(ids.size, 2 * self.N)
 
+    def set_lanes(self, n_lanes: int) -> None:
+        check(self._L.onion_gpu_set_lanes(self._h, n_lanes))
+
+    def answer_batch_lane(self, lane: int, client_ids, queries):
+        """Batch on one lane; returns (responses, stage-time dict of this call)."""
+        ids = _u64(client_ids).reshape(-1)
+        q = _u64(queries).reshape(-1)
+        out = np.zeros(ids.size * 2 * self.N, np.uint64)
+        t = OnionStageTimes()
+        check(self._L.onion_gpu_answer_batch_lane(self._h, lane, ids.size, u64ptr(ids), u64ptr(q),
+                                                  u64ptr(out), C.byref(t)))
+        return out.reshape(ids.size, 2 * self.N), {f: getattr(t, f) for f, _ in OnionStageTimes._fields_}
+
+    def scratch_bytes_per_query(self) -> int:
+        b = C.c_uint64()
+        check(self._L.onion_gpu_scratch_bytes_per_query(self._h, C.byref(b)))
+        return b.value
+
     def pack_response(self, resp) -> bytes:
         r = _u64(resp).reshape(-1)
         cap = self.info["response_bytes"]

@@ -33,8 +33,8 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_python_binding_covers_the_header():
-    from onionpirv2_amd import _capi, client
-    bound = set(_capi.exported_symbols()) | set(client._SIGS)
+    from onionpirv2_amd import _capi, client, serve
+    bound = set(_capi.exported_symbols()) | set(client._SIGS) | set(serve._SIGS)
     assert set(declared_functions()) <= bound
 
 
