@@ -11,11 +11,15 @@ dimension, other dimensions, modulus switch).  Every response is checked: the
 client must decrypt the exact seeded record.
 
 Also reported: the single-query path (queries/s, p50/p99 latency, database
-GB/s scanned), the first-dimension kernel against the HBM roofline (HIP
-events on the stream the kernel runs on), and the CPU oracle timed on this
-host on a bounded sample, whose responses must equal the GPU's bit for bit.
+GB/s scanned; eager launches and the replayed hipGraph), the first-dimension
+kernel against the HBM roofline (HIP events on the stream the kernel runs
+on), two batches in flight on two lanes, the serving loop under Poisson
+arrivals (p50/p99 from submit to response), three databases of different
+shape resident together (128x256, 256x1024, 512x4096) answering bursts of
+7 / 33 / 100 queries, and the CPU oracle timed on this host on a bounded
+sample, whose responses must equal the GPU's bit for bit.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extras]
 """
 from __future__ import annotations
 
@@ -68,6 +72,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-queries", type=int, default=2)
     ap.add_argument("--plain-bits", type=int, default=16)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the two-lane, serving and multi-database measurements")
+    ap.add_argument("--serve-rates", default="100,500,1000,2000", help="Poisson arrival rates (queries/s)")
+    ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (profiler byte-counter scale)")
     args = ap.parse_args()
@@ -109,16 +117,18 @@ def main() -> int:
 
     # ---- single-query path ---------------------------------------------------------
     single_lat, cores, stages = [], [], []
+    single_resp = {}
     srv.answer(0, queries[0])  # warm
     for r in range(args.single_reps):
         i = r % B
         t = time.perf_counter()
-        srv.answer(i, queries[i])
+        single_resp[i] = srv.answer(i, queries[i])
         single_lat.append((time.perf_counter() - t) * 1e3)
         st = srv.stage_times()
         cores.append(st["first_dim_core"])
         stages.append(st)
     log(f"[bench] single query p50 {pct(single_lat, 50):.3f} ms")
+    graph_single = single_query_graph(srv, queries, args.single_reps, single_resp)
 
     # ---- batched path (the timed steps) -----------------------------------------------
     for _ in range(args.warmup):
@@ -181,11 +191,184 @@ def main() -> int:
         "cpu_baseline": None,
     }
 
+    out["single_query_graph"] = graph_single
+    ok_all = bit_exact and graph_single["bit_exact"]
+
+    # ---- two lanes: two batches in flight on two streams --------------------------------
+    if not args.no_extras:
+        lanes = two_lanes(srv, ids, queries, resps, args.steps)
+        out["batched_two_lanes"] = lanes
+        ok_all = ok_all and lanes["bit_exact"]
+        for i in range(B):
+            if i not in single_resp:
+                single_resp[i] = srv.answer(i, queries[i])
+        srv_rates = [float(r) for r in args.serve_rates.split(",") if r]
+        out["serving"] = serving(srv, queries, single_resp, srv_rates, args.serve_seconds)
+        ok_all = ok_all and all(r["bit_exact"] for r in out["serving"]["rates"])
+        del srv, clients  # free the 8 GiB database before the multi-database case
+        out["multi_db"] = multi_db(args)
+        ok_all = ok_all and out["multi_db"]["bit_exact"]
+
     # ---- CPU baseline: the oracle on this host, same inputs, bounded sample -------------------
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args, clients, queries, idx, srv)
+        if args.no_extras:
+            out["cpu_baseline"] = cpu_baseline(args, clients, queries, idx, srv)
+        else:
+            out["cpu_baseline"] = cpu_baseline_fresh(args, queries, idx, single_resp)
     print(json.dumps(out), flush=True)
-    return 0 if bit_exact else 1
+    return 0 if ok_all else 1
+
+
+def single_query_graph(srv, queries, reps, single_resp):
+    """The single-query pipeline captured once as a hipGraph and replayed."""
+    srv.set_tuning(use_graph=1)
+    try:
+        srv.answer(0, queries[0])  # capture
+        lat, same = [], True
+        for r in range(reps):
+            i = r % len(queries)
+            t = time.perf_counter()
+            got = srv.answer(i, queries[i])
+            lat.append((time.perf_counter() - t) * 1e3)
+            if i in single_resp:
+                same = same and np.array_equal(got, single_resp[i])
+    finally:
+        srv.set_tuning(use_graph=0)
+    log(f"[bench] single query (graph) p50 {pct(lat, 50):.3f} ms")
+    return {"queries_per_s": 1e3 / float(np.mean(lat)), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+            "bit_exact": bool(same)}
+
+
+def two_lanes(srv, ids, queries, ref, steps):
+    """Two host threads each run `steps` batches on their own lane."""
+    import threading
+    srv.set_lanes(2)
+    srv.answer_batch_lane(1, ids, queries)  # allocate lane 1's scratch
+    res, errs = [], []
+
+    def run(lane):
+        try:
+            for _ in range(steps):
+                r, _ = srv.answer_batch_lane(lane, ids, queries)
+                res.append(np.array_equal(r, ref))
+        except Exception as e:  # noqa: BLE001
+            errs.append(repr(e))
+
+    th = [threading.Thread(target=run, args=(k,)) for k in range(2)]
+    t0 = time.perf_counter()
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    dt = time.perf_counter() - t0
+    n = 2 * steps * len(ids)
+    log(f"[bench] two lanes: {n / dt:.1f} queries/s")
+    return {"queries_per_s": n / dt, "batches": 2 * steps, "batch": len(ids),
+            "bit_exact": bool(not errs and all(res)), "errors": errs}
+
+
+def serving(srv, queries, single_resp, rates, seconds):
+    """Poisson arrivals through the serving loop (continuous batching, two
+    lanes); latency is submit -> response ready.  Every response must equal
+    the synchronous single-query answer to the same query."""
+    from onionpirv2_amd.serve import GpuServe
+    B = len(queries)
+    out = []
+    with GpuServe(max_batch=64, lanes_per_db=2) as sv:
+        db, cap = sv.add_db(srv)
+        rng = np.random.default_rng(99)
+        for rate in rates:
+            n = max(16, int(rate * seconds))
+            gaps = rng.exponential(1.0 / rate, size=n)
+            sv.reset_stats()
+            tickets = []
+            t0 = time.perf_counter()
+            due = t0
+            for k in range(n):
+                due += gaps[k]
+                while time.perf_counter() < due:
+                    pass
+                tickets.append((k % B, sv.submit(db, k % B, queries[k % B])))
+            same = True
+            for i, t in tickets:
+                same = same and np.array_equal(sv.wait(t), single_resp[i])
+            st = sv.stats(db)
+            offered = n / (due - t0)
+            log(f"[bench] serving at {rate:.0f} q/s offered: {st['queries_per_s']:.1f} q/s done, "
+                f"p50 {st['p50_ms']:.2f} ms p99 {st['p99_ms']:.2f} ms, mean batch {st['mean_batch']:.1f}")
+            out.append({"offered_qps": offered, "target_qps": rate, "queries": n,
+                        "completed_qps": st["queries_per_s"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+                        "max_ms": st["max_ms"], "mean_batch": st["mean_batch"], "batches": st["batches"],
+                        "bit_exact": bool(same and st["failed"] == 0)})
+    return {"lanes": 2, "max_batch": 64, "batch_cap": cap, "rates": out}
+
+
+def multi_db(args):
+    """Three databases resident together (128x256, 256x1024, 512x4096: 0.5 +
+    4 + 32 GiB); bursts of 7 / 33 / 100 queries spread over them go through
+    the serving loop; every record must decrypt."""
+    from onionpirv2_amd import GpuServer, dbgen
+    from onionpirv2_amd.client import GpuClient
+    from onionpirv2_amd.serve import GpuServe
+    shapes = [(128, 256), (256, 1024), (512, 4096)]
+    n_clients = 8
+    servers, clients = [], {}
+    t0 = time.time()
+    for d, (fst, other) in enumerate(shapes):
+        s = GpuServer(fst, other, plain_bits=args.plain_bits)
+        s.db_generate(args.db_seed + 1 + d, 0)
+        for c in range(n_clients):
+            cl = GpuClient(fst, other, client_id=c, seed=3000 + 100 * d + c, plain_bits=args.plain_bits)
+            s.set_galois_key(c, cl.galois_keys())
+            s.set_rgsw_key(c, cl.rgsw_key())
+            clients[(d, c)] = cl
+        servers.append(s)
+    resident = sum(s.info["num_pt"] * N * 8 for s in servers)
+    log(f"[bench] multi-db: {resident / 2**30:.1f} GiB resident, set up in {time.time() - t0:.1f} s")
+    rng = np.random.default_rng(4242)
+    bursts, ok = [], True
+    with GpuServe(max_batch=64, lanes_per_db=1) as sv:
+        for s in servers:
+            sv.add_db(s)
+        for burst in (7, 33, 100):
+            work = []
+            for k in range(burst):
+                d = k % 3
+                c = int(rng.integers(n_clients))
+                i = int(rng.integers(servers[d].info["num_pt"]))
+                work.append((d, c, i, clients[(d, c)].query(i, nonce=burst * 1000 + k)))
+            sv.reset_stats()
+            t = time.perf_counter()
+            tickets = [sv.submit(d, c, q) for d, c, _, q in work]
+            resps = [sv.wait(tk, db_id=w[0]) for tk, w in zip(tickets, work)]
+            wall = time.perf_counter() - t
+            good = sum(int(np.array_equal(clients[(d, c)].decrypt(r),
+                                          dbgen.plaintext(args.db_seed + 1 + d, 0, i, t=servers[d].info["t"])))
+                       for (d, c, i, _), r in zip(work, resps))
+            ok = ok and good == burst
+            per_db = []
+            for d in range(3):
+                st = sv.stats(d)
+                per_db.append({"shape": f"{shapes[d][0]}x{shapes[d][1]}", "queries": st["completed"],
+                               "p50_ms": st["p50_ms"], "max_ms": st["max_ms"], "batches": st["batches"]})
+            log(f"[bench] multi-db burst {burst}: {wall * 1e3:.1f} ms, {good}/{burst} records")
+            bursts.append({"burst": burst, "wall_ms": wall * 1e3, "queries_per_s": burst / wall,
+                           "records_ok": good, "per_db": per_db})
+    for s in servers:
+        s.close()
+    return {"resident_GiB": resident / 2**30, "shapes": [f"{a}x{b}" for a, b in shapes], "bursts": bursts,
+            "bit_exact": bool(ok)}
+
+
+def cpu_baseline_fresh(args, queries, idx, single_resp):
+    """cpu_baseline against the single-query GPU responses already taken."""
+    class _Resp:
+        def answer(self, i, q):
+            return single_resp[i]
+    from onionpirv2_amd.client import GpuClient
+    clients = [GpuClient(args.fst, args.other, client_id=i, seed=1000 + i, plain_bits=args.plain_bits)
+               for i in range(args.cpu_queries)]
+    return cpu_baseline(args, clients, queries, idx, _Resp())
 
 
 def cpu_baseline(args, clients, queries, idx, srv):
