@@ -87,6 +87,9 @@ typedef struct onion_tuning {
                              = nontemporal loads                             */
   uint32_t use_graph;     /* 1: each lane captures its per-batch pipeline in a
                              hipGraph once per batch size and replays it      */
+  uint32_t fd_mfma_min;   /* batches of at least this many queries (and >= 2)
+                             take the first dimension on the int8 matrix
+                             cores (exact signed-digit products); 0 = never  */
 } onion_tuning;
 
 typedef struct onion_server onion_server;
@@ -181,6 +184,9 @@ int onion_gpu_tap_convert(onion_server *srv, uint64_t client_id, const uint64_t 
 /* evaluate_first_dim (src/server.cpp:123-175): fst selection cts in
  * coefficient form -> other cts in coefficient form. */
 int onion_gpu_tap_first_dim(onion_server *srv, const uint64_t *sel, uint64_t *out);
+/* evaluate_first_dim for nq (1..64) queries at once through the batched
+ * kernels: nq x fst selection cts in, nq x other cts out (coefficient form). */
+int onion_gpu_tap_first_dim_batch(onion_server *srv, uint32_t nq, const uint64_t *sel, uint64_t *out);
 /* other_dim_mux (src/server.cpp:295-334): n cts and one RGSW (2l rows x 2N,
  * NTT form) -> n/2 cts. */
 int onion_gpu_tap_mux(onion_server *srv, const uint64_t *cts, uint32_t n, const uint64_t *rgsw,

@@ -39,7 +39,7 @@ class OnionStageTimes(C.Structure):
 
 class OnionTuning(C.Structure):
     _fields_ = [("fuse_min", C.c_uint32), ("fd_rows_b1", C.c_uint32), ("fd_tile_rows", C.c_uint32),
-                ("fd_b1_variant", C.c_uint32), ("use_graph", C.c_uint32)]
+                ("fd_b1_variant", C.c_uint32), ("use_graph", C.c_uint32), ("fd_mfma_min", C.c_uint32)]
 
 
 class OnionError(RuntimeError):
@@ -86,6 +86,7 @@ _SIGS = {
     "onion_gpu_tap_expand": (C.c_int, [C.c_void_p, C.c_uint64, U64P, U64P]),
     "onion_gpu_tap_convert": (C.c_int, [C.c_void_p, C.c_uint64, U64P, U64P]),
     "onion_gpu_tap_first_dim": (C.c_int, [C.c_void_p, U64P, U64P]),
+    "onion_gpu_tap_first_dim_batch": (C.c_int, [C.c_void_p, C.c_uint32, U64P, U64P]),
     "onion_gpu_tap_mux": (C.c_int, [C.c_void_p, U64P, C.c_uint32, U64P, U64P]),
     "onion_gpu_tap_ntt": (C.c_int, [C.c_void_p, C.c_int, C.c_int, U64P, C.c_uint32]),
     "onion_gpu_tap_galois": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, U64P, U64P]),

@@ -1,0 +1,262 @@
+// k_first_dim_mfma.hpp -- the batched first-dimension product on the int8
+// matrix cores, exact.
+//
+// The product is the reference's mat_mat_128 (src/matrix.cpp:220-235): for
+// every NTT slot l, C_l[r][c] = sum_j A_l[r][j] * B_l[j][c] as an integer
+// wrapping mod 2^128, then reduced mod q (delay_modulus, src/server.cpp:
+// 178-293).  Every operand is < q < 2^60.
+//
+// Signed base-256 digits.  A value a < 2^60 is written as
+//   a = sum_{i<8} d_i 256^i,  d_i in [-128, 127],
+// whose two's-complement bytes are those of V = (a + K) ^ K with
+// K = 0x8080808080808080 (a + K cannot overflow, and byte i of a + K is
+// d_i + 128).  Then
+//   C = sum_{m<15} 256^m S_m,   S_m = sum_{i+k=m} sum_j dA_i[j] dB_k[j],
+// and each S_m is a sum of at most 8 * fst products of magnitude <= 2^14, so
+// it is exact in int32 for fst <= 2^14.  One v_mfma_i32_32x32x32_i8 adds a
+// 32x32 tile of dA_i * dB_k over 32 values of j; a wave issues the 64 (i, k)
+// digit products of each 32-wide step of j into 15 diagonal accumulators.
+// The epilogue forms C mod 2^128 from the 15 signed sums and applies the same
+// Barrett reduction as the 64-bit path, so results are bit-identical.
+//
+// Operand layout.  MFMA lane l holds 16 bytes of row (l % 32) of the A tile
+// and 16 bytes of column (l % 32) of the B tile; for lane group g = l / 32 the
+// bytes are j = 8g .. 8g+7 and 16+8g .. 16+8g+7 of the 32-wide step (any
+// assignment works as long as A and B use the same one).  Digits therefore
+// travel in blocks of 8 consecutive j: a block holds the 8 digit planes of 8
+// values, plane i in bytes 8i .. 8i+7, byte t = value t (64 bytes).
+//   B planes  Bp[l][j/8][col][64 B], col = 2*query + poly, padded to NCP
+//             columns (NCP = 32, 64 or 128; written by k_sel_planes)
+//   A         read from the database words and turned into planes in
+//             registers
+//   output    out2[l][r][c] (uint64, canonical mod q, c < 2*nq), turned into
+//             coefficient-form ciphertexts by k_ntt_inv_gather.
+#pragma once
+
+#include "k_first_dim.hpp"
+
+namespace onion {
+
+typedef int v4i_t __attribute__((ext_vector_type(4)));
+typedef int v16i_t __attribute__((ext_vector_type(16)));
+
+constexpr uint64_t kDigitBias = 0x8080808080808080ULL;
+
+__device__ __forceinline__ uint64_t signed_digits(uint64_t a) { return (a + kDigitBias) ^ kDigitBias; }
+
+// 4x4 byte transpose: y_i byte t = x_t byte i.
+__device__ __forceinline__ void transpose4(uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3, uint32_t &y0,
+                                           uint32_t &y1, uint32_t &y2, uint32_t &y3) {
+  // v_perm_b32(S0, S1, sel): selector 0-3 picks bytes of S1, 4-7 bytes of S0
+  const uint32_t p0 = __builtin_amdgcn_perm(x1, x0, 0x06020400u);  // x0.0 x1.0 x0.2 x1.2
+  const uint32_t p1 = __builtin_amdgcn_perm(x1, x0, 0x07030501u);  // x0.1 x1.1 x0.3 x1.3
+  const uint32_t p2 = __builtin_amdgcn_perm(x3, x2, 0x06020400u);
+  const uint32_t p3 = __builtin_amdgcn_perm(x3, x2, 0x07030501u);
+  y0 = __builtin_amdgcn_perm(p2, p0, 0x05040100u);
+  y2 = __builtin_amdgcn_perm(p2, p0, 0x07060302u);
+  y1 = __builtin_amdgcn_perm(p3, p1, 0x05040100u);
+  y3 = __builtin_amdgcn_perm(p3, p1, 0x07060302u);
+}
+
+// Eight values (already in signed-digit form) -> one 64-byte plane block:
+// out[2i + h] = digit i of values 4h .. 4h+3.
+__device__ __forceinline__ void planes8(const uint64_t (&w)[8], uint32_t (&out)[16]) {
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint64_t *v = w + 4 * h;
+    transpose4((uint32_t)v[0], (uint32_t)v[1], (uint32_t)v[2], (uint32_t)v[3], out[h], out[2 + h], out[4 + h],
+               out[6 + h]);
+    transpose4((uint32_t)(v[0] >> 32), (uint32_t)(v[1] >> 32), (uint32_t)(v[2] >> 32), (uint32_t)(v[3] >> 32),
+               out[8 + h], out[10 + h], out[12 + h], out[14 + h]);
+  }
+}
+
+// Selection vectors as digit planes: block (j-block jb of 8 leaves, poly c)
+// of query b NTTs its 8 polynomials and writes, for each slot, one 64-byte
+// plane block to Bp[slot][jb][2b + c].  grid = (fst/8 * 2, nq); fst % 8 == 0.
+__global__ __launch_bounds__(kNttThreads) void k_sel_planes(const uint64_t *leaves, uint64_t leaf_stride,
+                                                            uint8_t *Bp, uint32_t fst, uint32_t ncp, NttTab T) {
+  __shared__ uint64_t lds[kNttLds];
+  const uint32_t jb = blockIdx.x / 2, c = blockIdx.x % 2, b = blockIdx.y;
+  uint64_t xs[8][8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    uint64_t x[8];
+    load_natural(x, leaves + (size_t)b * leaf_stride + (size_t)(jb * 8 + j) * 2 * kN + (size_t)c * kN);
+    ntt_fwd(x, lds, T);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) xs[j][r] = signed_digits(ntt_fwd_final(x[r], T.q));
+  }
+  const uint32_t col = 2 * b + c, nb = fst / 8;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const uint32_t slot = 8 * threadIdx.x + r;
+    uint64_t w[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) w[j] = xs[j][r];
+    uint32_t o[16];
+    planes8(w, o);
+    uint4 *dst = reinterpret_cast<uint4 *>(Bp + (((size_t)slot * nb + jb) * ncp + col) * 64);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dst[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+  }
+}
+
+__device__ __forceinline__ v4i_t operand(const uint32_t (&p0)[16], const uint32_t (&p1)[16], int i) {
+  v4i_t v;
+  v[0] = (int)p0[2 * i];
+  v[1] = (int)p0[2 * i + 1];
+  v[2] = (int)p1[2 * i];
+  v[3] = (int)p1[2 * i + 1];
+  return v;
+}
+
+// Raw operands of one 32-wide step of j for this lane: 2 x 8 database words
+// of its row, 2 plane blocks of its column.
+struct StepRaw {
+  uint4 a[8];  // words j0+8g .. +7 (a[0..3]) and j0+16+8g .. +7 (a[4..7])
+  uint4 b[8];  // plane blocks of j-blocks 4s+g (b[0..3]) and 4s+2+g (b[4..7])
+};
+
+__device__ __forceinline__ void load_step(StepRaw &R, const uint64_t *arow, bool row_ok, const uint8_t *bcol,
+                                          uint32_t s, uint32_t g, uint32_t ncp) {
+  const uint4 *a0 = reinterpret_cast<const uint4 *>(arow + 32 * s + 8 * g);
+  const uint4 *a1 = reinterpret_cast<const uint4 *>(arow + 32 * s + 16 + 8 * g);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    R.a[q] = row_ok ? a0[q] : make_uint4(0, 0, 0, 0);
+    R.a[4 + q] = row_ok ? a1[q] : make_uint4(0, 0, 0, 0);
+  }
+  const uint4 *b0 = reinterpret_cast<const uint4 *>(bcol + (size_t)(4 * s + g) * ncp * 64);
+  const uint4 *b1 = reinterpret_cast<const uint4 *>(bcol + (size_t)(4 * s + 2 + g) * ncp * 64);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    R.b[q] = b0[q];
+    R.b[4 + q] = b1[q];
+  }
+}
+
+__device__ __forceinline__ void a_planes(const uint4 *a, uint32_t (&out)[16]) {
+  uint64_t w[8];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    w[2 * q] = signed_digits(((uint64_t)a[q].y << 32) | a[q].x);
+    w[2 * q + 1] = signed_digits(((uint64_t)a[q].w << 32) | a[q].z);
+  }
+  planes8(w, out);
+}
+
+__device__ __forceinline__ void b_planes(const uint4 *b, uint32_t (&out)[16]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    out[4 * q] = b[q].x;
+    out[4 * q + 1] = b[q].y;
+    out[4 * q + 2] = b[q].z;
+    out[4 * q + 3] = b[q].w;
+  }
+}
+
+// C mod 2^128 from the 15 diagonal sums of one output, then mod q.
+__device__ __forceinline__ uint64_t diagonals_mod_q(const v16i_t (&acc)[15], int e, const Mod &mq) {
+  int64_t P0 = 0, P1 = 0, P2 = 0;  // weights 2^0, 2^40, 2^80; |P| < 2^61
+#pragma unroll
+  for (int d = 0; d < 5; ++d) {
+    P0 += (int64_t)acc[d][e] * ((int64_t)1 << (8 * d));
+    P1 += (int64_t)acc[d + 5][e] * ((int64_t)1 << (8 * d));
+    P2 += (int64_t)acc[d + 10][e] * ((int64_t)1 << (8 * d));
+  }
+  uint64_t lo = (uint64_t)P0, hi = (uint64_t)(P0 >> 63);
+  add128(lo, hi, (uint64_t)P1 << 40, (uint64_t)(P1 >> 24));
+  hi += (uint64_t)P2 << 16;
+  return barrett128(lo, hi, mq);
+}
+
+// First dimension for 2..64 queries.  CB = column blocks of 32 (NCP = 32 CB
+// padded columns); the workgroup's four waves cover RB = 4 / CB row blocks of
+// 32 rows x CB column blocks.  grid.x = N * n_rt with n_rt row tiles of
+// 32 RB rows; all row tiles of a slot run on one XCD (block id mod 8), so the
+// slot's plane blocks are read from that XCD's L2.  Requires fst % 32 == 0.
+template <int CB>
+__global__ __launch_bounds__(256) void k_first_dim_mfma(const uint64_t *__restrict__ db,
+                                                        const uint8_t *__restrict__ Bp,
+                                                        uint64_t *__restrict__ out2, uint32_t fst,
+                                                        uint32_t other, uint64_t num_pt, uint32_t ncols,
+                                                        uint32_t n_rt, Mod mq) {
+  constexpr int RB = 4 / CB, NCP = 32 * CB;
+  const uint32_t L = blockIdx.x, xcd = L & 7, seq = L >> 3;
+  const uint32_t slot = (seq / n_rt) * 8 + xcd, rt = seq % n_rt;
+  const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 5, li = l & 31;
+  const uint32_t rb = w / CB, cb = w % CB;
+  const uint32_t row0 = rt * (32 * RB) + rb * 32;
+  const uint32_t row = row0 + li, col = cb * 32 + li;
+  const bool row_ok = row < other;
+  const uint64_t *arow = db + (size_t)slot * num_pt + (size_t)(row_ok ? row : 0) * fst;
+  const uint8_t *bcol = Bp + (size_t)slot * (fst / 8) * NCP * 64 + (size_t)col * 64;
+
+  v16i_t acc[15];
+#pragma unroll
+  for (int d = 0; d < 15; ++d) acc[d] = (v16i_t){0};
+
+  // one raw buffer: the step's words become planes, then the buffer takes the
+  // next step's loads while the 64 products run
+  const uint32_t steps = fst / 32;
+  StepRaw raw;
+  load_step(raw, arow, row_ok, bcol, 0, g, NCP);
+  for (uint32_t s = 0; s < steps; ++s) {
+    uint32_t pa0[16], pa1[16], pb0[16], pb1[16];
+    a_planes(raw.a, pa0);
+    a_planes(raw.a + 4, pa1);
+    b_planes(raw.b, pb0);
+    b_planes(raw.b + 4, pb1);
+    if (s + 1 < steps) load_step(raw, arow, row_ok, bcol, s + 1, g, NCP);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const v4i_t A = operand(pa0, pa1, i);
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        acc[i + k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, operand(pb0, pb1, k), acc[i + k], 0, 0, 0);
+    }
+  }
+
+  // output (m, n) of register e: n = lane % 32, m = 8 (e / 4) + 4 g + e % 4
+  if (col < ncols) {
+    uint64_t *dst = out2 + (size_t)slot * other * ncols + col;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const uint32_t m = row0 + 8 * (e / 4) + 4 * g + (e % 4);
+      const uint64_t v = diagonals_mod_q(acc, e, mq);
+      if (m < other) dst[(size_t)m * ncols] = v;
+    }
+  }
+}
+
+// Inverse NTT of the first-dimension outputs gathered from out2[slot][r][c]
+// into fd (query b = c / 2, poly c % 2 of ciphertext r, coefficient form).
+// One workgroup per (r, c); when other % 8 == 0 the rows are dealt to XCDs
+// (r = xcd mod 8) with c fastest, so the workgroups that share a cache line
+// of out2 run on one XCD at about the same time.
+__global__ __launch_bounds__(kNttThreads) void k_ntt_inv_gather(const uint64_t *__restrict__ out2,
+                                                                uint32_t other, uint32_t ncols, uint64_t *fd,
+                                                                uint64_t fd_stride, NttTab T) {
+  __shared__ uint64_t lds[kNttLds];
+  uint32_t r, c;
+  if (other % 8 == 0) {
+    const uint32_t xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+    r = 8 * (k / ncols) + xcd;
+    c = k % ncols;
+  } else {
+    r = blockIdx.x / ncols;
+    c = blockIdx.x % ncols;
+  }
+  const size_t sstride = (size_t)other * ncols;
+  const uint64_t *src = out2 + (size_t)r * ncols + c;
+  uint64_t x[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) x[j] = src[(size_t)(8 * threadIdx.x + j) * sstride];
+  ntt_inv(x, lds, T);
+  uint64_t *dst = fd + (size_t)(c >> 1) * fd_stride + ((size_t)r * 2 + (c & 1)) * kN;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) dst[threadIdx.x + 256 * j] = ntt_inv_final(x[j], T);
+}
+
+}  // namespace onion

@@ -30,6 +30,7 @@
 #include "k_db.hpp"
 #include "k_expand.hpp"
 #include "k_first_dim.hpp"
+#include "k_first_dim_mfma.hpp"
 #include "k_gsw.hpp"
 #include "k_ntt.hpp"
 #include "k_tail.hpp"
@@ -138,7 +139,7 @@ struct onion_server {
   uint8_t *pack = nullptr;
   // measured on MI355X at 512x1024: 8 nontemporal 16-byte loads in flight per
   // lane and 256-row blocks stream the database at 94% of a plain read kernel
-  onion_tuning tune{1024, 256, 4, 3, 0};
+  onion_tuning tune{1024, 256, 4, 3, 0, 2};
   uint64_t tune_epoch = 1;
   Gate gate;
   mutable std::mutex last_mu;
@@ -324,10 +325,41 @@ void launch_fd_bat(const onion_server *s, Lane *L, uint32_t nq) {
     launch_fd_bat_tr<4, CPAD>(s, L, nq);
 }
 
+// The int8-MFMA product serves batches of at least tune.fd_mfma_min queries
+// when fst is a multiple of 32 and the lane's digit scratch holds the
+// slot-major outputs (N x other x 2nq words).
+bool use_mfma(const onion_server *s, const Lane *L, uint32_t nq) {
+  return nq >= 2 && s->tune.fd_mfma_min != 0 && nq >= s->tune.fd_mfma_min && s->P.fst_dim % 32 == 0 &&
+         (uint64_t)L->cap * s->digit_words >= (uint64_t)kN * s->P.other_dim * 2 * nq;
+}
+
+void run_first_dim_mfma(const onion_server *s, Lane *L, uint32_t nq, const uint64_t *sel, uint64_t sel_stride,
+                        bool time_core) {
+  const uint32_t fst = s->P.fst_dim, other = s->P.other_dim, ncp = cpad_for(nq), ncols = 2 * nq;
+  uint8_t *Bp = reinterpret_cast<uint8_t *>(L->Qs);
+  k_sel_planes<<<dim3(fst / 8 * 2, nq), kNttThreads, 0, L->st>>>(sel, sel_stride, Bp, fst, ncp, s->Tq);
+  if (time_core) (void)hipEventRecord(L->ev[6], L->st);
+  const uint32_t rb = 128 / ncp;  // row blocks of 32 per workgroup
+  const uint32_t n_rt = (other + 32 * rb - 1) / (32 * rb);
+  const dim3 grid(kN * n_rt);
+  if (ncp == 32)
+    k_first_dim_mfma<1><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
+  else if (ncp == 64)
+    k_first_dim_mfma<2><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
+  else
+    k_first_dim_mfma<4><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
+  if (time_core) (void)hipEventRecord(L->ev[7], L->st);
+  k_ntt_inv_gather<<<other * ncols, kNttThreads, 0, L->st>>>(L->digits, other, ncols, L->fd, s->fd_words, s->Tq);
+}
+
 // First dimension: fst selection cts per query (coefficient form, stride
 // sel_stride) -> L->fd (other cts per query, coefficient form).
 void run_first_dim(const onion_server *s, Lane *L, uint32_t nq, const uint64_t *sel, uint64_t sel_stride,
                    bool time_core) {
+  if (use_mfma(s, L, nq)) {
+    run_first_dim_mfma(s, L, nq, sel, sel_stride, time_core);
+    return;
+  }
   const uint32_t fst = s->P.fst_dim, other = s->P.other_dim, cpad = cpad_for(nq);
   const uint64_t num_pt = s->I.num_pt;
   k_sel_ntt<<<dim3(fst * 2, nq), kNttThreads, 0, L->st>>>(sel, sel_stride, L->Qs, fst, cpad, s->Tq);
@@ -893,6 +925,23 @@ int onion_gpu_tap_first_dim(onion_server *s, const uint64_t *sel, uint64_t *out)
   run_first_dim(s, L, 1, s->io, 0, false);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, L->fd, (size_t)s->P.other_dim * 2 * kN * 8, hipMemcpyDeviceToHost, L->st));
+  HIPCHK(hipStreamSynchronize(L->st));
+  return ONION_OK;
+}
+
+int onion_gpu_tap_first_dim_batch(onion_server *s, uint32_t nq, const uint64_t *sel, uint64_t *out) {
+  if (!s || !sel || !out) return fail(ONION_EINVAL, "null argument");
+  if (nq < 1 || nq > kMaxBatch) return fail(ONION_EINVAL, "nq must be in [1, 64]");
+  Shared g(s->gate);
+  Lane *L = s->lanes[0].get();
+  std::lock_guard<std::mutex> lk(L->mu);
+  int rc;
+  if ((rc = check_db(s)) || (rc = ensure_scratch(s, L, nq))) return rc;
+  const size_t sel_bytes = (size_t)s->P.fst_dim * 2 * kN * 8, out_bytes = (size_t)s->P.other_dim * 2 * kN * 8;
+  HIPCHK(hipMemcpy2DAsync(L->trees, s->tree_words * 8, sel, sel_bytes, sel_bytes, nq, hipMemcpyHostToDevice, L->st));
+  run_first_dim(s, L, nq, L->trees, s->tree_words, false);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy2DAsync(out, out_bytes, L->fd, s->fd_words * 8, out_bytes, nq, hipMemcpyDeviceToHost, L->st));
   HIPCHK(hipStreamSynchronize(L->st));
   return ONION_OK;
 }

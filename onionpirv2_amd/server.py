@@ -166,6 +166,14 @@ class GpuServer:
         check(self._L.onion_gpu_tap_first_dim(self._h, u64ptr(s), u64ptr(out)))
         return out
 
+    def tap_first_dim_batch(self, sel) -> np.ndarray:
+        """sel: nq x fst x 2N selection cts -> nq x other x 2N."""
+        s = _u64(sel).reshape(-1)
+        nq = s.size // (self.fst_dim * 2 * self.N)
+        out = np.zeros(nq * self.other_dim * 2 * self.N, np.uint64)
+        check(self._L.onion_gpu_tap_first_dim_batch(self._h, nq, u64ptr(s), u64ptr(out)))
+        return out.reshape(nq, -1)
+
     def tap_mux(self, cts, rgsw) -> np.ndarray:
         c = _u64(cts).reshape(-1)
         g = _u64(rgsw).reshape(-1)

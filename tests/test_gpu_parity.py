@@ -195,6 +195,38 @@ def test_first_dim_wraps_mod_2_128():
         exp = o.first_dim(db.reshape(-1), sel)
         got = srv.tap_first_dim(sel)
         assert np.array_equal(got, exp)
+        # the batched kernels (matrix-core digit products, then the 64-bit tile)
+        rng = np.random.default_rng(8)
+        sel2 = rng.integers(0, o.q, size=(o.fst, 2 * N), dtype=np.uint64)
+        batch = np.stack([sel, sel2, sel])
+        for mfma_min in (2, 0):
+            srv.set_tuning(fd_mfma_min=mfma_min)
+            gb = srv.tap_first_dim_batch(batch)
+            assert np.array_equal(gb[0], exp) and np.array_equal(gb[2], exp), f"fd_mfma_min {mfma_min}"
+            assert np.array_equal(gb[1], o.first_dim(db.reshape(-1), sel2))
+
+
+@pytest.mark.parametrize("nq", [2, 3, 16, 17, 33, 64])
+def test_first_dim_batch_matches_oracle(cfg1, nq):
+    """The batched first-dimension kernels (int8 matrix cores by default, the
+    64-bit VALU tile with fd_mfma_min = 0) against the oracle, query by query.
+    nq = 2..16 / 17..32 / 33..64 take 1 / 2 / 4 column blocks of 32; other =
+    32 leaves 96 of the 128 rows of a workgroup masked."""
+    o, srv = cfg1["o"], cfg1["srv"]
+    rng = np.random.default_rng(100 + nq)
+    sel = rng.integers(0, o.q, size=(nq, o.fst, 2 * N), dtype=np.uint64)
+    sel[0, :, :8] = o.q - 1
+    sel[-1, :3] = 0
+    got = srv.tap_first_dim_batch(sel)
+    check = sorted({0, nq // 2, nq - 1})
+    for b in check:
+        assert np.array_equal(got[b], o.first_dim(cfg1["db"], sel[b])), f"query {b}"
+    saved = srv.get_tuning()
+    try:
+        srv.set_tuning(fd_mfma_min=0)
+        assert np.array_equal(srv.tap_first_dim_batch(sel), got)
+    finally:
+        srv.set_tuning(**saved)
 
 
 @pytest.mark.parametrize("nq", [3, 17, 40])
@@ -232,7 +264,8 @@ def test_answer_batch_matches_single_and_oracle(cfg1, nq):
 @pytest.mark.parametrize("tuning", [dict(fuse_min=1), dict(fuse_min=1 << 30),
                                     dict(fd_rows_b1=64), dict(fd_tile_rows=2),
                                     dict(fd_b1_variant=1), dict(fd_b1_variant=2),
-                                    dict(fd_b1_variant=3)])
+                                    dict(fd_b1_variant=3), dict(fd_mfma_min=0),
+                                    dict(fd_mfma_min=6)])
 def test_tuning_choices_are_bit_identical(cfg1, tuning):
     """Every launch-shape choice (fused vs split external products, first-dim
     tile shapes) gives the oracle's bits, for a single query and a batch."""
