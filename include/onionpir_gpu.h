@@ -90,6 +90,9 @@ typedef struct onion_tuning {
   uint32_t fd_mfma_min;   /* batches of at least this many queries (and >= 2)
                              take the first dimension on the int8 matrix
                              cores (exact signed-digit products); 0 = never  */
+  uint32_t fd_mfma_variant; /* 0: operands loaded to registers one step
+                               ahead; 1: direct-to-LDS loads three steps
+                               ahead, shared by the workgroup's waves        */
 } onion_tuning;
 
 typedef struct onion_server onion_server;

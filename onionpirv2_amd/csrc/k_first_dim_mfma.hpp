@@ -230,6 +230,173 @@ __global__ __launch_bounds__(256) void k_first_dim_mfma(const uint64_t *__restri
   }
 }
 
+// ---- LDS-pipelined variant ------------------------------------------------------------
+//
+// Same tiles and arithmetic as k_first_dim_mfma, but the operands travel
+// global -> LDS by direct-to-LDS loads (global_load_lds_dwordx4), three steps
+// deep, so the matrix cores are not stalled by HBM latency and no VGPRs hold
+// loads in flight.  A step's operands are 16-byte "pieces": piece p of an A
+// row block holds, for reading lane l, the two database words
+//   p < 4:  j = 32 s + 8 g + 2 p, +1        p >= 4:  j = 32 s + 16 + 8 g + 2 (p-4), +1
+// of row (l % 32), and piece p of a B column block holds quarter (p % 4) of
+// the plane block of j-block 4 s + g (p < 4) or 4 s + 2 + g (p >= 4) of column
+// (l % 32); g = l / 32.  A piece is 1 KiB in LDS with lane l's 16 bytes at
+// 16 l, so both the DMA write and the reading ds_read_b128 are contiguous.
+// The RB row blocks and CB column blocks of a step (8 (RB + CB) pieces) are
+// loaded by the four waves together and shared through LDS.
+
+constexpr int kMfmaStages = 3;
+
+__device__ __forceinline__ void lds_dma16(const void *g, void *lds) {
+  __builtin_amdgcn_global_load_lds(g, (void __attribute__((address_space(3))) *)lds, 16, 0, 0);
+}
+
+// s_waitcnt vmcnt(n), other counters untouched (gfx9 encoding).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt(0x0F70 | (N & 15) | ((N >> 4) << 14));
+}
+
+template <int CB>
+struct MfmaPipe {
+  static constexpr int RB = 4 / CB;
+  static constexpr int PIECES = 8 * (RB + CB);  // per step
+  static constexpr int PER_WAVE = PIECES / 4;   // loads per wave per step
+  static constexpr int STAGE_BYTES = PIECES * 1024;
+};
+
+// Issue this wave's share of the loads of step s into LDS stage `stage`.
+template <int CB>
+__device__ __forceinline__ void issue_step(uint8_t *lds, int stage, uint32_t s, const uint64_t *a_base,
+                                           uint32_t fst, uint32_t row_base, uint32_t other, const uint8_t *b_base,
+                                           uint32_t w, uint32_t l) {
+  using P = MfmaPipe<CB>;
+  const uint32_t g = l >> 5, li = l & 31;
+#pragma unroll
+  for (int i = 0; i < P::PER_WAVE; ++i) {
+    const uint32_t piece = w + 4 * i;  // wave-uniform
+    uint8_t *dst = lds + stage * P::STAGE_BYTES + piece * 1024;
+    const uint32_t p = piece & 7;
+    const uint32_t blk = piece >> 3;
+    if (blk < (uint32_t)P::RB) {
+      uint32_t row = row_base + blk * 32 + li;
+      row = row < other ? row : 0;  // masked rows read row 0; their outputs are not stored
+      const uint32_t j = 32 * s + (p < 4 ? 8 * g + 2 * p : 16 + 8 * g + 2 * (p - 4));
+      lds_dma16(a_base + (size_t)row * fst + j, dst);
+    } else {
+      const uint32_t cb = blk - P::RB, col = cb * 32 + li;
+      const uint32_t jb = 4 * s + g + (p < 4 ? 0 : 2);
+      lds_dma16(b_base + ((size_t)jb * (32 * CB) + col) * 64 + 16 * (p & 3), dst);
+    }
+  }
+}
+
+typedef unsigned v4u_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint4 to_uint4(v4u_t v) { return make_uint4(v[0], v[1], v[2], v[3]); }
+
+// The 16 pieces (8 of A at byte address a, 8 of B at b, 1 KiB apart) of one
+// step from LDS.  Written as one asm block that waits for its own reads: the
+// compiler, which cannot tell these reads from the direct-to-LDS loads still
+// in flight for later steps, would otherwise wait for all of them.
+__device__ __forceinline__ void lds_read_step(uint4 (&ra)[8], uint4 (&rb)[8], uint32_t a, uint32_t b) {
+  v4u_t x0, x1, x2, x3, x4, x5, x6, x7, y0, y1, y2, y3, y4, y5, y6, y7;
+  asm volatile(
+      "ds_read_b128 %0, %16\n"
+      "ds_read_b128 %1, %16 offset:1024\n"
+      "ds_read_b128 %2, %16 offset:2048\n"
+      "ds_read_b128 %3, %16 offset:3072\n"
+      "ds_read_b128 %4, %16 offset:4096\n"
+      "ds_read_b128 %5, %16 offset:5120\n"
+      "ds_read_b128 %6, %16 offset:6144\n"
+      "ds_read_b128 %7, %16 offset:7168\n"
+      "ds_read_b128 %8, %17\n"
+      "ds_read_b128 %9, %17 offset:1024\n"
+      "ds_read_b128 %10, %17 offset:2048\n"
+      "ds_read_b128 %11, %17 offset:3072\n"
+      "ds_read_b128 %12, %17 offset:4096\n"
+      "ds_read_b128 %13, %17 offset:5120\n"
+      "ds_read_b128 %14, %17 offset:6144\n"
+      "ds_read_b128 %15, %17 offset:7168\n"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(x0), "=&v"(x1), "=&v"(x2), "=&v"(x3), "=&v"(x4), "=&v"(x5), "=&v"(x6), "=&v"(x7), "=&v"(y0),
+        "=&v"(y1), "=&v"(y2), "=&v"(y3), "=&v"(y4), "=&v"(y5), "=&v"(y6), "=&v"(y7)
+      : "v"(a), "v"(b)
+      : "memory");
+  ra[0] = to_uint4(x0); ra[1] = to_uint4(x1); ra[2] = to_uint4(x2); ra[3] = to_uint4(x3);
+  ra[4] = to_uint4(x4); ra[5] = to_uint4(x5); ra[6] = to_uint4(x6); ra[7] = to_uint4(x7);
+  rb[0] = to_uint4(y0); rb[1] = to_uint4(y1); rb[2] = to_uint4(y2); rb[3] = to_uint4(y3);
+  rb[4] = to_uint4(y4); rb[5] = to_uint4(y5); rb[6] = to_uint4(y6); rb[7] = to_uint4(y7);
+}
+
+template <int CB>
+__global__ __launch_bounds__(256) void k_first_dim_mfma_lds(const uint64_t *__restrict__ db,
+                                                            const uint8_t *__restrict__ Bp,
+                                                            uint64_t *__restrict__ out2, uint32_t fst,
+                                                            uint32_t other, uint64_t num_pt, uint32_t ncols,
+                                                            uint32_t n_rt, Mod mq) {
+  using P = MfmaPipe<CB>;
+  constexpr int RB = P::RB, NCP = 32 * CB;
+  __shared__ __attribute__((aligned(16))) uint8_t lds[kMfmaStages * P::STAGE_BYTES];
+  const uint32_t L = blockIdx.x, xcd = L & 7, seq = L >> 3;
+  const uint32_t slot = (seq / n_rt) * 8 + xcd, rt = seq % n_rt;
+  const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 5, li = l & 31;
+  const uint32_t rb = w / CB, cb = w % CB;
+  const uint32_t row_base = rt * (32 * RB);
+  const uint64_t *a_base = db + (size_t)slot * num_pt;
+  const uint8_t *b_base = Bp + (size_t)slot * (fst / 8) * NCP * 64;
+
+  v16i_t acc[15];
+#pragma unroll
+  for (int d = 0; d < 15; ++d) acc[d] = (v16i_t){0};
+
+  // LDS byte offset of the staging area (generic -> local address space)
+  const uint32_t lds_off =
+      (uint32_t)(uintptr_t)(__attribute__((address_space(3))) uint8_t *)lds + l * 16;
+  const uint32_t steps = fst / 32;
+  issue_step<CB>(lds, 0, 0, a_base, fst, row_base, other, b_base, w, l);
+  if (steps > 1) issue_step<CB>(lds, 1, 1, a_base, fst, row_base, other, b_base, w, l);
+  for (uint32_t s = 0; s < steps; ++s) {
+    // this wave's loads of step s are done once at most one later step is in flight
+    if (s + 1 < steps)
+      wait_vmcnt<P::PER_WAVE>();
+    else
+      wait_vmcnt<0>();
+    // every wave's pieces of step s landed, and every wave has read step s - 1
+    // (its reads wait for completion), so stage (s + 2) % 3 is free.  A plain
+    // s_barrier: the loads of later steps stay in flight across it.
+    asm volatile("s_barrier" ::: "memory");
+    if (s + 2 < steps) issue_step<CB>(lds, (s + 2) % kMfmaStages, s + 2, a_base, fst, row_base, other, b_base, w, l);
+    const uint32_t st = lds_off + (s % kMfmaStages) * P::STAGE_BYTES;
+    uint4 ra[8], rbv[8];
+    lds_read_step(ra, rbv, st + rb * 8 * 1024, st + (RB + cb) * 8 * 1024);
+    uint32_t pa0[16], pa1[16], pb0[16], pb1[16];
+    a_planes(ra, pa0);
+    a_planes(ra + 4, pa1);
+    b_planes(rbv, pb0);
+    b_planes(rbv + 4, pb1);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const v4i_t A = operand(pa0, pa1, i);
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        acc[i + k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, operand(pb0, pb1, k), acc[i + k], 0, 0, 0);
+    }
+  }
+
+  const uint32_t row0 = row_base + rb * 32, col = cb * 32 + li;
+  if (col < ncols) {
+    uint64_t *dst = out2 + (size_t)slot * other * ncols + col;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const uint32_t m = row0 + 8 * (e / 4) + 4 * g + (e % 4);
+      const uint64_t v = diagonals_mod_q(acc, e, mq);
+      if (m < other) dst[(size_t)m * ncols] = v;
+    }
+  }
+}
+
 // Inverse NTT of the first-dimension outputs gathered from out2[slot][r][c]
 // into fd (query b = c / 2, poly c % 2 of ciphertext r, coefficient form).
 // One workgroup per (r, c); when other % 8 == 0 the rows are dealt to XCDs

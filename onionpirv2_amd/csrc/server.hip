@@ -139,7 +139,7 @@ struct onion_server {
   uint8_t *pack = nullptr;
   // measured on MI355X at 512x1024: 8 nontemporal 16-byte loads in flight per
   // lane and 256-row blocks stream the database at 94% of a plain read kernel
-  onion_tuning tune{1024, 256, 4, 3, 0, 2};
+  onion_tuning tune{1024, 256, 4, 3, 0, 2, 1};
   uint64_t tune_epoch = 1;
   Gate gate;
   mutable std::mutex last_mu;
@@ -342,12 +342,23 @@ void run_first_dim_mfma(const onion_server *s, Lane *L, uint32_t nq, const uint6
   const uint32_t rb = 128 / ncp;  // row blocks of 32 per workgroup
   const uint32_t n_rt = (other + 32 * rb - 1) / (32 * rb);
   const dim3 grid(kN * n_rt);
-  if (ncp == 32)
-    k_first_dim_mfma<1><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
-  else if (ncp == 64)
-    k_first_dim_mfma<2><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
-  else
-    k_first_dim_mfma<4><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
+  const bool pipe = s->tune.fd_mfma_variant == 1;
+  if (ncp == 32) {
+    if (pipe)
+      k_first_dim_mfma_lds<1><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
+    else
+      k_first_dim_mfma<1><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
+  } else if (ncp == 64) {
+    if (pipe)
+      k_first_dim_mfma_lds<2><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
+    else
+      k_first_dim_mfma<2><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
+  } else {
+    if (pipe)
+      k_first_dim_mfma_lds<4><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
+    else
+      k_first_dim_mfma<4><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
+  }
   if (time_core) (void)hipEventRecord(L->ev[7], L->st);
   k_ntt_inv_gather<<<other * ncols, kNttThreads, 0, L->st>>>(L->digits, other, ncols, L->fd, s->fd_words, s->Tq);
 }
@@ -863,6 +874,7 @@ int onion_gpu_set_tuning(onion_server *s, const onion_tuning *t) {
   if (t->fd_tile_rows != 2 && t->fd_tile_rows != 4) return fail(ONION_EINVAL, "fd_tile_rows must be 2 or 4");
   if (t->fd_b1_variant > 3) return fail(ONION_EINVAL, "fd_b1_variant must be 0..3");
   if (t->use_graph > 1) return fail(ONION_EINVAL, "use_graph must be 0 or 1");
+  if (t->fd_mfma_variant > 1) return fail(ONION_EINVAL, "fd_mfma_variant must be 0 or 1");
   Exclusive x(s->gate);
   s->tune = *t;
   s->tune.fuse_min = std::max(1u, t->fuse_min);

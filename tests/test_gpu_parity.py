@@ -225,6 +225,8 @@ def test_first_dim_batch_matches_oracle(cfg1, nq):
     try:
         srv.set_tuning(fd_mfma_min=0)
         assert np.array_equal(srv.tap_first_dim_batch(sel), got)
+        srv.set_tuning(fd_mfma_min=2, fd_mfma_variant=1 - saved["fd_mfma_variant"])
+        assert np.array_equal(srv.tap_first_dim_batch(sel), got)
     finally:
         srv.set_tuning(**saved)
 
@@ -265,7 +267,7 @@ def test_answer_batch_matches_single_and_oracle(cfg1, nq):
                                     dict(fd_rows_b1=64), dict(fd_tile_rows=2),
                                     dict(fd_b1_variant=1), dict(fd_b1_variant=2),
                                     dict(fd_b1_variant=3), dict(fd_mfma_min=0),
-                                    dict(fd_mfma_min=6)])
+                                    dict(fd_mfma_min=6), dict(fd_mfma_variant=0)])
 def test_tuning_choices_are_bit_identical(cfg1, tuning):
     """Every launch-shape choice (fused vs split external products, first-dim
     tile shapes) gives the oracle's bits, for a single query and a batch."""
