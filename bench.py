@@ -74,6 +74,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-queries", type=int, default=2)
     ap.add_argument("--plain-bits", type=int, default=16)
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the built-in launch shapes instead of tuning on the live database")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the two-lane, serving and multi-database measurements")
     ap.add_argument("--serve-rates", default="100,500,1000,2000", help="Poisson arrival rates (queries/s)")
@@ -116,6 +118,9 @@ def main() -> int:
     queries = np.stack(queries)
     ids = np.arange(B, dtype=np.uint64)
     log(f"[bench] {B} clients keyed, queries built in {time.time() - t0:.2f} s")
+
+    tuning = srv.autotune([1, B], reps=2) if args.autotune else srv.get_tuning()
+    log(f"[bench] tuning {tuning}")
 
     # ---- single-query path ---------------------------------------------------------
     single_lat, cores, stages = [], [], []
@@ -207,6 +212,7 @@ def main() -> int:
                                   "traffic": traffic, "traffic_source": traffic_src, "avg_ms": core_ms,
                                   "algorithmic_bytes": alg_bytes},
         "cpu_baseline": None,
+        "tuning": tuning,
     }
 
     out["single_query_graph"] = graph_single
@@ -239,6 +245,7 @@ def main() -> int:
 
 def single_query_graph(srv, queries, reps, single_resp):
     """The single-query pipeline captured once as a hipGraph and replayed."""
+    saved = srv.get_tuning()
     srv.set_tuning(use_graph=1)
     try:
         srv.answer(0, queries[0])  # capture
@@ -251,7 +258,7 @@ def single_query_graph(srv, queries, reps, single_resp):
             if i in single_resp:
                 same = same and np.array_equal(got, single_resp[i])
     finally:
-        srv.set_tuning(use_graph=0)
+        srv.set_tuning(**saved)
     log(f"[bench] single query (graph) p50 {pct(lat, 50):.3f} ms")
     return {"queries_per_s": 1e3 / float(np.mean(lat)), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
             "bit_exact": bool(same)}

@@ -92,7 +92,9 @@ typedef struct onion_tuning {
                              cores (exact signed-digit products); 0 = never  */
   uint32_t fd_mfma_variant; /* 0: operands loaded to registers one step
                                ahead; 1: direct-to-LDS loads three steps
-                               ahead, shared by the workgroup's waves        */
+                               ahead, shared by the workgroup's waves; 2: as
+                               1 with each tile's digit products split over
+                               two waves (two waves per SIMD)               */
 } onion_tuning;
 
 typedef struct onion_server onion_server;
@@ -158,6 +160,13 @@ int onion_gpu_last_stage_times(const onion_server *srv, onion_stage_times *out);
 
 int onion_gpu_get_tuning(const onion_server *srv, onion_tuning *out);
 int onion_gpu_set_tuning(onion_server *srv, const onion_tuning *tuning);
+/* Pick the tuning on the live database (no reference counterpart): times the
+ * whole pipeline for each of the n_sizes batch sizes (1..64) with a synthetic
+ * client, minimum of `reps` runs, and walks each choice in turn keeping what
+ * lowers the summed device time per query.  Applies the result and writes it
+ * to *chosen (may be NULL).  Waits for batches in flight. */
+int onion_gpu_autotune(onion_server *srv, const uint32_t *batch_sizes, uint32_t n_sizes, uint32_t reps,
+                       onion_tuning *chosen);
 
 /* Concurrent pipelines (no reference counterpart: the reference server is
  * single-threaded).  A server owns n_lanes lanes; each lane has its own HIP

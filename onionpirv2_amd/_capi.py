@@ -79,6 +79,8 @@ _SIGS = {
     "onion_gpu_last_stage_times": (C.c_int, [C.c_void_p, C.POINTER(OnionStageTimes)]),
     "onion_gpu_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(OnionTuning)]),
     "onion_gpu_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(OnionTuning)]),
+    "onion_gpu_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32,
+                                     C.POINTER(OnionTuning)]),
     "onion_gpu_set_lanes": (C.c_int, [C.c_void_p, C.c_uint32]),
     "onion_gpu_answer_batch_lane": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, U64P, U64P, U64P,
                                               C.POINTER(OnionStageTimes)]),

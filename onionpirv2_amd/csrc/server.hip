@@ -139,7 +139,7 @@ struct onion_server {
   uint8_t *pack = nullptr;
   // measured on MI355X at 512x1024: 8 nontemporal 16-byte loads in flight per
   // lane and 256-row blocks stream the database at 94% of a plain read kernel
-  onion_tuning tune{1024, 256, 4, 3, 0, 2, 1};
+  onion_tuning tune{1024, 256, 4, 3, 0, 2, 0};
   uint64_t tune_epoch = 1;
   Gate gate;
   mutable std::mutex last_mu;
@@ -342,8 +342,16 @@ void run_first_dim_mfma(const onion_server *s, Lane *L, uint32_t nq, const uint6
   const uint32_t rb = 128 / ncp;  // row blocks of 32 per workgroup
   const uint32_t n_rt = (other + 32 * rb - 1) / (32 * rb);
   const dim3 grid(kN * n_rt);
-  const bool pipe = s->tune.fd_mfma_variant == 1;
-  if (ncp == 32) {
+  const uint32_t var = s->tune.fd_mfma_variant;
+  const bool pipe = var == 1;
+  if (var == 2) {
+    if (ncp == 32)
+      k_first_dim_mfma_x2<1><<<grid, 512, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
+    else if (ncp == 64)
+      k_first_dim_mfma_x2<2><<<grid, 512, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
+    else
+      k_first_dim_mfma_x2<4><<<grid, 512, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
+  } else if (ncp == 32) {
     if (pipe)
       k_first_dim_mfma_lds<1><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
     else
@@ -541,6 +549,130 @@ int answer_on_lane(onion_server *s, Lane *L, uint32_t n, const uint64_t *ids, co
     sum.first_dim_core += t.first_dim_core;
   }
   if (times) *times = sum;
+  return ONION_OK;
+}
+
+// ---- autotuning -------------------------------------------------------------------
+
+// Device time of one pass of the pipeline for n queries already bound on lane
+// L (minimum over reps, after one untimed pass that also captures the graph).
+int time_pipeline(onion_server *s, Lane *L, uint32_t n, uint32_t reps, double *ms) {
+  double best = 1e30;
+  for (uint32_t r = 0; r <= reps; ++r) {
+    if (s->tune.use_graph) {
+      hipGraphExec_t ex;
+      int rc = graph_for(s, L, n, &ex);
+      if (rc) return rc;
+      HIPCHK(hipEventRecord(L->ev[0], L->st));
+      HIPCHK(hipGraphLaunch(ex, L->st));
+      HIPCHK(hipEventRecord(L->ev[5], L->st));
+    } else {
+      HIPCHK(hipEventRecord(L->ev[0], L->st));
+      answer_device(s, L, n, false);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(L->ev[5], L->st));
+    }
+    HIPCHK(hipEventSynchronize(L->ev[5]));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, L->ev[0], L->ev[5]));
+    if (r > 0) best = std::min(best, (double)t);
+  }
+  *ms = best;
+  return ONION_OK;
+}
+
+// Coordinate descent over the launch-shape choices on the live database with
+// a synthetic client (uniform key words and queries: the work does not depend
+// on the values).  Score = sum over the batch sizes of device ms per query.
+// The caller holds the gate exclusively.
+int autotune_impl(onion_server *s, const uint32_t *sizes, uint32_t n_sizes, uint32_t reps, onion_tuning *out) {
+  Lane *L = s->lanes[0].get();
+  uint32_t maxn = 1;
+  bool has1 = false, hasb = false, has_small = false;
+  for (uint32_t i = 0; i < n_sizes; ++i) {
+    maxn = std::max(maxn, sizes[i]);
+    has1 |= sizes[i] == 1;
+    hasb |= sizes[i] >= 2;
+    has_small |= sizes[i] >= 2 && sizes[i] < 16;
+  }
+  int rc;
+  if ((rc = ensure_scratch(s, L, maxn))) return rc;
+  uint64_t *gk = nullptr, *rk = nullptr;
+  if ((rc = dalloc(&gk, s->I.galois_key_words)) || (rc = dalloc(&rk, s->I.rgsw_key_words))) {
+    dfree(gk);
+    return rc;
+  }
+  struct Cleanup {
+    uint64_t *a, *b;
+    ~Cleanup() { dfree(a); dfree(b); }
+  } cleanup{gk, rk};
+  {
+    std::vector<uint64_t> h(std::max<size_t>(s->I.galois_key_words, s->I.rgsw_key_words));
+    uint64_t key = stream_key(0x5475ULL, 1);
+    for (size_t i = 0; i < h.size(); ++i) h[i] = rand64_keyed(key, i) % s->I.q;
+    HIPCHK(hipMemcpy(gk, h.data(), (size_t)s->I.galois_key_words * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(rk, h.data(), (size_t)s->I.rgsw_key_words * 8, hipMemcpyHostToDevice));
+    std::vector<uint64_t> q(2 * kN);
+    for (uint32_t b = 0; b < maxn; ++b) {
+      for (size_t i = 0; i < q.size(); ++i) q[i] = rand64_keyed(key, (uint64_t)(b + 1) << 32 | i) % s->I.q;
+      HIPCHK(hipMemcpy(L->trees + (size_t)b * s->tree_words + 2 * kN, q.data(), 2 * kN * 8, hipMemcpyHostToDevice));
+      L->h_gk[b] = gk;
+      L->h_rk[b] = rk;
+    }
+    HIPCHK(hipMemcpy(L->gk_ptrs, L->h_gk.data(), maxn * sizeof(void *), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(L->rk_ptrs, L->h_rk.data(), maxn * sizeof(void *), hipMemcpyHostToDevice));
+  }
+  auto score = [&](const onion_tuning &t, double *sc) -> int {
+    s->tune = t;
+    ++s->tune_epoch;
+    double total = 0;
+    for (uint32_t i = 0; i < n_sizes; ++i) {
+      double ms;
+      int r = time_pipeline(s, L, sizes[i], reps, &ms);
+      if (r) return r;
+      total += ms / sizes[i];
+    }
+    *sc = total;
+    return ONION_OK;
+  };
+  onion_tuning best = s->tune;
+  double best_sc;
+  if ((rc = score(best, &best_sc))) return rc;
+  struct Axis {
+    uint32_t onion_tuning::*field;
+    std::vector<uint32_t> values;
+    bool active;
+  };
+  const std::vector<Axis> axes = {
+      {&onion_tuning::fd_mfma_variant, {0, 1, 2}, hasb},
+      {&onion_tuning::fd_mfma_min, {2, 4, 8, 16, 0}, has_small},
+      {&onion_tuning::fd_tile_rows, {2, 4}, hasb},
+      {&onion_tuning::fd_b1_variant, {0, 1, 2, 3}, has1},
+      {&onion_tuning::fd_rows_b1, {64, 128, 256, 512}, has1},
+      {&onion_tuning::fuse_min, {1, 256, 1024, 4096, 1u << 30}, true},
+      {&onion_tuning::use_graph, {0, 1}, true},
+  };
+  for (const Axis &ax : axes) {
+    if (!ax.active) continue;
+    for (uint32_t v : ax.values) {
+      if (best.*ax.field == v) continue;
+      onion_tuning t = best;
+      t.*ax.field = v;
+      double sc;
+      if ((rc = score(t, &sc))) {
+        s->tune = best;
+        ++s->tune_epoch;
+        return rc;
+      }
+      if (sc < 0.99 * best_sc) {  // 1% hysteresis against timing noise
+        best = t;
+        best_sc = sc;
+      }
+    }
+  }
+  s->tune = best;
+  ++s->tune_epoch;
+  if (out) *out = best;
   return ONION_OK;
 }
 
@@ -874,12 +1006,25 @@ int onion_gpu_set_tuning(onion_server *s, const onion_tuning *t) {
   if (t->fd_tile_rows != 2 && t->fd_tile_rows != 4) return fail(ONION_EINVAL, "fd_tile_rows must be 2 or 4");
   if (t->fd_b1_variant > 3) return fail(ONION_EINVAL, "fd_b1_variant must be 0..3");
   if (t->use_graph > 1) return fail(ONION_EINVAL, "use_graph must be 0 or 1");
-  if (t->fd_mfma_variant > 1) return fail(ONION_EINVAL, "fd_mfma_variant must be 0 or 1");
+  if (t->fd_mfma_variant > 2) return fail(ONION_EINVAL, "fd_mfma_variant must be 0, 1 or 2");
   Exclusive x(s->gate);
   s->tune = *t;
   s->tune.fuse_min = std::max(1u, t->fuse_min);
   ++s->tune_epoch;
   return ONION_OK;
+}
+
+int onion_gpu_autotune(onion_server *s, const uint32_t *sizes, uint32_t n_sizes, uint32_t reps,
+                       onion_tuning *chosen) {
+  if (!s || !sizes) return fail(ONION_EINVAL, "null argument");
+  if (n_sizes < 1 || n_sizes > 16) return fail(ONION_EINVAL, "n_sizes must be in [1, 16]");
+  for (uint32_t i = 0; i < n_sizes; ++i)
+    if (sizes[i] < 1 || sizes[i] > kMaxBatch) return fail(ONION_EINVAL, "batch sizes must be in [1, 64]");
+  if (reps < 1 || reps > 20) return fail(ONION_EINVAL, "reps must be in [1, 20]");
+  Exclusive x(s->gate);
+  int rc;
+  if ((rc = check_db(s))) return rc;
+  return autotune_impl(s, sizes, n_sizes, reps, chosen);
 }
 
 int onion_gpu_last_stage_times(const onion_server *s, onion_stage_times *out) {

@@ -141,6 +141,13 @@ class GpuServer:
         t = _capi.OnionTuning(*[cur[f] for f, _ in _capi.OnionTuning._fields_])
         check(self._L.onion_gpu_set_tuning(self._h, C.byref(t)))
 
+    def autotune(self, batch_sizes=(1, 64), reps: int = 3) -> dict:
+        """Pick and apply the fastest launch shapes for these batch sizes."""
+        sizes = (C.c_uint32 * len(batch_sizes))(*batch_sizes)
+        t = _capi.OnionTuning()
+        check(self._L.onion_gpu_autotune(self._h, sizes, len(batch_sizes), reps, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in _capi.OnionTuning._fields_}
+
     def stage_times(self) -> dict:
         t = OnionStageTimes()
         check(self._L.onion_gpu_last_stage_times(self._h, C.byref(t)))

@@ -225,8 +225,9 @@ def test_first_dim_batch_matches_oracle(cfg1, nq):
     try:
         srv.set_tuning(fd_mfma_min=0)
         assert np.array_equal(srv.tap_first_dim_batch(sel), got)
-        srv.set_tuning(fd_mfma_min=2, fd_mfma_variant=1 - saved["fd_mfma_variant"])
-        assert np.array_equal(srv.tap_first_dim_batch(sel), got)
+        for var in (0, 1, 2):
+            srv.set_tuning(fd_mfma_min=2, fd_mfma_variant=var)
+            assert np.array_equal(srv.tap_first_dim_batch(sel), got), f"variant {var}"
     finally:
         srv.set_tuning(**saved)
 
@@ -267,7 +268,8 @@ def test_answer_batch_matches_single_and_oracle(cfg1, nq):
                                     dict(fd_rows_b1=64), dict(fd_tile_rows=2),
                                     dict(fd_b1_variant=1), dict(fd_b1_variant=2),
                                     dict(fd_b1_variant=3), dict(fd_mfma_min=0),
-                                    dict(fd_mfma_min=6), dict(fd_mfma_variant=0)])
+                                    dict(fd_mfma_min=6), dict(fd_mfma_variant=0),
+                                    dict(fd_mfma_variant=1), dict(fd_mfma_variant=2)])
 def test_tuning_choices_are_bit_identical(cfg1, tuning):
     """Every launch-shape choice (fused vs split external products, first-dim
     tile shapes) gives the oracle's bits, for a single query and a batch."""
@@ -287,5 +289,22 @@ def test_tuning_choices_are_bit_identical(cfg1, tuning):
         got = srv.answer_batch([CID] * 5, np.stack([q] * 5))
         for i in range(5):
             assert np.array_equal(got[i], exp)
+    finally:
+        srv.set_tuning(**saved)
+
+
+def test_autotune_keeps_results_bit_identical(cfg1):
+    """The autotuner only changes launch shapes: after it, single and batched
+    answers still equal the oracle's."""
+    o, srv = cfg1["o"], cfg1["srv"]
+    saved = srv.get_tuning()
+    try:
+        chosen = srv.autotune([1, 5], reps=2)
+        assert chosen == srv.get_tuning()
+        q = o.query(CLIENT_SEED, 501, cfg1["sk"], 99)
+        exp, _ = o.answer(cfg1["db"], cfg1["gk"], cfg1["rkn"], q)
+        assert np.array_equal(srv.answer(CID, q), exp)
+        got = srv.answer_batch([CID] * 5, np.stack([q] * 5))
+        assert all(np.array_equal(g, exp) for g in got)
     finally:
         srv.set_tuning(**saved)
