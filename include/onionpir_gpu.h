@@ -5,8 +5,9 @@
  * function returns ONION_OK (0) or a negative ONION_E* code, and
  * onion_gpu_last_error() returns the message of the calling thread's last
  * failure.  Host buffers are borrowed for the duration of a call; the server
- * handle owns all device memory.  Calls on one handle are serialised by an
- * internal mutex.
+ * handle owns all device memory.  Batches on different lanes of one handle run
+ * concurrently; calls on one lane are serialised; database, key, tuning and
+ * lane changes wait for the batches in flight.
  *
  * Data formats (N = ring degree, one data limb):
  *   ciphertext      2*N uint64, c0 then c1, coefficient form, values < q
