@@ -17,8 +17,11 @@ single-query one against the HBM roofline (HIP events on the stream the
 kernels run on), two batches in flight on two lanes, the serving loop under Poisson
 arrivals (p50/p99 from submit to response), three databases of different
 shape resident together (128x256, 256x1024, 512x4096) answering bursts of
-7 / 33 / 100 queries, and the CPU oracle timed on this host on a bounded
-sample, whose responses must equal the GPU's bit for bit.
+7 / 33 / 100 queries, the same plus 512x8192 (100.5 GiB resident) with 1000
+clients' keys in one shared device key store under Poisson arrivals, a
+256x512 single-query shape, and the CPU oracle timed on this host on a
+bounded sample (one core, then one oracle server per core), whose responses
+must equal the GPU's bit for bit.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extras]
 """
@@ -74,12 +77,17 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-queries", type=int, default=2)
     ap.add_argument("--plain-bits", type=int, default=16)
+    ap.add_argument("--cpu-cores", type=int, default=16,
+                    help="threads of the multi-core CPU baseline (one oracle server each); 1 = skip")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the built-in launch shapes instead of tuning on the live database")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the two-lane, serving and multi-database measurements")
     ap.add_argument("--serve-rates", default="100,500,1000,2000", help="Poisson arrival rates (queries/s)")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--config5-clients", type=int, default=1000,
+                    help="clients of the four-database serving workload (0 = skip it)")
+    ap.add_argument("--config5-rates", default="50,200,500,1000", help="its Poisson rates (queries/s)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (profiler byte-counter scale)")
     args = ap.parse_args()
@@ -256,9 +264,14 @@ def main() -> int:
         srv_rates = [float(r) for r in args.serve_rates.split(",") if r]
         out["serving"] = serving(srv, queries, single_resp, srv_rates, args.serve_seconds)
         ok_all = ok_all and all(r["bit_exact"] for r in out["serving"]["rates"])
-        del srv, clients  # free the 8 GiB database before the multi-database case
+        del srv, clients  # free the 8 GiB database before the other workloads
+        out["config_256x512"] = single_shape(args, 256, 512)
+        ok_all = ok_all and out["config_256x512"]["bit_exact"]
         out["multi_db"] = multi_db(args)
         ok_all = ok_all and out["multi_db"]["bit_exact"]
+        if args.config5_clients:
+            out["config5"] = config5(args)
+            ok_all = ok_all and out["config5"]["bit_exact"]
 
     # ---- CPU baseline: the oracle on this host, same inputs, bounded sample -------------------
     if not args.no_cpu:
@@ -353,6 +366,128 @@ def serving(srv, queries, single_resp, rates, seconds):
                         "max_ms": st["max_ms"], "mean_batch": st["mean_batch"], "batches": st["batches"],
                         "bit_exact": bool(same and st["failed"] == 0)})
     return {"lanes": 2, "max_batch": 64, "batch_cap": cap, "rates": out}
+
+
+def single_shape(args, fst, other, reps=20):
+    """Single queries on another shape (256x512: 2 GiB database): latency,
+    device stage times and the first-dimension kernel's HBM rate."""
+    from onionpirv2_amd import GpuServer, dbgen
+    from onionpirv2_amd.client import GpuClient
+    s = GpuServer(fst, other, plain_bits=args.plain_bits)
+    s.db_generate(args.db_seed + 7, 0)
+    c = GpuClient(fst, other, client_id=1, seed=77, plain_bits=args.plain_bits)
+    s.set_galois_key(1, c.galois_keys())
+    s.set_rgsw_key(1, c.rgsw_key())
+    num_pt = s.info["num_pt"]
+    idx = [int(i) for i in np.random.default_rng(5).integers(0, num_pt, size=reps)]
+    qs = [c.query(i, nonce=500 + k) for k, i in enumerate(idx)]
+    s.answer(1, qs[0])
+    lat, st, ok = [], [], 0
+    for k in range(reps):
+        t = time.perf_counter()
+        r = s.answer(1, qs[k])
+        lat.append((time.perf_counter() - t) * 1e3)
+        st.append(s.stage_times())
+        ok += int(np.array_equal(c.decrypt(r), dbgen.plaintext(args.db_seed + 7, 0, idx[k], t=s.info["t"])))
+    db_bytes = num_pt * N * 8
+    core = float(np.mean([x["first_dim_core"] for x in st]))
+    alg = db_bytes + N * fst * 16 + other * 2 * N * 8
+    s.close()
+    log(f"[bench] {fst}x{other}: single query p50 {pct(lat, 50):.3f} ms, first dim {alg / core / 1e6:.0f} GB/s")
+    return {"shape": f"{fst}x{other}", "db_GiB": db_bytes / 2**30, "queries": reps,
+            "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+            "device_ms": float(np.mean([x["total"] for x in st])),
+            "stage_ms": {k: float(np.mean([x[k] for x in st])) for k in st[0]},
+            "first_dim_GBps": alg / (core * 1e-3) / 1e9, "first_dim_frac_of_hbm_peak": alg / (core * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+            "records_ok": ok, "bit_exact": ok == reps}
+
+
+def config5(args):
+    """Four databases resident together (128x256, 256x1024, 512x4096,
+    512x8192: 100.5 GiB NTT form) sharing one device key store that holds the
+    keys of `config5_clients` clients, served by the serving loop (one lane per
+    database, scratch admitted from a 120 GiB budget) under Poisson arrivals;
+    each query goes to a uniformly drawn database, client and record.  Every
+    response must decrypt to its record."""
+    from onionpirv2_amd import GpuServer, dbgen
+    from onionpirv2_amd.client import GpuClient
+    from onionpirv2_amd.serve import GpuServe
+    shapes = [(128, 256), (256, 1024), (512, 4096), (512, 8192)]
+    nc = args.config5_clients
+    t0 = time.time()
+    servers = []
+    for d, (fst, other) in enumerate(shapes):
+        s = GpuServer(fst, other, plain_bits=args.plain_bits)
+        s.db_generate(args.db_seed + 11 + d, 0)
+        if servers:
+            s.share_keys(servers[0])
+        servers.append(s)
+    resident = sum(s.info["num_pt"] * N * 8 for s in servers)
+    t_db = time.time() - t0
+    t0 = time.time()
+    tall = shapes[-1]  # the tallest expansion: its keys serve all four databases
+    for c in range(nc):
+        cl = GpuClient(tall[0], tall[1], client_id=c, seed=90000 + c, plain_bits=args.plain_bits)
+        servers[0].set_galois_key(c, cl.galois_keys())
+        servers[0].set_rgsw_key(c, cl.rgsw_key())
+    t_keys = time.time() - t0
+    key_bytes = nc * (servers[-1].info["galois_key_words"] + servers[-1].info["rgsw_key_words"]) * 8
+    log(f"[bench] config5: {resident / 2**30:.1f} GiB resident ({t_db:.1f} s), {nc} clients' keys "
+        f"({key_bytes / 2**30:.2f} GiB, one shared store) in {t_keys:.1f} s")
+    rng = np.random.default_rng(555)
+    qclients = {}
+
+    def client(d, c):
+        if (d, c) not in qclients:
+            qclients[(d, c)] = GpuClient(shapes[d][0], shapes[d][1], client_id=c, seed=90000 + c,
+                                         plain_bits=args.plain_bits)
+        return qclients[(d, c)]
+
+    rates = [float(r) for r in args.config5_rates.split(",") if r]
+    runs, ok_all = [], True
+    with GpuServe(max_batch=64, lanes_per_db=1, scratch_budget=120 << 30) as sv:
+        caps = [sv.add_db(s)[1] for s in servers]
+        for rate in rates:
+            n = max(40, int(rate * args.serve_seconds))
+            work = []
+            for k in range(n):
+                d = int(rng.integers(4))
+                c = int(rng.integers(nc))
+                i = int(rng.integers(servers[d].info["num_pt"]))
+                work.append((d, c, i, client(d, c).query(i, nonce=int(rate) * 100000 + k)))
+            gaps = rng.exponential(1.0 / rate, size=n)
+            sv.reset_stats()
+            tickets = []
+            t0 = time.perf_counter()
+            due = t0
+            for k, (d, c, i, q) in enumerate(work):
+                due += gaps[k]
+                while time.perf_counter() < due:
+                    pass
+                tickets.append(sv.submit(d, c, q))
+            good = 0
+            for (d, c, i, q), tk in zip(work, tickets):
+                r = sv.wait(tk, db_id=d)
+                good += int(np.array_equal(client(d, c).decrypt(r),
+                                           dbgen.plaintext(args.db_seed + 11 + d, 0, i, t=servers[d].info["t"])))
+            st = sv.stats()
+            per_db = []
+            for d in range(4):
+                sd = sv.stats(d)
+                per_db.append({"shape": f"{shapes[d][0]}x{shapes[d][1]}", "queries": sd["completed"],
+                               "p50_ms": sd["p50_ms"], "p99_ms": sd["p99_ms"], "mean_batch": sd["mean_batch"]})
+            ok_all = ok_all and good == n
+            log(f"[bench] config5 at {rate:.0f} q/s: {st['queries_per_s']:.1f} q/s done, p50 {st['p50_ms']:.1f} ms "
+                f"p99 {st['p99_ms']:.1f} ms, {good}/{n} records")
+            runs.append({"target_qps": rate, "offered_qps": n / (due - t0), "queries": n,
+                         "completed_qps": st["queries_per_s"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+                         "max_ms": st["max_ms"], "mean_batch": st["mean_batch"], "records_ok": good,
+                         "per_db": per_db})
+    for s in reversed(servers):
+        s.close()
+    return {"resident_GiB": resident / 2**30, "shapes": [f"{a}x{b}" for a, b in shapes], "clients": nc,
+            "key_store_GiB": key_bytes / 2**30, "batch_caps": caps, "scratch_budget_GiB": 120,
+            "runs": runs, "bit_exact": bool(ok_all)}
 
 
 def multi_db(args):
@@ -450,11 +585,38 @@ def cpu_baseline(args, clients, queries, idx, srv):
         match += int(np.array_equal(resp, srv.answer(i, queries[i])))
         log(f"[bench] oracle query {i}: {tm['total']:.2f} s "
             + " ".join(f"{k}={v:.2f}" for k, v in tm.items()))
-    return {"value": 1.0 / float(np.mean(times)), "unit": "queries/s", "cores": 1, "kind": "port",
-            "sample": f"{args.cpu_queries} full single queries of the same workload (same DB, clients, "
-                      f"queries) through the C oracle, 1 thread",
-            "seconds_per_query": float(np.mean(times)),
-            "responses_equal_gpu": f"{match}/{args.cpu_queries}"}
+    out = {"value": 1.0 / float(np.mean(times)), "unit": "queries/s", "cores": 1, "kind": "port",
+           "sample": f"{args.cpu_queries} full single queries of the same workload (same DB, clients, "
+                     f"queries) through the C oracle, 1 thread",
+           "seconds_per_query": float(np.mean(times)),
+           "responses_equal_gpu": f"{match}/{args.cpu_queries}"}
+    if args.cpu_cores > 1:
+        out["multicore"] = cpu_multicore(args, o, db, clients, queries, srv)
+    return out
+
+
+def cpu_multicore(args, o, db, clients, queries, srv):
+    """k independent single-threaded oracle servers (one per host core, the
+    database shared read-only), each answering one full query at once."""
+    import concurrent.futures as cf
+
+    from oracle_lib import Oracle
+
+    k = max(1, min(args.cpu_cores, os.cpu_count() or 1))
+    n = len(clients)
+    orcs = [Oracle(fst=args.fst, other=args.other, plain_bits=args.plain_bits) for _ in range(k)]
+    keys = [(c.galois_keys(), o.rgsw_key_ntt(c.rgsw_key())) for c in clients]
+    t0 = time.time()
+    with cf.ThreadPoolExecutor(k) as ex:
+        resps = list(ex.map(lambda i: orcs[i].answer(db, keys[i % n][0], keys[i % n][1], queries[i % n])[0],
+                            range(k)))
+    wall = time.time() - t0
+    match = sum(int(np.array_equal(r, srv.answer(i % n, queries[i % n]))) for i, r in enumerate(resps))
+    log(f"[bench] oracle on {k} cores: {k} queries in {wall:.2f} s")
+    return {"value": k / wall, "unit": "queries/s", "cores": k, "kind": "port",
+            "sample": f"{k} full single queries of the same workload, one per thread, each on its own "
+                      f"oracle server over the shared database",
+            "responses_equal_gpu": f"{match}/{k}"}
 
 
 if __name__ == "__main__":

@@ -133,7 +133,10 @@ int onion_gpu_db_upload_plain(onion_server *srv, uint64_t pt_lo, uint64_t n_pt,
  * (debug/parity tap). */
 int onion_gpu_db_read_ntt(onion_server *srv, uint64_t pt, uint64_t *out_n);
 
-/* PirServer::set_client_galois_key (src/server.cpp:480-484). */
+/* PirServer::set_client_galois_key (src/server.cpp:480-484).  n_words = levels
+ * x 4N with expan_height <= levels <= log2 N: a client may upload more levels
+ * than this database needs, so that one upload serves every database that
+ * shares the key store (onion_gpu_share_keys). */
 int onion_gpu_set_galois_key(onion_server *srv, uint64_t client_id, const uint64_t *keys,
                              size_t n_words);
 /* PirServer::set_client_gsw_key (src/server.cpp:486-500): coefficient-form
@@ -141,6 +144,12 @@ int onion_gpu_set_galois_key(onion_server *srv, uint64_t client_id, const uint64
 int onion_gpu_set_rgsw_key(onion_server *srv, uint64_t client_id, const uint64_t *rows,
                            size_t n_words);
 int onion_gpu_drop_client(onion_server *srv, uint64_t client_id);
+/* Make dst use src's device key store (no reference counterpart: the reference
+ * keeps keys per PirServer).  Both servers then see every key uploaded through
+ * either; a key upload waits for the batches in flight on all of them.  dst
+ * must hold no keys yet and use the same moduli and key gadget.  The store
+ * lives until its last server is destroyed. */
+int onion_gpu_share_keys(onion_server *dst, onion_server *src);
 
 /* PirServer::make_query (src/server.cpp:520-579): one query ciphertext in,
  * the mod-switched response ciphertext out (2*N uint64). */

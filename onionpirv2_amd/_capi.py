@@ -72,6 +72,7 @@ _SIGS = {
     "onion_gpu_set_galois_key": (C.c_int, [C.c_void_p, C.c_uint64, U64P, C.c_size_t]),
     "onion_gpu_set_rgsw_key": (C.c_int, [C.c_void_p, C.c_uint64, U64P, C.c_size_t]),
     "onion_gpu_drop_client": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "onion_gpu_share_keys": (C.c_int, [C.c_void_p, C.c_void_p]),
     "onion_gpu_answer": (C.c_int, [C.c_void_p, C.c_uint64, U64P, U64P]),
     "onion_gpu_answer_batch": (C.c_int, [C.c_void_p, C.c_uint32, U64P, U64P, U64P]),
     "onion_gpu_pack_response": (C.c_int, [C.c_void_p, U64P, C.c_char_p, C.c_size_t,

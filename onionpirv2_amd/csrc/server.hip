@@ -54,8 +54,9 @@ int fail(int code, const std::string &msg) {
   } while (0)
 
 struct ClientKeys {
-  uint64_t *gk = nullptr;  // h x 4N, NTT form
+  uint64_t *gk = nullptr;  // levels x 4N, NTT form
   uint64_t *rk = nullptr;  // 2 l_key x 2N, NTT form
+  uint32_t gk_levels = 0;
 };
 
 constexpr uint32_t kMaxBatch = 64;  // queries per pipeline pass
@@ -94,7 +95,30 @@ class Gate {
   }
 };
 
+void dfree_any(void *p) {
+  if (p) (void)hipFree(p);
+}
+
 }  // namespace
+
+// Device key store: the keys of every client, shared by all servers (databases)
+// that use it.  A client's Galois key of level a belongs to Galois element
+// 1 + (N >> a) whatever the database shape, so one upload with enough levels
+// serves databases of every expansion height up to that many levels.  Batches
+// hold the store's gate shared while their kernels may read keys; uploads and
+// drops hold it exclusively.
+struct KeyStore {
+  std::map<uint64_t, ClientKeys> clients;
+  Gate gate;
+  uint64_t q = 0, p = 0;
+  uint32_t l_key = 0;
+  ~KeyStore() {
+    for (auto &kv : clients) {
+      dfree_any(kv.second.gk);
+      dfree_any(kv.second.rk);
+    }
+  }
+};
 
 // One pipeline: stream, events, per-query scratch for `cap` queries, the
 // device arrays of per-query key pointers, and the graphs captured for it.
@@ -129,8 +153,8 @@ struct onion_server {
   uint64_t *db = nullptr;
   std::vector<uint8_t> pt_written;
   uint64_t n_written = 0;
-  // keys
-  std::map<uint64_t, ClientKeys> clients;
+  // client keys (possibly shared with other servers)
+  std::shared_ptr<KeyStore> keys;
   // per-query scratch strides in words
   uint64_t tree_words = 0, rgsw_words = 0, fd_words = 0, digit_words = 0;
   std::vector<std::unique_ptr<Lane>> lanes;  // lane 0 also serves taps and ingest
@@ -239,10 +263,15 @@ uint64_t scratch_bytes_per_query(const onion_server *s) {
   return words * 8 + 2 * sizeof(void *) + (qs + kMaxBatch - 1) / kMaxBatch;
 }
 
+// The caller holds the key store's gate (shared or exclusive).
 int lookup(const onion_server *s, uint64_t id, ClientKeys &out, bool need_rk) {
-  auto it = s->clients.find(id);
-  if (it == s->clients.end() || !it->second.gk)
+  auto it = s->keys->clients.find(id);
+  if (it == s->keys->clients.end() || !it->second.gk)
     return fail(ONION_ENOKEY, "no Galois keys for client " + std::to_string(id));
+  if (it->second.gk_levels < s->I.expan_height)
+    return fail(ONION_ENOKEY, "Galois keys of client " + std::to_string(id) + " cover " +
+                                  std::to_string(it->second.gk_levels) + " levels; this database needs " +
+                                  std::to_string(s->I.expan_height));
   if (need_rk && !it->second.rk)
     return fail(ONION_ENOKEY, "no RGSW key for client " + std::to_string(id));
   out = it->second;
@@ -779,6 +808,10 @@ int onion_gpu_create(const onion_params *params, onion_server **out) {
     return fail(rc, msg);
   }
   s->pt_written.assign(I.num_pt, 0);
+  s->keys = std::make_shared<KeyStore>();
+  s->keys->q = I.q;
+  s->keys->p = I.p;
+  s->keys->l_key = P.gsw_l_key;
   *out = s;
   return ONION_OK;
 }
@@ -790,10 +823,7 @@ int onion_gpu_destroy(onion_server *s) {
     for (auto &L : s->lanes)
       if (L) destroy_lane(L.get());
     s->lanes.clear();
-    for (auto &kv : s->clients) {
-      dfree(kv.second.gk);
-      dfree(kv.second.rk);
-    }
+    s->keys.reset();  // the store goes with its last server
     void *bufs[] = {s->tabs, s->db, s->io, s->pack};
     for (void *b : bufs) dfree(b);
   }
@@ -896,19 +926,29 @@ int onion_gpu_db_read_ntt(onion_server *s, uint64_t pt, uint64_t *out) {
 
 int onion_gpu_set_galois_key(onion_server *s, uint64_t id, const uint64_t *keys, size_t n) {
   if (!s || !keys) return fail(ONION_EINVAL, "null argument");
-  if (n != s->I.galois_key_words)
-    return fail(ONION_EINVAL, "Galois key size " + std::to_string(n) + " != " +
-                                  std::to_string(s->I.galois_key_words));
-  Exclusive x(s->gate);
+  const size_t level_words = 4 * (size_t)kN, max_levels = 11;  // log2 N
+  if (n % level_words || n / level_words < s->I.expan_height || n / level_words > max_levels)
+    return fail(ONION_EINVAL, "Galois key size " + std::to_string(n) + " is not " +
+                                  std::to_string(level_words) + " x levels with " +
+                                  std::to_string(s->I.expan_height) + " <= levels <= 11");
+  Shared g(s->gate);
+  std::shared_ptr<KeyStore> ks = s->keys;
+  Exclusive kx(ks->gate);  // no batch of any server sharing the store is in flight
   hipStream_t st = s->lanes[0]->st;
-  ClientKeys &ck = s->clients[id];
+  ClientKeys &ck = ks->clients[id];
+  if (ck.gk && ck.gk_levels * level_words < n) {
+    dfree(ck.gk);
+    ck.gk = nullptr;
+    ck.gk_levels = 0;
+  }
   if (!ck.gk) {
     int rc = dalloc(&ck.gk, n);
     if (rc) {
-      if (!ck.rk) s->clients.erase(id);
+      if (!ck.rk) ks->clients.erase(id);
       return rc;
     }
   }
+  ck.gk_levels = (uint32_t)(n / level_words);
   HIPCHK(hipMemcpyAsync(ck.gk, keys, n * 8, hipMemcpyHostToDevice, st));
   HIPCHK(hipStreamSynchronize(st));
   return ONION_OK;
@@ -919,13 +959,15 @@ int onion_gpu_set_rgsw_key(onion_server *s, uint64_t id, const uint64_t *rows, s
   if (n != s->I.rgsw_key_words)
     return fail(ONION_EINVAL, "RGSW key size " + std::to_string(n) + " != " +
                                   std::to_string(s->I.rgsw_key_words));
-  Exclusive x(s->gate);
+  Shared g(s->gate);
+  std::shared_ptr<KeyStore> ks = s->keys;
+  Exclusive kx(ks->gate);
   hipStream_t st = s->lanes[0]->st;
-  ClientKeys &ck = s->clients[id];
+  ClientKeys &ck = ks->clients[id];
   if (!ck.rk) {
     int rc = dalloc(&ck.rk, n);
     if (rc) {
-      if (!ck.gk) s->clients.erase(id);
+      if (!ck.gk) ks->clients.erase(id);
       return rc;
     }
   }
@@ -939,18 +981,41 @@ int onion_gpu_set_rgsw_key(onion_server *s, uint64_t id, const uint64_t *rows, s
 
 int onion_gpu_drop_client(onion_server *s, uint64_t id) {
   if (!s) return fail(ONION_EINVAL, "null server");
-  Exclusive x(s->gate);
-  auto it = s->clients.find(id);
-  if (it == s->clients.end()) return fail(ONION_ENOKEY, "unknown client " + std::to_string(id));
+  Shared g(s->gate);
+  std::shared_ptr<KeyStore> ks = s->keys;
+  Exclusive kx(ks->gate);
+  auto it = ks->clients.find(id);
+  if (it == ks->clients.end()) return fail(ONION_ENOKEY, "unknown client " + std::to_string(id));
   dfree(it->second.gk);
   dfree(it->second.rk);
-  s->clients.erase(it);
+  ks->clients.erase(it);
+  return ONION_OK;
+}
+
+int onion_gpu_share_keys(onion_server *dst, onion_server *src) {
+  if (!dst || !src) return fail(ONION_EINVAL, "null argument");
+  if (dst == src) return ONION_OK;
+  static std::mutex share_mu;  // one sharing change at a time: no lock-order cycle between servers
+  std::lock_guard<std::mutex> sm(share_mu);
+  Exclusive x(dst->gate);
+  Shared g(src->gate);
+  if (dst->device != src->device) return fail(ONION_EINVAL, "servers on different devices");
+  if (dst->I.q != src->I.q || dst->I.p != src->I.p || dst->P.gsw_l_key != src->P.gsw_l_key)
+    return fail(ONION_EINVAL, "servers differ in moduli or key gadget");
+  if (dst->keys == src->keys) return ONION_OK;
+  {
+    Shared kg(dst->keys->gate);
+    if (!dst->keys->clients.empty())
+      return fail(ONION_EINVAL, "this database already holds client keys; drop them first");
+  }
+  dst->keys = src->keys;
   return ONION_OK;
 }
 
 int onion_gpu_answer(onion_server *s, uint64_t id, const uint64_t *query, uint64_t *resp) {
   if (!s || !query || !resp) return fail(ONION_EINVAL, "null argument");
   Shared g(s->gate);
+  Shared kg(s->keys->gate);
   Lane *L = s->lanes[0].get();
   std::lock_guard<std::mutex> lk(L->mu);
   return answer_on_lane(s, L, 1, &id, query, resp, nullptr);
@@ -960,6 +1025,7 @@ int onion_gpu_answer_batch(onion_server *s, uint32_t n, const uint64_t *ids, con
                            uint64_t *resps) {
   if (!s || (n && (!ids || !queries || !resps))) return fail(ONION_EINVAL, "null argument");
   Shared g(s->gate);
+  Shared kg(s->keys->gate);
   Lane *L = s->lanes[0].get();
   std::lock_guard<std::mutex> lk(L->mu);
   return answer_on_lane(s, L, n, ids, queries, resps, nullptr);
@@ -971,6 +1037,7 @@ int onion_gpu_answer_batch_lane(onion_server *s, uint32_t lane, uint32_t n, cons
   Shared g(s->gate);
   if (lane >= s->lanes.size())
     return fail(ONION_EINVAL, "lane " + std::to_string(lane) + " of " + std::to_string(s->lanes.size()));
+  Shared kg(s->keys->gate);
   Lane *L = s->lanes[lane].get();
   std::lock_guard<std::mutex> lk(L->mu);
   return answer_on_lane(s, L, n, ids, queries, resps, times);
@@ -1039,6 +1106,7 @@ int onion_gpu_last_stage_times(const onion_server *s, onion_stage_times *out) {
 int onion_gpu_tap_expand(onion_server *s, uint64_t id, const uint64_t *query, uint64_t *leaves) {
   if (!s || !query || !leaves) return fail(ONION_EINVAL, "null argument");
   Shared g(s->gate);
+  Shared kg(s->keys->gate);
   Lane *L = s->lanes[0].get();
   std::lock_guard<std::mutex> lk(L->mu);
   int rc;
@@ -1055,6 +1123,7 @@ int onion_gpu_tap_expand(onion_server *s, uint64_t id, const uint64_t *query, ui
 int onion_gpu_tap_convert(onion_server *s, uint64_t id, const uint64_t *leaves, uint64_t *rgsw) {
   if (!s || !leaves || !rgsw) return fail(ONION_EINVAL, "null argument");
   Shared g(s->gate);
+  Shared kg(s->keys->gate);
   Lane *L = s->lanes[0].get();
   std::lock_guard<std::mutex> lk(L->mu);
   int rc;
@@ -1143,6 +1212,7 @@ int onion_gpu_tap_galois(onion_server *s, uint64_t id, uint32_t level, const uin
   if (!s || !ct || !out) return fail(ONION_EINVAL, "null argument");
   if (level >= s->I.expan_height) return fail(ONION_EINVAL, "level >= expansion height");
   Shared g(s->gate);
+  Shared kg(s->keys->gate);
   Lane *L = s->lanes[0].get();
   std::lock_guard<std::mutex> lk(L->mu);
   int rc;

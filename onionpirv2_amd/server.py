@@ -90,6 +90,11 @@ class GpuServer:
     def drop_client(self, client_id: int) -> None:
         check(self._L.onion_gpu_drop_client(self._h, client_id))
 
+    def share_keys(self, other: "GpuServer") -> None:
+        """Use other's device key store (this server must hold no keys yet)."""
+        check(self._L.onion_gpu_share_keys(self._h, other._h))
+        self._key_owner = other  # keep the store's other user alive
+
     # ---- answers ------------------------------------------------------------------------
     def answer(self, client_id: int, query) -> np.ndarray:
         q = _u64(query).reshape(-1)

@@ -61,3 +61,16 @@ def test_truncated_response_raises(pair):
     _, c, _ = pair
     with pytest.raises(OnionValueError):
         c.unpack_response(b"\x00" * 100)
+
+
+def test_keys_of_a_smaller_shape_are_a_prefix():
+    """A client's Galois key of level a (element 1 + (N >> a)) and its RGSW key do
+    not depend on the database shape, so keys generated for the tallest
+    expansion serve every database that shares a device key store."""
+    from onionpirv2_amd.client import GpuClient
+    small = GpuClient(128, 256, client_id=3, seed=42, plain_bits=16)   # h = 8
+    big = GpuClient(512, 8192, client_id=3, seed=42, plain_bits=16)    # h = 10
+    gs, gb = small.galois_keys(), big.galois_keys()
+    assert gs.size == 8 * 4 * 2048 and gb.size == 10 * 4 * 2048
+    assert np.array_equal(gb[: gs.size], gs)
+    assert np.array_equal(small.rgsw_key(), big.rgsw_key())

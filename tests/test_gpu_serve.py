@@ -226,3 +226,93 @@ def test_serve_poll_and_timeout(dbs):
         with pytest.raises(ServeTimeout):
             sv.wait(t, timeout_ms=0)  # the lane lingers 200 ms for more arrivals
         assert np.array_equal(sv.wait(t), srv.answer(int(c[0]), q[0]))
+
+
+def test_shared_key_store_across_shapes(dbs):
+    """Two new databases (same shapes and seeds as the fixture's) share one
+    device key store, filled once with keys tall enough for both: answers
+    equal those of the fixture's servers, which hold their own keys."""
+    from onionpirv2_amd import GpuServer, OnionKeyError, OnionValueError
+    from onionpirv2_amd.client import GpuClient
+    (f0, o0), (f1, o1) = SHAPES
+    a = GpuServer(f0, o0, plain_bits=PLAIN_BITS)   # h = 7
+    b = GpuServer(f1, o1, plain_bits=PLAIN_BITS)   # h = 8
+    a.db_generate(100, 0)
+    b.db_generate(101, 0)
+    b.share_keys(a)
+    for cid in range(N_CLIENTS):
+        tall = GpuClient(f1, o1, client_id=cid, seed=500 + cid, plain_bits=PLAIN_BITS)
+        a.set_galois_key(cid, tall.galois_keys())   # 8 levels, uploaded through the h = 7 server
+        a.set_rgsw_key(cid, tall.rgsw_key())
+    for d, srv in enumerate((a, b)):
+        cids, idx, qs = make_queries(dbs, d, 5, seed=950 + d)
+        got = srv.answer_batch(cids, qs)
+        ref = dbs["servers"][d]
+        for i in range(5):
+            assert np.array_equal(got[i], ref.answer(int(cids[i]), qs[i])), f"db {d} query {i}"
+    # keys too short for the taller database
+    short = GpuClient(f0, o0, client_id=77, seed=777, plain_bits=PLAIN_BITS)
+    a.set_galois_key(77, short.galois_keys())
+    a.set_rgsw_key(77, short.rgsw_key())
+    q = GpuClient(f1, o1, client_id=77, seed=777, plain_bits=PLAIN_BITS).query(3, nonce=1)
+    with pytest.raises(OnionKeyError):
+        b.answer(77, q)
+    a.answer(77, short.query(3, nonce=1))  # fine on the shorter database
+    # a server with keys of its own cannot join a store
+    c = GpuServer(f0, o0, plain_bits=PLAIN_BITS)
+    c.set_galois_key(1, short.galois_keys())
+    with pytest.raises(OnionValueError):
+        c.share_keys(a)
+    c.close()
+    b.close()
+    a.close()
+
+
+def test_key_upload_while_batches_run_on_a_sharing_server(dbs):
+    """Uploads through one server wait for batches in flight on another server
+    of the same store; nothing deadlocks and every answer stays right."""
+    from onionpirv2_amd import GpuServer
+    from onionpirv2_amd.client import GpuClient
+    (f0, o0), (f1, o1) = SHAPES
+    a = GpuServer(f0, o0, plain_bits=PLAIN_BITS)
+    b = GpuServer(f1, o1, plain_bits=PLAIN_BITS)
+    a.db_generate(100, 0)
+    b.db_generate(101, 0)
+    b.share_keys(a)
+    tall = {cid: GpuClient(f1, o1, client_id=cid, seed=500 + cid, plain_bits=PLAIN_BITS) for cid in range(N_CLIENTS)}
+    for cid, c in tall.items():
+        a.set_galois_key(cid, c.galois_keys())
+        a.set_rgsw_key(cid, c.rgsw_key())
+    cids, idx, qs = make_queries(dbs, 1, 6, seed=990)
+    ref = b.answer_batch(cids, qs)
+    errors, results = [], []
+
+    def batches():
+        try:
+            for _ in range(6):
+                results.append(b.answer_batch(cids, qs))
+        except Exception as e:  # noqa: BLE001
+            errors.append(e)
+
+    def uploads():
+        try:
+            for k in range(6):
+                c = tall[k % N_CLIENTS]
+                a.set_galois_key(k % N_CLIENTS, c.galois_keys())
+                a.set_rgsw_key(k % N_CLIENTS, c.rgsw_key())
+                extra = GpuClient(f1, o1, client_id=200 + k, seed=2000 + k, plain_bits=PLAIN_BITS)
+                a.set_galois_key(200 + k, extra.galois_keys())
+                a.set_rgsw_key(200 + k, extra.rgsw_key())
+        except Exception as e:  # noqa: BLE001
+            errors.append(e)
+
+    th = [threading.Thread(target=batches), threading.Thread(target=uploads)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in th), "deadlock"
+    assert not errors, errors
+    assert len(results) == 6 and all(np.array_equal(r, ref) for r in results)
+    b.close()
+    a.close()
