@@ -93,9 +93,7 @@ typedef struct onion_tuning {
                              cores (exact signed-digit products); 0 = never  */
   uint32_t fd_mfma_variant; /* 0: operands loaded to registers one step
                                ahead; 1: direct-to-LDS loads three steps
-                               ahead, shared by the workgroup's waves; 2: as
-                               1 with each tile's digit products split over
-                               two waves (two waves per SIMD)               */
+                               ahead, shared by the workgroup's waves        */
 } onion_tuning;
 
 typedef struct onion_server onion_server;

@@ -397,172 +397,6 @@ __global__ __launch_bounds__(256) void k_first_dim_mfma_lds(const uint64_t *__re
   }
 }
 
-// ---- two waves per output tile ---------------------------------------------------------
-//
-// k_first_dim_mfma_lds keeps all 15 diagonal accumulators of a 32x32 tile in
-// one wave (240 AGPRs), which leaves one wave per SIMD, so a wave's operand
-// conversion, LDS reads, barrier and epilogue stall its own matrix-core work.
-// Here a tile belongs to a pair of waves that split its 64 digit products:
-// half X takes the pairs (i, k) with i + k < 7 and (0,7) (1,6) (2,5) (3,4)
-// (diagonals 0..7), half Y the pairs with i + k > 7 and (4,3) (5,2) (6,1)
-// (7,0) (diagonals 7..14); 8 accumulators each, so two waves fit per SIMD
-// and one wave's non-matrix work overlaps the other's products.  Each half
-// forms its share of C mod 2^128; Y hands its share to X through LDS, and X
-// adds, reduces mod q and stores.  Workgroup: 8 waves, 4 pairs.
-
-// digit pair i * 8 + k of each half, in an issue order that keeps products
-// into the same accumulator at least 4 apart
-__constant__ constexpr uint8_t kPairsX[32] = {6,  5,  4,  3,  13, 12, 11, 7,  20, 19, 2,  10, 27, 18, 26, 14,
-                                              34, 1,  9,  17, 25, 33, 41, 21, 0,  8,  16, 24, 32, 40, 48, 28};
-__constant__ constexpr uint8_t kPairsY[32] = {15, 23, 31, 35, 22, 30, 38, 39, 29, 37, 42, 45, 36, 44, 46, 47,
-                                              43, 49, 51, 52, 50, 53, 54, 55, 56, 57, 58, 59, 60, 61, 62, 63};
-
-template <int CB>
-struct MfmaPipe2 {
-  static constexpr int RB = 4 / CB;
-  static constexpr int PIECES = 8 * (RB + CB);
-  static constexpr int PER_WAVE = PIECES / 8;
-  static constexpr int STAGE_BYTES = PIECES * 1024;
-  static_assert(PIECES % 8 == 0, "pieces per wave");
-};
-
-template <int CB>
-__device__ __forceinline__ void issue_step2(uint8_t *lds, int stage, uint32_t s, const uint64_t *a_base,
-                                            uint32_t fst, uint32_t row_base, uint32_t other,
-                                            const uint8_t *b_base, uint32_t w, uint32_t l) {
-  using P = MfmaPipe2<CB>;
-  const uint32_t g = l >> 5, li = l & 31;
-#pragma unroll
-  for (int i = 0; i < P::PER_WAVE; ++i) {
-    const uint32_t piece = w + 8 * i;  // wave-uniform
-    uint8_t *dst = lds + stage * P::STAGE_BYTES + piece * 1024;
-    const uint32_t p = piece & 7, blk = piece >> 3;
-    if (blk < (uint32_t)P::RB) {
-      uint32_t row = row_base + blk * 32 + li;
-      row = row < other ? row : 0;  // masked rows read row 0; their outputs are not stored
-      const uint32_t j = 32 * s + (p < 4 ? 8 * g + 2 * p : 16 + 8 * g + 2 * (p - 4));
-      lds_dma16(a_base + (size_t)row * fst + j, dst);
-    } else {
-      const uint32_t cb = blk - P::RB, col = cb * 32 + li;
-      const uint32_t jb = 4 * s + g + (p < 4 ? 0 : 2);
-      lds_dma16(b_base + ((size_t)jb * (32 * CB) + col) * 64 + 16 * (p & 3), dst);
-    }
-  }
-}
-
-// lo:hi += sext(P) * 2^sh, mod 2^128 (sh < 64 or sh >= 64)
-__device__ __forceinline__ void add_shifted(uint64_t &lo, uint64_t &hi, int64_t P, int sh) {
-  if (sh == 0) {
-    add128(lo, hi, (uint64_t)P, (uint64_t)(P >> 63));
-  } else if (sh < 64) {
-    add128(lo, hi, (uint64_t)P << sh, (uint64_t)(P >> (64 - sh)));
-  } else {
-    hi += (uint64_t)P << (sh - 64);
-  }
-}
-
-// sum_{d < 8} acc[d][e] * 256^(d0 + d) mod 2^128
-__device__ __forceinline__ void partial128(const v16i_t (&acc)[8], int e, int d0, uint64_t &lo, uint64_t &hi) {
-  int64_t P0 = 0, P1 = 0;  // |P| < 2^52
-#pragma unroll
-  for (int d = 0; d < 4; ++d) {
-    P0 += (int64_t)acc[d][e] * ((int64_t)1 << (8 * d));
-    P1 += (int64_t)acc[d + 4][e] * ((int64_t)1 << (8 * d));
-  }
-  lo = hi = 0;
-  add_shifted(lo, hi, P0, 8 * d0);
-  add_shifted(lo, hi, P1, 8 * d0 + 32);
-}
-
-template <bool HX>
-__device__ __forceinline__ void mfma_half(v16i_t (&acc)[8], const uint32_t (&pa0)[16], const uint32_t (&pa1)[16],
-                                          const uint32_t (&pb0)[16], const uint32_t (&pb1)[16]) {
-#pragma unroll
-  for (int n = 0; n < 32; ++n) {
-    const int ik = HX ? kPairsX[n] : kPairsY[n];
-    const int i = ik >> 3, k = ik & 7, d = HX ? i + k : i + k - 7;
-    acc[d] = __builtin_amdgcn_mfma_i32_32x32x32_i8(operand(pa0, pa1, i), operand(pb0, pb1, k), acc[d], 0, 0, 0);
-  }
-}
-
-template <int CB>
-__global__ __launch_bounds__(512) void k_first_dim_mfma_x2(const uint64_t *__restrict__ db,
-                                                           const uint8_t *__restrict__ Bp,
-                                                           uint64_t *__restrict__ out2, uint32_t fst,
-                                                           uint32_t other, uint64_t num_pt, uint32_t ncols,
-                                                           uint32_t n_rt, Mod mq) {
-  using P = MfmaPipe2<CB>;
-  constexpr int RB = P::RB, NCP = 32 * CB;
-  __shared__ __attribute__((aligned(16))) uint8_t lds[kMfmaStages * P::STAGE_BYTES];
-  const uint32_t L = blockIdx.x, xcd = L & 7, seq = L >> 3;
-  const uint32_t slot = (seq / n_rt) * 8 + xcd, rt = seq % n_rt;
-  const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 5, li = l & 31;
-  const uint32_t pair = w >> 1, half = w & 1;
-  const uint32_t rb = pair / CB, cb = pair % CB;
-  const uint32_t row_base = rt * (32 * RB);
-  const uint64_t *a_base = db + (size_t)slot * num_pt;
-  const uint8_t *b_base = Bp + (size_t)slot * (fst / 8) * NCP * 64;
-
-  v16i_t acc[8];
-#pragma unroll
-  for (int d = 0; d < 8; ++d) acc[d] = (v16i_t){0};
-
-  const uint32_t lds_off = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) uint8_t *)lds + l * 16;
-  const uint32_t steps = fst / 32;
-  issue_step2<CB>(lds, 0, 0, a_base, fst, row_base, other, b_base, w, l);
-  if (steps > 1) issue_step2<CB>(lds, 1, 1, a_base, fst, row_base, other, b_base, w, l);
-  for (uint32_t s = 0; s < steps; ++s) {
-    if (s + 1 < steps)
-      wait_vmcnt<P::PER_WAVE>();
-    else
-      wait_vmcnt<0>();
-    asm volatile("s_barrier" ::: "memory");
-    if (s + 2 < steps)
-      issue_step2<CB>(lds, (s + 2) % kMfmaStages, s + 2, a_base, fst, row_base, other, b_base, w, l);
-    const uint32_t st = lds_off + (s % kMfmaStages) * P::STAGE_BYTES;
-    uint4 ra[8], rbv[8];
-    lds_read_step(ra, rbv, st + rb * 8 * 1024, st + (RB + cb) * 8 * 1024);
-    uint32_t pa0[16], pa1[16], pb0[16], pb1[16];
-    a_planes(ra, pa0);
-    a_planes(ra + 4, pa1);
-    b_planes(rbv, pb0);
-    b_planes(rbv + 4, pb1);
-    if (half == 0)
-      mfma_half<true>(acc, pa0, pa1, pb0, pb1);
-    else
-      mfma_half<false>(acc, pa0, pa1, pb0, pb1);
-  }
-
-  // Y's share of each output through LDS (the staging area is free once every
-  // wave is past its last read), then X finishes
-  asm volatile("s_barrier" ::: "memory");
-  uint64_t *xch = reinterpret_cast<uint64_t *>(lds) + (size_t)pair * 16 * 64 * 2;
-  if (half == 1) {
-#pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      uint64_t lo, hi;
-      partial128(acc, e, 7, lo, hi);
-      xch[(e * 64 + l) * 2] = lo;
-      xch[(e * 64 + l) * 2 + 1] = hi;
-    }
-  }
-  __syncthreads();
-  if (half == 0) {
-    const uint32_t row0 = row_base + rb * 32, col = cb * 32 + li;
-    if (col < ncols) {
-      uint64_t *dst = out2 + (size_t)slot * other * ncols + col;
-#pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        uint64_t lo, hi;
-        partial128(acc, e, 0, lo, hi);
-        add128(lo, hi, xch[(e * 64 + l) * 2], xch[(e * 64 + l) * 2 + 1]);
-        const uint32_t m = row0 + 8 * (e / 4) + 4 * g + (e % 4);
-        if (m < other) dst[(size_t)m * ncols] = barrett128(lo, hi, mq);
-      }
-    }
-  }
-}
-
 // Inverse NTT of the first-dimension outputs gathered from out2[slot][r][c]
 // into fd (query b = c / 2, poly c % 2 of ciphertext r, coefficient form).
 // One workgroup per (r, c); when other % 8 == 0 the rows are dealt to XCDs

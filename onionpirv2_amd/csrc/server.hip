@@ -371,16 +371,8 @@ void run_first_dim_mfma(const onion_server *s, Lane *L, uint32_t nq, const uint6
   const uint32_t rb = 128 / ncp;  // row blocks of 32 per workgroup
   const uint32_t n_rt = (other + 32 * rb - 1) / (32 * rb);
   const dim3 grid(kN * n_rt);
-  const uint32_t var = s->tune.fd_mfma_variant;
-  const bool pipe = var == 1;
-  if (var == 2) {
-    if (ncp == 32)
-      k_first_dim_mfma_x2<1><<<grid, 512, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
-    else if (ncp == 64)
-      k_first_dim_mfma_x2<2><<<grid, 512, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
-    else
-      k_first_dim_mfma_x2<4><<<grid, 512, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
-  } else if (ncp == 32) {
+  const bool pipe = s->tune.fd_mfma_variant == 1;
+  if (ncp == 32) {
     if (pipe)
       k_first_dim_mfma_lds<1><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
     else
@@ -673,7 +665,7 @@ int autotune_impl(onion_server *s, const uint32_t *sizes, uint32_t n_sizes, uint
     bool active;
   };
   const std::vector<Axis> axes = {
-      {&onion_tuning::fd_mfma_variant, {0, 1, 2}, hasb},
+      {&onion_tuning::fd_mfma_variant, {0, 1}, hasb},
       {&onion_tuning::fd_mfma_min, {2, 4, 8, 16, 0}, has_small},
       {&onion_tuning::fd_tile_rows, {2, 4}, hasb},
       {&onion_tuning::fd_b1_variant, {0, 1, 2, 3}, has1},
@@ -1073,7 +1065,7 @@ int onion_gpu_set_tuning(onion_server *s, const onion_tuning *t) {
   if (t->fd_tile_rows != 2 && t->fd_tile_rows != 4) return fail(ONION_EINVAL, "fd_tile_rows must be 2 or 4");
   if (t->fd_b1_variant > 3) return fail(ONION_EINVAL, "fd_b1_variant must be 0..3");
   if (t->use_graph > 1) return fail(ONION_EINVAL, "use_graph must be 0 or 1");
-  if (t->fd_mfma_variant > 2) return fail(ONION_EINVAL, "fd_mfma_variant must be 0, 1 or 2");
+  if (t->fd_mfma_variant > 1) return fail(ONION_EINVAL, "fd_mfma_variant must be 0 or 1");
   Exclusive x(s->gate);
   s->tune = *t;
   s->tune.fuse_min = std::max(1u, t->fuse_min);
