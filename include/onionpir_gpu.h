@@ -76,9 +76,11 @@ typedef struct onion_stage_times {
  * bit-identical results; only speed differs. */
 typedef struct onion_tuning {
   uint32_t fuse_min;     /* external products (conversion, mux levels) run one
-                            workgroup per ciphertext when a stage has at least
-                            this many ciphertexts across the batch; below it
-                            every digit NTT gets its own workgroup            */
+                            workgroup per ciphertext, and an expansion level
+                            one launch of one workgroup per (node, component),
+                            when a stage has at least this many ciphertexts
+                            across the batch; below it every transform gets
+                            its own workgroup                               */
   uint32_t fd_rows_b1;   /* rows per workgroup of the single-query first-
                             dimension kernel (multiple of 32, 32..1024)       */
   uint32_t fd_tile_rows; /* thread rows of the batched first-dimension tile

@@ -5,13 +5,16 @@
 // At level a (k = 2^a) the active nodes are i = k .. k + n_active - 1.  Node i
 // computes c' = Subs(c_i, N/k + 1): the Galois permutation of c0 and c1, then
 // key switching of sigma(c1) with the special prime p, and the children
-// c_2i = c_i + c',  c_2i+1 = (c_i - c') * x^-k.  Two launches per level:
+// c_2i = c_i + c',  c_2i+1 = (c_i - c') * x^-k.  Narrow levels take two
+// launches:
 //   k_expand_ks      -- one workgroup per (node, component, limb q|p):
 //                       NTT(sigma(c1)) * key[component][limb], INTT -> X
 //   k_expand_combine -- one workgroup per node: the rounding division by p
 //                       (SEAL switch_key_inplace), sigma(c0), the children.
 // Splitting the key switch four ways keeps each workgroup at two transforms
-// deep, so a level's latency is two transforms, not six.
+// deep, so a level's latency is two transforms, not six.  Wide levels take
+// one launch, k_expand_fused (one workgroup per (node, component), four
+// transforms deep), which keeps X out of memory.
 // Batched: blockIdx.y selects the query; each query has its own tree, key and
 // X scratch.
 #pragma once
@@ -69,6 +72,34 @@ __global__ __launch_bounds__(kNttThreads) void k_expand_ks(const uint64_t *trees
   for (int j = 0; j < 8; ++j) dst[threadIdx.x + 256 * j] = ntt_inv_final(x[j], T);
 }
 
+// The rounding division of the key switch and the children for one
+// coefficient e of component comp (shared by k_expand_combine and
+// k_expand_fused).
+__device__ __forceinline__ void expand_child(uint64_t *left, uint64_t *right, const uint64_t *c0, int comp, int e,
+                                             uint64_t xq, uint64_t xp, uint32_t k, uint32_t ginv, uint64_t q,
+                                             uint64_t p, const KsConst &ks) {
+  // SEAL switch_key_inplace: floor((X + floor(p/2)) / p) mod q, with
+  // X = CRT(X_q, X_p):  (X_q - ((X_p + floor(p/2)) mod p) + floor(p/2)) * p^-1
+  uint64_t last = xp + ks.half_p;
+  if (last >= p) last -= p;
+  const uint64_t lq = last >= q ? last - q : last;  // p < 2q
+  const uint64_t diff = sub_mod(add_mod(xq, ks.half_p_mod_q, q), lq, q);
+  uint64_t ksv = mul_shoup_lazy(diff, ks.p_inv, ks.p_inv_s, q);
+  if (ksv >= q) ksv -= q;
+  // c'_0 = sigma(c0) + KS_0, c'_1 = KS_1 (c1 was cleared before the key switch)
+  const uint64_t cp = comp == 0 ? add_mod(galois_gather(c0, e, ginv, q), ksv, q) : ksv;
+  const uint64_t a = c0[(size_t)comp * kN + e];
+  left[(size_t)comp * kN + e] = add_mod(a, cp, q);
+  uint64_t d = sub_mod(a, cp, q);
+  // multiply by x^-k: slot e moves to e - k; wrapped slots are negated
+  int dst = e - (int)k;
+  if (dst < 0) {
+    dst += kN;
+    d = d ? q - d : 0;
+  }
+  right[(size_t)comp * kN + dst] = d;
+}
+
 __global__ __launch_bounds__(256) void k_expand_combine(uint64_t *trees, uint64_t tree_stride,
                                                         const uint64_t *xbuf, uint64_t x_stride,
                                                         uint32_t level, uint32_t ginv, uint64_t q,
@@ -83,30 +114,53 @@ __global__ __launch_bounds__(256) void k_expand_combine(uint64_t *trees, uint64_
   for (int comp = 0; comp < 2; ++comp) {
     for (int j = 0; j < 8; ++j) {
       const int e = threadIdx.x + 256 * j;
-      const uint64_t xq = X[(size_t)(2 * comp) * kN + e];
-      const uint64_t xp = X[(size_t)(2 * comp + 1) * kN + e];
-      // SEAL switch_key_inplace: floor((X + floor(p/2)) / p) mod q, with
-      // X = CRT(X_q, X_p):  (X_q - ((X_p + floor(p/2)) mod p) + floor(p/2)) * p^-1
-      uint64_t last = xp + ks.half_p;
-      if (last >= p) last -= p;
-      const uint64_t lq = last >= q ? last - q : last;  // p < 2q
-      const uint64_t diff = sub_mod(add_mod(xq, ks.half_p_mod_q, q), lq, q);
-      uint64_t ksv = mul_shoup_lazy(diff, ks.p_inv, ks.p_inv_s, q);
-      if (ksv >= q) ksv -= q;
-      // c'_0 = sigma(c0) + KS_0, c'_1 = KS_1 (c1 was cleared before the key switch)
-      const uint64_t cp = comp == 0 ? add_mod(galois_gather(c0, e, ginv, q), ksv, q) : ksv;
-      const uint64_t a = c0[(size_t)comp * kN + e];
-      left[(size_t)comp * kN + e] = add_mod(a, cp, q);
-      uint64_t d = sub_mod(a, cp, q);
-      // multiply by x^-k: slot e moves to e - k; wrapped slots are negated
-      int dst = e - (int)k;
-      if (dst < 0) {
-        dst += kN;
-        d = d ? q - d : 0;
-      }
-      right[(size_t)comp * kN + dst] = d;
+      expand_child(left, right, c0, comp, e, X[(size_t)(2 * comp) * kN + e], X[(size_t)(2 * comp + 1) * kN + e], k,
+                   ginv, q, p, ks);
     }
   }
+}
+
+// One level in one launch for wide levels: one workgroup per (node,
+// component) runs both limbs of its key switch (4 transforms) and writes that
+// component of both children, so X never leaves the workgroup.
+__global__ __launch_bounds__(kNttThreads) void k_expand_fused(uint64_t *trees, uint64_t tree_stride,
+                                                              const uint64_t *const *key_ptrs, uint32_t level,
+                                                              uint32_t ginv, NttTab Tq, NttTab Tp, Mod mq, Mod mp,
+                                                              KsConst ks) {
+  __shared__ uint64_t lds[kNttLds];
+  const uint32_t nl = blockIdx.x >> 1, comp = blockIdx.x & 1, k = 1u << level;
+  const uint64_t node = (uint64_t)k + nl;
+  uint64_t *tree = trees + (size_t)blockIdx.y * tree_stride;
+  const uint64_t *c0 = tree + node * 2 * kN;
+  const uint64_t *c1 = c0 + kN;
+  const uint64_t *key = key_ptrs[blockIdx.y] + (size_t)level * 4 * kN + (size_t)(2 * comp) * kN;
+  uint64_t xq[8], x[8], kv[8];
+  // limb q: NTT(sigma(c1)) * key, INTT
+#pragma unroll
+  for (int j = 0; j < 8; ++j) x[j] = galois_gather(c1, threadIdx.x + 256 * j, ginv, Tq.q);
+  ntt_fwd(x, lds, Tq);
+  load_blocked(kv, key);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) x[j] = mulmod_barrett(x[j], kv[j], mq);
+  ntt_inv(x, lds, Tq);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) xq[j] = ntt_inv_final(x[j], Tq);
+  // limb p (sigma(c1)'s residues < q < p serve unchanged)
+#pragma unroll
+  for (int j = 0; j < 8; ++j) x[j] = galois_gather(c1, threadIdx.x + 256 * j, ginv, Tq.q);
+  ntt_fwd(x, lds, Tp);
+  load_blocked(kv, key + kN);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) x[j] = mulmod_barrett(x[j], kv[j], mp);
+  ntt_inv(x, lds, Tp);
+  // children: every thread reads its own coefficients of this node and writes
+  // its own coefficients of the children (the tree slots differ), so no
+  // barrier is needed between reading c_i and writing c_2i, c_2i+1
+  uint64_t *left = tree + (2 * node) * 2 * kN, *right = tree + (2 * node + 1) * 2 * kN;
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    expand_child(left, right, c0, (int)comp, threadIdx.x + 256 * j, xq[j], ntt_inv_final(x[j], Tp), k, ginv,
+                 Tq.q, Tp.q, ks);
 }
 
 }  // namespace onion

@@ -302,7 +302,13 @@ int check_db(const onion_server *s) {
 // ---- pipeline stages (nq queries; query b's ciphertext at tree b, slot 1) ------------
 
 // Nodes 2^a .. 2^a + n_active - 1 of every query's tree; X scratch in digits.
+// Levels with at least tune.fuse_min nodes across the batch run fused.
 void expand_level(const onion_server *s, Lane *L, uint32_t nq, uint32_t a, uint32_t n_active) {
+  if ((uint64_t)n_active * nq >= s->tune.fuse_min) {
+    k_expand_fused<<<dim3(n_active * 2, nq), kNttThreads, 0, L->st>>>(L->trees, s->tree_words, L->gk_ptrs, a,
+                                                                      s->ginv[a], s->Tq, s->Tp, s->mq, s->mp, s->ks);
+    return;
+  }
   k_expand_ks<<<dim3(n_active * 4, nq), kNttThreads, 0, L->st>>>(
       L->trees, s->tree_words, L->gk_ptrs, a, s->ginv[a], L->digits, s->digit_words, s->Tq, s->Tp, s->mq, s->mp);
   k_expand_combine<<<dim3(n_active, nq), 256, 0, L->st>>>(L->trees, s->tree_words, L->digits, s->digit_words, a,

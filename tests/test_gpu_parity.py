@@ -283,6 +283,7 @@ def test_tuning_choices_are_bit_identical(cfg1, tuning):
         assert np.array_equal(srv.tap_mux(cts, g), o.mux_level(cts, g))
         q = o.query(CLIENT_SEED, 300, cfg1["sk"], 777)
         leaves = o.expand(q, cfg1["gk"])
+        assert np.array_equal(srv.tap_expand(CID, q), leaves)
         assert np.array_equal(srv.tap_convert(CID, leaves), o.convert(leaves, cfg1["rkn"]))
         exp, _ = o.answer(cfg1["db"], cfg1["gk"], cfg1["rkn"], q)
         assert np.array_equal(srv.answer(CID, q), exp)
