@@ -194,27 +194,40 @@ int onion_serve_add_db(onion_serve *sv, onion_server *srv, uint32_t *db_id, uint
   uint64_t bpq = 0;
   if (onion_gpu_scratch_bytes_per_query(srv, &bpq) != ONION_OK) return fail(ONION_EINVAL, onion_gpu_last_error());
   const uint32_t lanes = sv->cfg.lanes_per_db;
-  std::unique_lock<std::mutex> g(sv->m);
-  if (sv->stop) return fail(ONION_ESTATE, "serving loop stopped");
-  for (auto &D : sv->dbs)
-    if (D->srv == srv) return fail(ONION_EINVAL, "server already added");
   uint32_t cap = sv->cfg.max_batch;
-  if (sv->cfg.scratch_budget) {
-    const uint64_t left = sv->cfg.scratch_budget > sv->budget_used ? sv->cfg.scratch_budget - sv->budget_used : 0;
-    cap = (uint32_t)std::min<uint64_t>(cap, left / lanes / bpq);
-    if (cap == 0)
-      return fail(ONION_ENOMEM, "scratch budget: " + std::to_string(left) + " bytes left, " + std::to_string(lanes) +
-                                    " lanes x " + std::to_string(bpq) + " bytes per query needed");
+  uint64_t reserved = 0;
+  {
+    // admission: claim the budget under the lock, allocate outside it
+    std::lock_guard<std::mutex> g(sv->m);
+    if (sv->stop) return fail(ONION_ESTATE, "serving loop stopped");
+    for (auto &D : sv->dbs)
+      if (D->srv == srv) return fail(ONION_EINVAL, "server already added");
+    if (sv->cfg.scratch_budget) {
+      const uint64_t left = sv->cfg.scratch_budget > sv->budget_used ? sv->cfg.scratch_budget - sv->budget_used : 0;
+      cap = (uint32_t)std::min<uint64_t>(cap, left / lanes / bpq);
+      if (cap == 0)
+        return fail(ONION_ENOMEM, "scratch budget: " + std::to_string(left) + " bytes left, " +
+                                      std::to_string(lanes) + " lanes x " + std::to_string(bpq) +
+                                      " bytes per query needed");
+    }
+    reserved = (uint64_t)lanes * cap * bpq;
+    sv->budget_used += reserved;
   }
   int rc = onion_gpu_set_lanes(srv, lanes);
   for (uint32_t k = 0; rc == ONION_OK && k < lanes; ++k) rc = onion_gpu_reserve(srv, k, cap);
-  if (rc != ONION_OK) return fail(rc, onion_gpu_last_error());
+  std::lock_guard<std::mutex> g(sv->m);
+  bool dup = false;
+  for (auto &D : sv->dbs) dup |= D->srv == srv;
+  if (rc != ONION_OK || dup || sv->stop) {
+    sv->budget_used -= reserved;
+    if (rc != ONION_OK) return fail(rc, onion_gpu_last_error());
+    return dup ? fail(ONION_EINVAL, "server already added") : fail(ONION_ESTATE, "serving loop stopped");
+  }
   auto D = std::make_unique<Db>();
   D->srv = srv;
   D->cap = cap;
   D->lanes = lanes;
   D->ct_words = (size_t)2 * (info.galois_key_words / (4 * info.expan_height));
-  sv->budget_used += (uint64_t)lanes * cap * bpq;
   Db *Dp = D.get();
   sv->dbs.push_back(std::move(D));
   const uint32_t id = (uint32_t)sv->dbs.size() - 1;
