@@ -93,9 +93,17 @@ typedef struct onion_tuning {
   uint32_t fd_mfma_min;   /* batches of at least this many queries (and >= 2)
                              take the first dimension on the int8 matrix
                              cores (exact signed-digit products); 0 = never  */
-  uint32_t fd_mfma_variant; /* 0: operands loaded to registers one step
-                               ahead; 1: direct-to-LDS loads three steps
+  uint32_t fd_mfma_variant; /* 0: operands loaded to registers ahead of their
+                               step; 1: direct-to-LDS loads three steps
                                ahead, shared by the workgroup's waves        */
+  uint32_t db_planes;     /* 1: keep a second device copy of the database in
+                             signed-digit planes (as many bytes again as the
+                             database), rebuilt on every database write, so
+                             the matrix-core first dimension reads operands
+                             as stored; 0: free it and convert inside the
+                             kernel.  Only for first dimensions that are a
+                             multiple of 32.  The server starts with 1 and
+                             falls back to 0 when the copy does not fit.    */
 } onion_tuning;
 
 typedef struct onion_server onion_server;

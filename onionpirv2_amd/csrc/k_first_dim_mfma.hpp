@@ -20,15 +20,17 @@
 // Barrett reduction as the 64-bit path, so results are bit-identical.
 //
 // Operand layout.  MFMA lane l holds 16 bytes of row (l % 32) of the A tile
-// and 16 bytes of column (l % 32) of the B tile; for lane group g = l / 32 the
-// bytes are j = 8g .. 8g+7 and 16+8g .. 16+8g+7 of the 32-wide step (any
-// assignment works as long as A and B use the same one).  Digits therefore
-// travel in blocks of 8 consecutive j: a block holds the 8 digit planes of 8
-// values, plane i in bytes 8i .. 8i+7, byte t = value t (64 bytes).
-//   B planes  Bp[l][j/8][col][64 B], col = 2*query + poly, padded to NCP
-//             columns (NCP = 32, 64 or 128; written by k_sel_planes)
-//   A         read from the database words and turned into planes in
-//             registers
+// and 16 bytes of column (l % 32) of the B tile, both in the same k order;
+// since the sum runs over all 32 k, any assignment of j to (lane group, byte)
+// shared by A and B gives the same result.  Here byte b of lane group
+// g = l / 32 is j = 16 g + b of the 32-wide step, so an operand holds one
+// digit of 16 consecutive values of j.
+//   B planes  Bp[l][j/16][col][128 B], col = 2*query + poly, padded to NCP
+//             columns (NCP = 32, 64 or 128; written by k_sel_planes).  Byte
+//             16 k + b of a record is digit k of value j = 16 (j/16) + b, so
+//             the 16 bytes at 16 k are the operand for digit k as they stand.
+//   A         each lane reads the 16 consecutive database words of its row
+//             and turns them into digit planes in registers
 //   output    out2[l][r][c] (uint64, canonical mod q, c < 2*nq), turned into
 //             coefficient-form ciphertexts by k_ntt_inv_gather.
 #pragma once
@@ -72,8 +74,9 @@ __device__ __forceinline__ void planes8(const uint64_t (&w)[8], uint32_t (&out)[
 }
 
 // Selection vectors as digit planes: block (j-block jb of 8 leaves, poly c)
-// of query b NTTs its 8 polynomials and writes, for each slot, one 64-byte
-// plane block to Bp[slot][jb][2b + c].  grid = (fst/8 * 2, nq); fst % 8 == 0.
+// of query b NTTs its 8 polynomials and writes, for each slot, its half
+// (bytes 16 k + 8 (jb % 2) .. +7 for digit k) of the 128-byte record
+// Bp[slot][jb / 2][2b + c].  grid = (fst/8 * 2, nq); fst % 16 == 0.
 __global__ __launch_bounds__(kNttThreads) void k_sel_planes(const uint64_t *leaves, uint64_t leaf_stride,
                                                             uint8_t *Bp, uint32_t fst, uint32_t ncp, NttTab T) {
   __shared__ uint64_t lds[kNttLds];
@@ -87,7 +90,7 @@ __global__ __launch_bounds__(kNttThreads) void k_sel_planes(const uint64_t *leav
 #pragma unroll
     for (int r = 0; r < 8; ++r) xs[j][r] = signed_digits(ntt_fwd_final(x[r], T.q));
   }
-  const uint32_t col = 2 * b + c, nb = fst / 8;
+  const uint32_t col = 2 * b + c, nr = fst / 16;
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
     const uint32_t slot = 8 * threadIdx.x + r;
@@ -96,12 +99,14 @@ __global__ __launch_bounds__(kNttThreads) void k_sel_planes(const uint64_t *leav
     for (int j = 0; j < 8; ++j) w[j] = xs[j][r];
     uint32_t o[16];
     planes8(w, o);
-    uint4 *dst = reinterpret_cast<uint4 *>(Bp + (((size_t)slot * nb + jb) * ncp + col) * 64);
+    uint2 *dst = reinterpret_cast<uint2 *>(Bp + (((size_t)slot * nr + jb / 2) * ncp + col) * 128 + 8 * (jb % 2));
 #pragma unroll
-    for (int q = 0; q < 4; ++q) dst[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+    for (int k = 0; k < 8; ++k) dst[2 * k] = make_uint2(o[2 * k], o[2 * k + 1]);
   }
 }
 
+// A operand of digit i: plane i of values j = 16g .. 16g+7 (p0), then of
+// 16g+8 .. 16g+15 (p1).
 __device__ __forceinline__ v4i_t operand(const uint32_t (&p0)[16], const uint32_t (&p1)[16], int i) {
   v4i_t v;
   v[0] = (int)p0[2 * i];
@@ -111,29 +116,39 @@ __device__ __forceinline__ v4i_t operand(const uint32_t (&p0)[16], const uint32_
   return v;
 }
 
-// Raw operands of one 32-wide step of j for this lane: 2 x 8 database words
-// of its row, 2 plane blocks of its column.
+// B operand of digit k: the 16 bytes at 16 k of the lane's record.
+__device__ __forceinline__ v4i_t b_operand(const uint4 &v) {
+  v4i_t r;
+  r[0] = (int)v.x;
+  r[1] = (int)v.y;
+  r[2] = (int)v.z;
+  r[3] = (int)v.w;
+  return r;
+}
+
+// Raw operands of one 32-wide step of j for this lane: the 16 database words
+// j0+16g .. +15 of its row, the plane record j/16 = 2s+g of its column.  Rows
+// past the matrix read row 0 instead (arow points there); their outputs are
+// not stored, and no other output depends on them.
 struct StepRaw {
-  uint4 a[8];  // words j0+8g .. +7 (a[0..3]) and j0+16+8g .. +7 (a[4..7])
-  uint4 b[8];  // plane blocks of j-blocks 4s+g (b[0..3]) and 4s+2+g (b[4..7])
+  uint4 a[8];
+  uint4 b[8];
 };
 
-__device__ __forceinline__ void load_step(StepRaw &R, const uint64_t *arow, bool row_ok, const uint8_t *bcol,
-                                          uint32_t s, uint32_t g, uint32_t ncp) {
-  const uint4 *a0 = reinterpret_cast<const uint4 *>(arow + 32 * s + 8 * g);
-  const uint4 *a1 = reinterpret_cast<const uint4 *>(arow + 32 * s + 16 + 8 * g);
+__device__ __forceinline__ const uint4 *a_step(const uint64_t *arow, uint32_t s, uint32_t g) {
+  return reinterpret_cast<const uint4 *>(arow + 32 * s + 16 * g);
+}
+__device__ __forceinline__ const uint4 *b_step(const uint8_t *bcol, uint32_t s, uint32_t g, uint32_t ncp) {
+  return reinterpret_cast<const uint4 *>(bcol + (size_t)(2 * s + g) * ncp * 128);
+}
+
+__device__ __forceinline__ void load_step(StepRaw &R, const uint64_t *arow, const uint8_t *bcol, uint32_t s,
+                                          uint32_t g, uint32_t ncp) {
+  const uint4 *a = a_step(arow, s, g), *b = b_step(bcol, s, g, ncp);
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    R.a[q] = row_ok ? a0[q] : make_uint4(0, 0, 0, 0);
-    R.a[4 + q] = row_ok ? a1[q] : make_uint4(0, 0, 0, 0);
-  }
-  const uint4 *b0 = reinterpret_cast<const uint4 *>(bcol + (size_t)(4 * s + g) * ncp * 64);
-  const uint4 *b1 = reinterpret_cast<const uint4 *>(bcol + (size_t)(4 * s + 2 + g) * ncp * 64);
+  for (int q = 0; q < 8; ++q) R.a[q] = a[q];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    R.b[q] = b0[q];
-    R.b[4 + q] = b1[q];
-  }
+  for (int q = 0; q < 8; ++q) R.b[q] = b[q];
 }
 
 __device__ __forceinline__ void a_planes(const uint4 *a, uint32_t (&out)[16]) {
@@ -144,16 +159,6 @@ __device__ __forceinline__ void a_planes(const uint4 *a, uint32_t (&out)[16]) {
     w[2 * q + 1] = signed_digits(((uint64_t)a[q].w << 32) | a[q].z);
   }
   planes8(w, out);
-}
-
-__device__ __forceinline__ void b_planes(const uint4 *b, uint32_t (&out)[16]) {
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    out[4 * q] = b[q].x;
-    out[4 * q + 1] = b[q].y;
-    out[4 * q + 2] = b[q].z;
-    out[4 * q + 3] = b[q].w;
-  }
 }
 
 // C mod 2^128 from the 15 diagonal sums of one output, then mod q.
@@ -189,9 +194,8 @@ __global__ __launch_bounds__(256) void k_first_dim_mfma(const uint64_t *__restri
   const uint32_t rb = w / CB, cb = w % CB;
   const uint32_t row0 = rt * (32 * RB) + rb * 32;
   const uint32_t row = row0 + li, col = cb * 32 + li;
-  const bool row_ok = row < other;
-  const uint64_t *arow = db + (size_t)slot * num_pt + (size_t)(row_ok ? row : 0) * fst;
-  const uint8_t *bcol = Bp + (size_t)slot * (fst / 8) * NCP * 64 + (size_t)col * 64;
+  const uint64_t *arow = db + (size_t)slot * num_pt + (size_t)(row < other ? row : 0) * fst;
+  const uint8_t *bcol = Bp + (size_t)slot * (fst / 16) * NCP * 128 + (size_t)col * 128;
 
   v16i_t acc[15];
 #pragma unroll
@@ -201,24 +205,126 @@ __global__ __launch_bounds__(256) void k_first_dim_mfma(const uint64_t *__restri
   // next step's loads while the 64 products run
   const uint32_t steps = fst / 32;
   StepRaw raw;
-  load_step(raw, arow, row_ok, bcol, 0, g, NCP);
+  load_step(raw, arow, bcol, 0, g, NCP);
   for (uint32_t s = 0; s < steps; ++s) {
-    uint32_t pa0[16], pa1[16], pb0[16], pb1[16];
+    uint32_t pa0[16], pa1[16];
     a_planes(raw.a, pa0);
     a_planes(raw.a + 4, pa1);
-    b_planes(raw.b, pb0);
-    b_planes(raw.b + 4, pb1);
-    if (s + 1 < steps) load_step(raw, arow, row_ok, bcol, s + 1, g, NCP);
+    v4i_t B[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) B[k] = b_operand(raw.b[k]);
+    if (s + 1 < steps) load_step(raw, arow, bcol, s + 1, g, NCP);
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const v4i_t A = operand(pa0, pa1, i);
 #pragma unroll
       for (int k = 0; k < 8; ++k)
-        acc[i + k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, operand(pb0, pb1, k), acc[i + k], 0, 0, 0);
+        acc[i + k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, B[k], acc[i + k], 0, 0, 0);
     }
   }
 
   // output (m, n) of register e: n = lane % 32, m = 8 (e / 4) + 4 g + e % 4
+  if (col < ncols) {
+    uint64_t *dst = out2 + (size_t)slot * other * ncols + col;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const uint32_t m = row0 + 8 * (e / 4) + 4 * g + (e % 4);
+      const uint64_t v = diagonals_mod_q(acc, e, mq);
+      if (m < other) dst[(size_t)m * ncols] = v;
+    }
+  }
+}
+
+// ---- pre-converted database --------------------------------------------------------------
+//
+// The database as digit planes: dbp has the byte size and indexing of db, and
+// the 16 words j = 16 jp .. 16 jp + 15 of (slot, row) hold instead the record
+// of that pair of j-blocks in the B-plane format (byte 16 k + b = digit k of
+// value 16 jp + b).  The database is static between writes, so converting it
+// once per write takes the whole A conversion out of the first dimension: a
+// lane's 8 A operands of a step are its 128-byte record as loaded.
+
+// Records rec_lo .. rec_hi - 1 of every slot (a record is 16 words of one
+// row; num_pt % 16 == 0).  grid = (ceil((rec_hi - rec_lo) / 256), N).
+__global__ __launch_bounds__(256) void k_db_planes(const uint64_t *__restrict__ db, uint8_t *__restrict__ dbp,
+                                                   uint64_t num_pt, uint64_t rec_lo, uint64_t rec_hi) {
+  const uint64_t r = rec_lo + (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= rec_hi) return;
+  const size_t w0 = (size_t)blockIdx.y * num_pt + 16 * r;
+  const uint4 *src = reinterpret_cast<const uint4 *>(db + w0);
+  uint4 a[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) a[q] = src[q];
+  uint32_t p0[16], p1[16];
+  a_planes(a, p0);
+  a_planes(a + 4, p1);
+  uint4 *dst = reinterpret_cast<uint4 *>(dbp + 8 * w0);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) dst[k] = make_uint4(p0[2 * k], p0[2 * k + 1], p1[2 * k], p1[2 * k + 1]);
+}
+
+// The products of one step from a lane's A and B records.
+__device__ __forceinline__ void step_products(v16i_t (&acc)[15], const StepRaw &R) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const v4i_t A = b_operand(R.a[i]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      acc[i + k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, b_operand(R.b[k]), acc[i + k], 0, 0, 0);
+  }
+}
+
+__device__ __forceinline__ void load_records(StepRaw &R, const uint8_t *arec, const uint8_t *brec, uint32_t s,
+                                             uint32_t ncp) {
+  const uint4 *a = reinterpret_cast<const uint4 *>(arec + (size_t)s * 256);
+  const uint4 *b = reinterpret_cast<const uint4 *>(brec + (size_t)s * 2 * ncp * 128);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) R.a[q] = a[q];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) R.b[q] = b[q];
+}
+
+// First dimension for 2..64 queries from the pre-converted database: the tiles
+// of k_first_dim_mfma, with three register buffers so each step's records are
+// loaded two steps before their products.
+template <int CB>
+__global__ __launch_bounds__(256) void k_first_dim_mfma_pre(const uint8_t *__restrict__ dbp,
+                                                            const uint8_t *__restrict__ Bp,
+                                                            uint64_t *__restrict__ out2, uint32_t fst,
+                                                            uint32_t other, uint64_t num_pt, uint32_t ncols,
+                                                            uint32_t n_rt, Mod mq) {
+  constexpr int RB = 4 / CB, NCP = 32 * CB;
+  const uint32_t L = blockIdx.x, xcd = L & 7, seq = L >> 3;
+  const uint32_t slot = (seq / n_rt) * 8 + xcd, rt = seq % n_rt;
+  const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 5, li = l & 31;
+  const uint32_t rb = w / CB, cb = w % CB;
+  const uint32_t row0 = rt * (32 * RB) + rb * 32;
+  const uint32_t row = row0 + li, col = cb * 32 + li;
+  // record 2s + g of the row (rows past the matrix read row 0; not stored)
+  const uint8_t *arec = dbp + ((size_t)slot * num_pt + (size_t)(row < other ? row : 0) * fst) * 8 + (size_t)g * 128;
+  const uint8_t *brec = Bp + ((size_t)slot * (fst / 16) + g) * NCP * 128 + (size_t)col * 128;
+
+  v16i_t acc[15];
+#pragma unroll
+  for (int d = 0; d < 15; ++d) acc[d] = (v16i_t){0};
+
+  const uint32_t steps = fst / 32, body = steps - steps % 3;
+  StepRaw R[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) load_records(R[d], arec, brec, min((uint32_t)d, steps - 1), NCP);
+  // groups of three steps; a buffer is refilled (with step s + d + 3, or
+  // again the last step once past the end) as soon as its products issue
+  for (uint32_t s = 0; s < body; s += 3) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      step_products(acc, R[d]);
+      load_records(R[d], arec, brec, min(s + d + 3, steps - 1), NCP);
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 2; ++d)
+    if (body + d < steps) step_products(acc, R[d]);
+
   if (col < ncols) {
     uint64_t *dst = out2 + (size_t)slot * other * ncols + col;
 #pragma unroll
@@ -237,11 +343,11 @@ __global__ __launch_bounds__(256) void k_first_dim_mfma(const uint64_t *__restri
 // deep, so the matrix cores are not stalled by HBM latency and no VGPRs hold
 // loads in flight.  A step's operands are 16-byte "pieces": piece p of an A
 // row block holds, for reading lane l, the two database words
-//   p < 4:  j = 32 s + 8 g + 2 p, +1        p >= 4:  j = 32 s + 16 + 8 g + 2 (p-4), +1
-// of row (l % 32), and piece p of a B column block holds quarter (p % 4) of
-// the plane block of j-block 4 s + g (p < 4) or 4 s + 2 + g (p >= 4) of column
-// (l % 32); g = l / 32.  A piece is 1 KiB in LDS with lane l's 16 bytes at
-// 16 l, so both the DMA write and the reading ds_read_b128 are contiguous.
+// j = 32 s + 16 g + 2 p, +1 of row (l % 32), and piece p of a B column block
+// holds bytes 16 p .. 16 p + 15 (the operand for digit p) of plane record
+// 2 s + g of column (l % 32); g = l / 32.  A piece is 1 KiB in LDS with lane
+// l's 16 bytes at 16 l, so both the DMA write and the reading ds_read_b128
+// are contiguous.
 // The RB row blocks and CB column blocks of a step (8 (RB + CB) pieces) are
 // loaded by the four waves together and shared through LDS.
 
@@ -282,12 +388,10 @@ __device__ __forceinline__ void issue_step(uint8_t *lds, int stage, uint32_t s, 
     if (blk < (uint32_t)P::RB) {
       uint32_t row = row_base + blk * 32 + li;
       row = row < other ? row : 0;  // masked rows read row 0; their outputs are not stored
-      const uint32_t j = 32 * s + (p < 4 ? 8 * g + 2 * p : 16 + 8 * g + 2 * (p - 4));
-      lds_dma16(a_base + (size_t)row * fst + j, dst);
+      lds_dma16(a_base + (size_t)row * fst + 32 * s + 16 * g + 2 * p, dst);
     } else {
       const uint32_t cb = blk - P::RB, col = cb * 32 + li;
-      const uint32_t jb = 4 * s + g + (p < 4 ? 0 : 2);
-      lds_dma16(b_base + ((size_t)jb * (32 * CB) + col) * 64 + 16 * (p & 3), dst);
+      lds_dma16(b_base + ((size_t)(2 * s + g) * (32 * CB) + col) * 128 + 16 * p, dst);
     }
   }
 }
@@ -330,7 +434,9 @@ __device__ __forceinline__ void lds_read_step(uint4 (&ra)[8], uint4 (&rb)[8], ui
   rb[4] = to_uint4(y4); rb[5] = to_uint4(y5); rb[6] = to_uint4(y6); rb[7] = to_uint4(y7);
 }
 
-template <int CB>
+// PRE: db is the digit-plane copy (k_db_planes; same indexing), so a piece of
+// A is already the operand of one digit.
+template <int CB, bool PRE>
 __global__ __launch_bounds__(256) void k_first_dim_mfma_lds(const uint64_t *__restrict__ db,
                                                             const uint8_t *__restrict__ Bp,
                                                             uint64_t *__restrict__ out2, uint32_t fst,
@@ -345,7 +451,7 @@ __global__ __launch_bounds__(256) void k_first_dim_mfma_lds(const uint64_t *__re
   const uint32_t rb = w / CB, cb = w % CB;
   const uint32_t row_base = rt * (32 * RB);
   const uint64_t *a_base = db + (size_t)slot * num_pt;
-  const uint8_t *b_base = Bp + (size_t)slot * (fst / 8) * NCP * 64;
+  const uint8_t *b_base = Bp + (size_t)slot * (fst / 16) * NCP * 128;
 
   v16i_t acc[15];
 #pragma unroll
@@ -371,17 +477,17 @@ __global__ __launch_bounds__(256) void k_first_dim_mfma_lds(const uint64_t *__re
     const uint32_t st = lds_off + (s % kMfmaStages) * P::STAGE_BYTES;
     uint4 ra[8], rbv[8];
     lds_read_step(ra, rbv, st + rb * 8 * 1024, st + (RB + cb) * 8 * 1024);
-    uint32_t pa0[16], pa1[16], pb0[16], pb1[16];
-    a_planes(ra, pa0);
-    a_planes(ra + 4, pa1);
-    b_planes(rbv, pb0);
-    b_planes(rbv + 4, pb1);
+    uint32_t pa0[16], pa1[16];
+    if (!PRE) {
+      a_planes(ra, pa0);
+      a_planes(ra + 4, pa1);
+    }
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-      const v4i_t A = operand(pa0, pa1, i);
+      const v4i_t A = PRE ? b_operand(ra[i]) : operand(pa0, pa1, i);
 #pragma unroll
       for (int k = 0; k < 8; ++k)
-        acc[i + k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, operand(pb0, pb1, k), acc[i + k], 0, 0, 0);
+        acc[i + k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, b_operand(rbv[k]), acc[i + k], 0, 0, 0);
     }
   }
 

@@ -149,8 +149,9 @@ struct onion_server {
   std::vector<uint32_t> ginv;  // inverse of each level's Galois element mod 2N
   uint64_t ms_M = 0;
   uint32_t ms_f = 0;
-  // database
+  // database, and its signed-digit plane copy (tune.db_planes)
   uint64_t *db = nullptr;
+  uint8_t *dbp = nullptr;
   std::vector<uint8_t> pt_written;
   uint64_t n_written = 0;
   // client keys (possibly shared with other servers)
@@ -163,7 +164,7 @@ struct onion_server {
   uint8_t *pack = nullptr;
   // measured on MI355X at 512x1024: 8 nontemporal 16-byte loads in flight per
   // lane and 256-row blocks stream the database at 94% of a plain read kernel
-  onion_tuning tune{1024, 256, 4, 3, 0, 2, 0};
+  onion_tuning tune{1024, 256, 4, 3, 0, 2, 1, 1};
   uint64_t tune_epoch = 1;
   Gate gate;
   mutable std::mutex last_mu;
@@ -292,6 +293,36 @@ int bind_keys(const onion_server *s, Lane *L, uint32_t n, const uint64_t *ids, b
   return ONION_OK;
 }
 
+// Rebuild the plane records that cover plaintexts pt_lo .. pt_lo + n - 1 of
+// every slot from the database words (stream st; the caller holds the gate
+// exclusively).  A record is 16 plaintexts of one row; num_pt % 16 == 0
+// whenever the copy exists.
+void planes_update(const onion_server *s, hipStream_t st, uint64_t pt_lo, uint64_t n) {
+  if (!s->dbp || !n) return;
+  const uint64_t lo = pt_lo / 16, hi = (pt_lo + n + 15) / 16;
+  k_db_planes<<<dim3((uint32_t)((hi - lo + 255) / 256), kN), 256, 0, st>>>(s->db, s->dbp, s->I.num_pt, lo, hi);
+}
+
+// Allocate (and fill) or free the plane copy to match tune.db_planes.  The
+// caller holds the gate exclusively, so no batch is in flight.
+int sync_planes(onion_server *s) {
+  const bool want = s->tune.db_planes && s->P.fst_dim % 32 == 0;
+  if (want == (s->dbp != nullptr)) return ONION_OK;
+  hipStream_t st = s->lanes[0]->st;
+  if (!want) {
+    HIPCHK(hipStreamSynchronize(st));
+    dfree(s->dbp);
+    s->dbp = nullptr;
+    return ONION_OK;
+  }
+  int rc;
+  if ((rc = dalloc(&s->dbp, (size_t)s->I.num_pt * kN * 8))) return rc;
+  planes_update(s, st, 0, s->I.num_pt);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  return ONION_OK;
+}
+
 int check_db(const onion_server *s) {
   if (!s->db || s->n_written != s->I.num_pt)
     return fail(ONION_ESTATE, "database incomplete: " + std::to_string(s->n_written) + " of " +
@@ -368,6 +399,27 @@ bool use_mfma(const onion_server *s, const Lane *L, uint32_t nq) {
          (uint64_t)L->cap * s->digit_words >= (uint64_t)kN * s->P.other_dim * 2 * nq;
 }
 
+// The product kernel for CB column blocks: from the plane copy when the server
+// keeps one, else converting the database words in the kernel.
+template <int CB>
+void launch_mfma(const onion_server *s, Lane *L, const uint8_t *Bp, dim3 grid, uint32_t n_rt, uint32_t ncols) {
+  const uint32_t fst = s->P.fst_dim, other = s->P.other_dim;
+  const uint64_t np = s->I.num_pt;
+  const bool lds = s->tune.fd_mfma_variant == 1;
+  if (s->dbp) {
+    if (lds)
+      k_first_dim_mfma_lds<CB, true><<<grid, 256, 0, L->st>>>(reinterpret_cast<const uint64_t *>(s->dbp), Bp,
+                                                             L->digits, fst, other, np, ncols, n_rt, s->mq);
+    else
+      k_first_dim_mfma_pre<CB><<<grid, 256, 0, L->st>>>(s->dbp, Bp, L->digits, fst, other, np, ncols, n_rt, s->mq);
+  } else if (lds) {
+    k_first_dim_mfma_lds<CB, false><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, np, ncols, n_rt,
+                                                              s->mq);
+  } else {
+    k_first_dim_mfma<CB><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, np, ncols, n_rt, s->mq);
+  }
+}
+
 void run_first_dim_mfma(const onion_server *s, Lane *L, uint32_t nq, const uint64_t *sel, uint64_t sel_stride,
                         bool time_core) {
   const uint32_t fst = s->P.fst_dim, other = s->P.other_dim, ncp = cpad_for(nq), ncols = 2 * nq;
@@ -377,23 +429,12 @@ void run_first_dim_mfma(const onion_server *s, Lane *L, uint32_t nq, const uint6
   const uint32_t rb = 128 / ncp;  // row blocks of 32 per workgroup
   const uint32_t n_rt = (other + 32 * rb - 1) / (32 * rb);
   const dim3 grid(kN * n_rt);
-  const bool pipe = s->tune.fd_mfma_variant == 1;
-  if (ncp == 32) {
-    if (pipe)
-      k_first_dim_mfma_lds<1><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
-    else
-      k_first_dim_mfma<1><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
-  } else if (ncp == 64) {
-    if (pipe)
-      k_first_dim_mfma_lds<2><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
-    else
-      k_first_dim_mfma<2><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
-  } else {
-    if (pipe)
-      k_first_dim_mfma_lds<4><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
-    else
-      k_first_dim_mfma<4><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, s->I.num_pt, ncols, n_rt, s->mq);
-  }
+  if (ncp == 32)
+    launch_mfma<1>(s, L, Bp, grid, n_rt, ncols);
+  else if (ncp == 64)
+    launch_mfma<2>(s, L, Bp, grid, n_rt, ncols);
+  else
+    launch_mfma<4>(s, L, Bp, grid, n_rt, ncols);
   if (time_core) (void)hipEventRecord(L->ev[7], L->st);
   k_ntt_inv_gather<<<other * ncols, kNttThreads, 0, L->st>>>(L->digits, other, ncols, L->fd, s->fd_words, s->Tq);
 }
@@ -806,6 +847,14 @@ int onion_gpu_create(const onion_params *params, onion_server **out) {
     return fail(rc, msg);
   }
   s->pt_written.assign(I.num_pt, 0);
+  // the plane copy when it fits; without it the products convert in-kernel
+  if ((rc = sync_planes(s)) == ONION_ENOMEM) {
+    s->tune.db_planes = 0;
+  } else if (rc) {
+    std::string msg = g_err;
+    onion_gpu_destroy(s);
+    return fail(rc, msg);
+  }
   s->keys = std::make_shared<KeyStore>();
   s->keys->q = I.q;
   s->keys->p = I.p;
@@ -822,7 +871,7 @@ int onion_gpu_destroy(onion_server *s) {
       if (L) destroy_lane(L.get());
     s->lanes.clear();
     s->keys.reset();  // the store goes with its last server
-    void *bufs[] = {s->tabs, s->db, s->io, s->pack};
+    void *bufs[] = {s->tabs, s->db, s->dbp, s->io, s->pack};
     for (void *b : bufs) dfree(b);
   }
   delete s;
@@ -882,6 +931,7 @@ int onion_gpu_db_generate(onion_server *s, uint64_t seed, int mode) {
     k_db_generate<<<(uint32_t)n, kNttThreads, 0, st>>>(s->db, lo, s->I.num_pt, key, mode, s->P.plain_bits - 1,
                                                        s->lift, s->Tq);
   }
+  planes_update(s, st, 0, s->I.num_pt);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(st));
   std::fill(s->pt_written.begin(), s->pt_written.end(), 1);
@@ -902,6 +952,7 @@ int onion_gpu_db_upload_plain(onion_server *s, uint64_t pt_lo, uint64_t n_pt, co
     const uint64_t n = std::min(chunk, n_pt - off);
     HIPCHK(hipMemcpyAsync(s->io, coeffs + off * kN, n * kN * 8, hipMemcpyHostToDevice, st));
     k_db_plain<<<(uint32_t)n, kNttThreads, 0, st>>>(s->io, s->db, pt_lo + off, s->I.num_pt, s->lift, s->Tq);
+    planes_update(s, st, pt_lo + off, n);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     for (uint64_t i = 0; i < n; ++i) {
@@ -1072,10 +1123,17 @@ int onion_gpu_set_tuning(onion_server *s, const onion_tuning *t) {
   if (t->fd_b1_variant > 3) return fail(ONION_EINVAL, "fd_b1_variant must be 0..3");
   if (t->use_graph > 1) return fail(ONION_EINVAL, "use_graph must be 0 or 1");
   if (t->fd_mfma_variant > 1) return fail(ONION_EINVAL, "fd_mfma_variant must be 0 or 1");
+  if (t->db_planes > 1) return fail(ONION_EINVAL, "db_planes must be 0 or 1");
   Exclusive x(s->gate);
+  const uint32_t had = s->tune.db_planes;
   s->tune = *t;
   s->tune.fuse_min = std::max(1u, t->fuse_min);
-  ++s->tune_epoch;
+  ++s->tune_epoch;  // graphs captured before hold the old kernels and buffers
+  int rc;
+  if ((rc = sync_planes(s))) {
+    s->tune.db_planes = had && s->dbp ? 1 : 0;
+    return rc;
+  }
   return ONION_OK;
 }
 

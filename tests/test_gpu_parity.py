@@ -199,11 +199,37 @@ def test_first_dim_wraps_mod_2_128():
         rng = np.random.default_rng(8)
         sel2 = rng.integers(0, o.q, size=(o.fst, 2 * N), dtype=np.uint64)
         batch = np.stack([sel, sel2, sel])
-        for mfma_min in (2, 0):
-            srv.set_tuning(fd_mfma_min=mfma_min)
+        for mfma_min, planes in ((2, 1), (2, 0), (0, 0)):
+            srv.set_tuning(fd_mfma_min=mfma_min, db_planes=planes)
             gb = srv.tap_first_dim_batch(batch)
             assert np.array_equal(gb[0], exp) and np.array_equal(gb[2], exp), f"fd_mfma_min {mfma_min}"
             assert np.array_equal(gb[1], o.first_dim(db.reshape(-1), sel2))
+
+
+def test_partial_upload_updates_the_plane_copy():
+    """Plaintexts 21 .. 50 rewritten after db_generate: the records they share
+    with untouched plaintexts (16 plaintexts each) are rebuilt, so the
+    matrix-core product from the plane copy still matches the oracle."""
+    from onionpirv2_amd import GpuServer
+    o = Oracle(fst=64, other=4)
+    rng = np.random.default_rng(31)
+    lo, n = 21, 30
+    pts = rng.integers(0, o.t, size=(n, N), dtype=np.uint64)
+    db = o.db_build(3, 0).reshape(N, o.num_pt).copy()
+    for i in range(n):
+        p = pts[i]
+        db[:, lo + i] = o.ntt(np.where(p >= (o.t + 1) // 2, p + o.q - o.t, p))
+    sel = rng.integers(0, o.q, size=(3, o.fst, 2 * N), dtype=np.uint64)
+    with GpuServer(64, 4) as srv:
+        assert srv.get_tuning()["db_planes"] == 1
+        srv.db_generate(3, 0)
+        srv.db_upload_plain(lo, pts)
+        srv.set_tuning(fd_mfma_min=2)
+        got = srv.tap_first_dim_batch(sel)
+        for b in range(3):
+            assert np.array_equal(got[b], o.first_dim(db.reshape(-1), sel[b])), f"query {b}"
+        srv.set_tuning(db_planes=0)
+        assert np.array_equal(srv.tap_first_dim_batch(sel), got)
 
 
 @pytest.mark.parametrize("nq", [2, 3, 16, 17, 33, 64])
@@ -225,9 +251,9 @@ def test_first_dim_batch_matches_oracle(cfg1, nq):
     try:
         srv.set_tuning(fd_mfma_min=0)
         assert np.array_equal(srv.tap_first_dim_batch(sel), got)
-        for var in (0, 1):
-            srv.set_tuning(fd_mfma_min=2, fd_mfma_variant=var)
-            assert np.array_equal(srv.tap_first_dim_batch(sel), got), f"variant {var}"
+        for var, planes in ((0, 1), (1, 1), (0, 0), (1, 0)):
+            srv.set_tuning(fd_mfma_min=2, fd_mfma_variant=var, db_planes=planes)
+            assert np.array_equal(srv.tap_first_dim_batch(sel), got), f"variant {var} planes {planes}"
     finally:
         srv.set_tuning(**saved)
 
@@ -269,7 +295,8 @@ def test_answer_batch_matches_single_and_oracle(cfg1, nq):
                                     dict(fd_b1_variant=1), dict(fd_b1_variant=2),
                                     dict(fd_b1_variant=3), dict(fd_mfma_min=0),
                                     dict(fd_mfma_min=6), dict(fd_mfma_variant=0),
-                                    dict(fd_mfma_variant=1)])
+                                    dict(fd_mfma_variant=1), dict(db_planes=0),
+                                    dict(db_planes=0, fd_mfma_variant=1)])
 def test_tuning_choices_are_bit_identical(cfg1, tuning):
     """Every launch-shape choice (fused vs split external products, first-dim
     tile shapes) gives the oracle's bits, for a single query and a batch."""
