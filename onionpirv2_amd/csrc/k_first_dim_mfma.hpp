@@ -23,14 +23,22 @@
 // and 16 bytes of column (l % 32) of the B tile, both in the same k order;
 // since the sum runs over all 32 k, any assignment of j to (lane group, byte)
 // shared by A and B gives the same result.  Here byte b of lane group
-// g = l / 32 is j = 16 g + b of the 32-wide step, so an operand holds one
-// digit of 16 consecutive values of j.
-//   B planes  Bp[l][j/16][col][128 B], col = 2*query + poly, padded to NCP
-//             columns (NCP = 32, 64 or 128; written by k_sel_planes).  Byte
-//             16 k + b of a record is digit k of value j = 16 (j/16) + b, so
-//             the 16 bytes at 16 k are the operand for digit k as they stand.
-//   A         each lane reads the 16 consecutive database words of its row
-//             and turns them into digit planes in registers
+// g = l / 32 is j = 16 g + b of the 32-wide step, so an operand ("chunk")
+// holds one digit of 16 consecutive values of j of one row or column.
+//
+// Chunk order.  The plane buffers keep chunks in the order the matrix cores
+// consume them:
+//   chunk (tile, s, i, l)  at  16 * (((tile * steps + s) * 8 + i) * 64 + l)
+// for step s (steps = fst / 32), digit i and lane l = 32 g + (row or column
+// within the 32-block), where tile = slot * n_blocks + block.  A wave reads
+// digit i of a step as one contiguous 1 KiB load that lands in its operand
+// registers as it stands (8 cache lines per load rather than 64).
+//   B planes  written by k_sel_planes; 32-column blocks of col = 2*query +
+//             poly, padded to NCP columns (NCP = 32, 64 or 128)
+//   A planes  the database's digit-plane copy (k_db_planes), 32-row blocks,
+//             rows padded to a multiple of 32
+//   A words   without the copy each lane reads the 16 database words of its
+//             row and converts them in registers
 //   output    out2[l][r][c] (uint64, canonical mod q, c < 2*nq), turned into
 //             coefficient-form ciphertexts by k_ntt_inv_gather.
 #pragma once
@@ -43,8 +51,14 @@ typedef int v4i_t __attribute__((ext_vector_type(4)));
 typedef int v16i_t __attribute__((ext_vector_type(16)));
 
 constexpr uint64_t kDigitBias = 0x8080808080808080ULL;
+constexpr uint32_t kStepBytes = 8 * 1024;  // one tile's 8 chunks x 64 lanes
 
 __device__ __forceinline__ uint64_t signed_digits(uint64_t a) { return (a + kDigitBias) ^ kDigitBias; }
+
+// Byte offset of chunk (tile, s, i, l).
+__device__ __forceinline__ size_t chunk_offset(size_t tile, uint32_t steps, uint32_t s, uint32_t i, uint32_t l) {
+  return (((tile * steps + s) * 8 + i) * 64 + l) * 16;
+}
 
 // 4x4 byte transpose: y_i byte t = x_t byte i.
 __device__ __forceinline__ void transpose4(uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3, uint32_t &y0,
@@ -60,8 +74,8 @@ __device__ __forceinline__ void transpose4(uint32_t x0, uint32_t x1, uint32_t x2
   y3 = __builtin_amdgcn_perm(p3, p1, 0x07060302u);
 }
 
-// Eight values (already in signed-digit form) -> one 64-byte plane block:
-// out[2i + h] = digit i of values 4h .. 4h+3.
+// Eight values (already in signed-digit form) -> their 8 digit planes of 8
+// bytes: out[2i + h] = digit i of values 4h .. 4h+3.
 __device__ __forceinline__ void planes8(const uint64_t (&w)[8], uint32_t (&out)[16]) {
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
@@ -73,84 +87,7 @@ __device__ __forceinline__ void planes8(const uint64_t (&w)[8], uint32_t (&out)[
   }
 }
 
-// Selection vectors as digit planes: block (j-block jb of 8 leaves, poly c)
-// of query b NTTs its 8 polynomials and writes, for each slot, its half
-// (bytes 16 k + 8 (jb % 2) .. +7 for digit k) of the 128-byte record
-// Bp[slot][jb / 2][2b + c].  grid = (fst/8 * 2, nq); fst % 16 == 0.
-__global__ __launch_bounds__(kNttThreads) void k_sel_planes(const uint64_t *leaves, uint64_t leaf_stride,
-                                                            uint8_t *Bp, uint32_t fst, uint32_t ncp, NttTab T) {
-  __shared__ uint64_t lds[kNttLds];
-  const uint32_t jb = blockIdx.x / 2, c = blockIdx.x % 2, b = blockIdx.y;
-  uint64_t xs[8][8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    uint64_t x[8];
-    load_natural(x, leaves + (size_t)b * leaf_stride + (size_t)(jb * 8 + j) * 2 * kN + (size_t)c * kN);
-    ntt_fwd(x, lds, T);
-#pragma unroll
-    for (int r = 0; r < 8; ++r) xs[j][r] = signed_digits(ntt_fwd_final(x[r], T.q));
-  }
-  const uint32_t col = 2 * b + c, nr = fst / 16;
-#pragma unroll
-  for (int r = 0; r < 8; ++r) {
-    const uint32_t slot = 8 * threadIdx.x + r;
-    uint64_t w[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) w[j] = xs[j][r];
-    uint32_t o[16];
-    planes8(w, o);
-    uint2 *dst = reinterpret_cast<uint2 *>(Bp + (((size_t)slot * nr + jb / 2) * ncp + col) * 128 + 8 * (jb % 2));
-#pragma unroll
-    for (int k = 0; k < 8; ++k) dst[2 * k] = make_uint2(o[2 * k], o[2 * k + 1]);
-  }
-}
-
-// A operand of digit i: plane i of values j = 16g .. 16g+7 (p0), then of
-// 16g+8 .. 16g+15 (p1).
-__device__ __forceinline__ v4i_t operand(const uint32_t (&p0)[16], const uint32_t (&p1)[16], int i) {
-  v4i_t v;
-  v[0] = (int)p0[2 * i];
-  v[1] = (int)p0[2 * i + 1];
-  v[2] = (int)p1[2 * i];
-  v[3] = (int)p1[2 * i + 1];
-  return v;
-}
-
-// B operand of digit k: the 16 bytes at 16 k of the lane's record.
-__device__ __forceinline__ v4i_t b_operand(const uint4 &v) {
-  v4i_t r;
-  r[0] = (int)v.x;
-  r[1] = (int)v.y;
-  r[2] = (int)v.z;
-  r[3] = (int)v.w;
-  return r;
-}
-
-// Raw operands of one 32-wide step of j for this lane: the 16 database words
-// j0+16g .. +15 of its row, the plane record j/16 = 2s+g of its column.  Rows
-// past the matrix read row 0 instead (arow points there); their outputs are
-// not stored, and no other output depends on them.
-struct StepRaw {
-  uint4 a[8];
-  uint4 b[8];
-};
-
-__device__ __forceinline__ const uint4 *a_step(const uint64_t *arow, uint32_t s, uint32_t g) {
-  return reinterpret_cast<const uint4 *>(arow + 32 * s + 16 * g);
-}
-__device__ __forceinline__ const uint4 *b_step(const uint8_t *bcol, uint32_t s, uint32_t g, uint32_t ncp) {
-  return reinterpret_cast<const uint4 *>(bcol + (size_t)(2 * s + g) * ncp * 128);
-}
-
-__device__ __forceinline__ void load_step(StepRaw &R, const uint64_t *arow, const uint8_t *bcol, uint32_t s,
-                                          uint32_t g, uint32_t ncp) {
-  const uint4 *a = a_step(arow, s, g), *b = b_step(bcol, s, g, ncp);
-#pragma unroll
-  for (int q = 0; q < 8; ++q) R.a[q] = a[q];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) R.b[q] = b[q];
-}
-
+// Eight raw words (four uint4) -> their digit planes.
 __device__ __forceinline__ void a_planes(const uint4 *a, uint32_t (&out)[16]) {
   uint64_t w[8];
 #pragma unroll
@@ -161,172 +98,169 @@ __device__ __forceinline__ void a_planes(const uint4 *a, uint32_t (&out)[16]) {
   planes8(w, out);
 }
 
-// C mod 2^128 from the 15 diagonal sums of one output, then mod q.
-__device__ __forceinline__ uint64_t diagonals_mod_q(const v16i_t (&acc)[15], int e, const Mod &mq) {
-  int64_t P0 = 0, P1 = 0, P2 = 0;  // weights 2^0, 2^40, 2^80; |P| < 2^61
+// Selection vectors as digit planes: block (j-block jb of 8 leaves, poly c)
+// of query b NTTs its 8 polynomials and writes, for each slot, the half
+// (bytes 8 (jb % 2) .. +7) of the 8 chunks (step jb / 4, lane group
+// (jb / 2) % 2) of column 2b + c.  grid = (fst/8 * 2, nq); fst % 32 == 0.
+// The 64 NTT outputs of a thread stay in registers (two waves per SIMD let
+// the compiler use more than 128 VGPRs).
+template <int J>
+__device__ __forceinline__ void sel_leaves(uint64_t (&xs)[8][8], const uint64_t *src, uint64_t *lds,
+                                           const NttTab &T) {
+  // leaf J, then the following ones: a recursion on a constant J, so xs is
+  // indexed by constants (a loop here is not unrolled and puts xs in scratch)
+  uint64_t x[8];
+  load_natural(x, src + (size_t)J * 2 * kN);
+  ntt_fwd(x, lds, T);
 #pragma unroll
-  for (int d = 0; d < 5; ++d) {
-    P0 += (int64_t)acc[d][e] * ((int64_t)1 << (8 * d));
-    P1 += (int64_t)acc[d + 5][e] * ((int64_t)1 << (8 * d));
-    P2 += (int64_t)acc[d + 10][e] * ((int64_t)1 << (8 * d));
-  }
-  uint64_t lo = (uint64_t)P0, hi = (uint64_t)(P0 >> 63);
-  add128(lo, hi, (uint64_t)P1 << 40, (uint64_t)(P1 >> 24));
-  hi += (uint64_t)P2 << 16;
-  return barrett128(lo, hi, mq);
+  for (int r = 0; r < 8; ++r) xs[J][r] = signed_digits(ntt_fwd_final(x[r], T.q));
+  if constexpr (J + 1 < 8) sel_leaves<J + 1>(xs, src, lds, T);
 }
 
-// First dimension for 2..64 queries.  CB = column blocks of 32 (NCP = 32 CB
-// padded columns); the workgroup's four waves cover RB = 4 / CB row blocks of
-// 32 rows x CB column blocks.  grid.x = N * n_rt with n_rt row tiles of
-// 32 RB rows; all row tiles of a slot run on one XCD (block id mod 8), so the
-// slot's plane blocks are read from that XCD's L2.  Requires fst % 32 == 0.
-template <int CB>
-__global__ __launch_bounds__(256) void k_first_dim_mfma(const uint64_t *__restrict__ db,
-                                                        const uint8_t *__restrict__ Bp,
-                                                        uint64_t *__restrict__ out2, uint32_t fst,
-                                                        uint32_t other, uint64_t num_pt, uint32_t ncols,
-                                                        uint32_t n_rt, Mod mq) {
-  constexpr int RB = 4 / CB, NCP = 32 * CB;
-  const uint32_t L = blockIdx.x, xcd = L & 7, seq = L >> 3;
-  const uint32_t slot = (seq / n_rt) * 8 + xcd, rt = seq % n_rt;
-  const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 5, li = l & 31;
-  const uint32_t rb = w / CB, cb = w % CB;
-  const uint32_t row0 = rt * (32 * RB) + rb * 32;
-  const uint32_t row = row0 + li, col = cb * 32 + li;
-  const uint64_t *arow = db + (size_t)slot * num_pt + (size_t)(row < other ? row : 0) * fst;
-  const uint8_t *bcol = Bp + (size_t)slot * (fst / 16) * NCP * 128 + (size_t)col * 128;
-
-  v16i_t acc[15];
+__global__ __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_sel_planes(
+    const uint64_t *leaves, uint64_t leaf_stride, uint8_t *Bp, uint32_t fst, uint32_t ncp, NttTab T) {
+  __shared__ uint64_t lds[kNttLds];
+  const uint32_t jb = blockIdx.x / 2, c = blockIdx.x % 2, b = blockIdx.y;
+  uint64_t xs[8][8];
+  sel_leaves<0>(xs, leaves + (size_t)b * leaf_stride + (size_t)jb * 8 * 2 * kN + (size_t)c * kN, lds, T);
+  const uint32_t col = 2 * b + c, steps = fst / 32, n_cb = ncp / 32;
+  const uint32_t l = 32 * ((jb / 2) % 2) + col % 32;
 #pragma unroll
-  for (int d = 0; d < 15; ++d) acc[d] = (v16i_t){0};
-
-  // one raw buffer: the step's words become planes, then the buffer takes the
-  // next step's loads while the 64 products run
-  const uint32_t steps = fst / 32;
-  StepRaw raw;
-  load_step(raw, arow, bcol, 0, g, NCP);
-  for (uint32_t s = 0; s < steps; ++s) {
-    uint32_t pa0[16], pa1[16];
-    a_planes(raw.a, pa0);
-    a_planes(raw.a + 4, pa1);
-    v4i_t B[8];
+  for (int r = 0; r < 8; ++r) {
+    const uint32_t slot = 8 * threadIdx.x + r;
+    uint64_t w[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) B[k] = b_operand(raw.b[k]);
-    if (s + 1 < steps) load_step(raw, arow, bcol, s + 1, g, NCP);
+    for (int j = 0; j < 8; ++j) w[j] = xs[j][r];
+    uint32_t o[16];
+    planes8(w, o);
+    uint8_t *dst = Bp + chunk_offset((size_t)slot * n_cb + col / 32, steps, jb / 4, 0, l) + 8 * (jb % 2);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const v4i_t A = operand(pa0, pa1, i);
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-        acc[i + k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, B[k], acc[i + k], 0, 0, 0);
-    }
-  }
-
-  // output (m, n) of register e: n = lane % 32, m = 8 (e / 4) + 4 g + e % 4
-  if (col < ncols) {
-    uint64_t *dst = out2 + (size_t)slot * other * ncols + col;
-#pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const uint32_t m = row0 + 8 * (e / 4) + 4 * g + (e % 4);
-      const uint64_t v = diagonals_mod_q(acc, e, mq);
-      if (m < other) dst[(size_t)m * ncols] = v;
-    }
+    for (int k = 0; k < 8; ++k)
+      *reinterpret_cast<uint2 *>(dst + (size_t)k * 1024) = make_uint2(o[2 * k], o[2 * k + 1]);
   }
 }
 
-// ---- pre-converted database --------------------------------------------------------------
-//
-// The database as digit planes: dbp has the byte size and indexing of db, and
-// the 16 words j = 16 jp .. 16 jp + 15 of (slot, row) hold instead the record
-// of that pair of j-blocks in the B-plane format (byte 16 k + b = digit k of
-// value 16 jp + b).  The database is static between writes, so converting it
-// once per write takes the whole A conversion out of the first dimension: a
-// lane's 8 A operands of a step are its 128-byte record as loaded.
-
-// Records rec_lo .. rec_hi - 1 of every slot (a record is 16 words of one
-// row; num_pt % 16 == 0).  grid = (ceil((rec_hi - rec_lo) / 256), N).
+// The database's digit-plane copy: records rec_lo .. rec_hi - 1 of every
+// slot, a record being the 16 words j = 16 jp .. 16 jp + 15 of one row
+// (record = row * fst / 16 + jp; num_pt = fst * other, fst % 32 == 0).  Each
+// becomes the 8 chunks (tile slot * n_rb + row / 32, step jp / 2, lane
+// 32 (jp % 2) + row % 32).  grid = (ceil((rec_hi - rec_lo) / 256), N).
 __global__ __launch_bounds__(256) void k_db_planes(const uint64_t *__restrict__ db, uint8_t *__restrict__ dbp,
-                                                   uint64_t num_pt, uint64_t rec_lo, uint64_t rec_hi) {
+                                                   uint32_t fst, uint32_t other, uint64_t rec_lo,
+                                                   uint64_t rec_hi) {
   const uint64_t r = rec_lo + (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (r >= rec_hi) return;
-  const size_t w0 = (size_t)blockIdx.y * num_pt + 16 * r;
-  const uint4 *src = reinterpret_cast<const uint4 *>(db + w0);
+  const uint32_t per_row = fst / 16, n_rb = (other + 31) / 32;
+  const uint32_t row = (uint32_t)(r / per_row), jp = (uint32_t)(r % per_row);
+  const size_t slot = blockIdx.y;
+  const uint4 *src = reinterpret_cast<const uint4 *>(db + slot * fst * other + 16 * r);
   uint4 a[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) a[q] = src[q];
   uint32_t p0[16], p1[16];
   a_planes(a, p0);
   a_planes(a + 4, p1);
-  uint4 *dst = reinterpret_cast<uint4 *>(dbp + 8 * w0);
+  uint8_t *dst = dbp + chunk_offset(slot * n_rb + row / 32, fst / 32, jp / 2, 0, 32 * (jp % 2) + row % 32);
 #pragma unroll
-  for (int k = 0; k < 8; ++k) dst[k] = make_uint4(p0[2 * k], p0[2 * k + 1], p1[2 * k], p1[2 * k + 1]);
+  for (int k = 0; k < 8; ++k)
+    *reinterpret_cast<uint4 *>(dst + (size_t)k * 1024) = make_uint4(p0[2 * k], p0[2 * k + 1], p1[2 * k], p1[2 * k + 1]);
 }
 
-// The products of one step from a lane's A and B records.
-__device__ __forceinline__ void step_products(v16i_t (&acc)[15], const StepRaw &R) {
+// Operand of digit i from planes: plane i of values j = 16g .. 16g+7 (p0),
+// then of 16g+8 .. 16g+15 (p1).
+__device__ __forceinline__ v4i_t operand(const uint32_t (&p0)[16], const uint32_t (&p1)[16], int i) {
+  v4i_t v;
+  v[0] = (int)p0[2 * i];
+  v[1] = (int)p0[2 * i + 1];
+  v[2] = (int)p1[2 * i];
+  v[3] = (int)p1[2 * i + 1];
+  return v;
+}
+
+// Operand from a chunk as loaded.
+__device__ __forceinline__ v4i_t as_operand(const uint4 &v) {
+  v4i_t r;
+  r[0] = (int)v.x;
+  r[1] = (int)v.y;
+  r[2] = (int)v.z;
+  r[3] = (int)v.w;
+  return r;
+}
+
+// The 64 products of one step from B chunks and A chunks (step_products) or
+// A database words converted in registers (step_products_words).
+__device__ __forceinline__ void step_products(v16i_t (&acc)[15], const uint4 (&A)[8], const uint4 (&B)[8]) {
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
-    const v4i_t A = b_operand(R.a[i]);
+    const v4i_t a = as_operand(A[i]);
 #pragma unroll
     for (int k = 0; k < 8; ++k)
-      acc[i + k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, b_operand(R.b[k]), acc[i + k], 0, 0, 0);
+      acc[i + k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, as_operand(B[k]), acc[i + k], 0, 0, 0);
+  }
+}
+__device__ __forceinline__ void step_products_words(v16i_t (&acc)[15], const uint4 (&A)[8], const uint4 (&B)[8]) {
+  uint32_t pa0[16], pa1[16];
+  a_planes(A, pa0);
+  a_planes(A + 4, pa1);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const v4i_t a = operand(pa0, pa1, i);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      acc[i + k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, as_operand(B[k]), acc[i + k], 0, 0, 0);
   }
 }
 
-__device__ __forceinline__ void load_records(StepRaw &R, const uint8_t *arec, const uint8_t *brec, uint32_t s,
-                                             uint32_t ncp) {
-  const uint4 *a = reinterpret_cast<const uint4 *>(arec + (size_t)s * 256);
-  const uint4 *b = reinterpret_cast<const uint4 *>(brec + (size_t)s * 2 * ncp * 128);
+// C mod 2^128 from the 15 diagonal sums of one output, then mod q.
+__device__ __forceinline__ uint64_t diagonals_mod_q(const v16i_t (&acc)[15], int e, const Mod &mq) {
+  // P_k = sum_{d<5} S_{5k+d} 2^{8d} (|P_k| < 2^61), formed in two's complement
+  uint64_t P[3];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) R.a[q] = a[q];
+  for (int k = 0; k < 3; ++k) {
+    P[k] = (uint64_t)(int64_t)acc[5 * k][e];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) R.b[q] = b[q];
+    for (int d = 1; d < 5; ++d) P[k] += (uint64_t)(int64_t)acc[5 * k + d][e] << (8 * d);
+  }
+  // C = P0 + P1 2^40 + P2 2^80 mod 2^128
+  uint64_t lo = P[0], hi = (uint64_t)((int64_t)P[0] >> 63);
+  add128(lo, hi, P[1] << 40, (uint64_t)((int64_t)P[1] >> 24));
+  hi += P[2] << 16;
+  return barrett128(lo, hi, mq);
 }
 
-// First dimension for 2..64 queries from the pre-converted database: the tiles
-// of k_first_dim_mfma, with three register buffers so each step's records are
-// loaded two steps before their products.
+// Common tile geometry.  CB = column blocks of 32 (NCP = 32 CB padded
+// columns); the workgroup's four waves cover RB = 4 / CB row blocks of 32
+// rows x CB column blocks.  grid.x = N * n_rt with n_rt row tiles of 32 RB
+// rows; all row tiles of a slot run on one XCD (block id mod 8), so the
+// slot's B chunks are read from that XCD's L2.  Requires fst % 32 == 0.
 template <int CB>
-__global__ __launch_bounds__(256) void k_first_dim_mfma_pre(const uint8_t *__restrict__ dbp,
-                                                            const uint8_t *__restrict__ Bp,
-                                                            uint64_t *__restrict__ out2, uint32_t fst,
-                                                            uint32_t other, uint64_t num_pt, uint32_t ncols,
-                                                            uint32_t n_rt, Mod mq) {
-  constexpr int RB = 4 / CB, NCP = 32 * CB;
-  const uint32_t L = blockIdx.x, xcd = L & 7, seq = L >> 3;
-  const uint32_t slot = (seq / n_rt) * 8 + xcd, rt = seq % n_rt;
-  const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 5, li = l & 31;
-  const uint32_t rb = w / CB, cb = w % CB;
-  const uint32_t row0 = rt * (32 * RB) + rb * 32;
-  const uint32_t row = row0 + li, col = cb * 32 + li;
-  // record 2s + g of the row (rows past the matrix read row 0; not stored)
-  const uint8_t *arec = dbp + ((size_t)slot * num_pt + (size_t)(row < other ? row : 0) * fst) * 8 + (size_t)g * 128;
-  const uint8_t *brec = Bp + ((size_t)slot * (fst / 16) + g) * NCP * 128 + (size_t)col * 128;
-
-  v16i_t acc[15];
-#pragma unroll
-  for (int d = 0; d < 15; ++d) acc[d] = (v16i_t){0};
-
-  const uint32_t steps = fst / 32, body = steps - steps % 3;
-  StepRaw R[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) load_records(R[d], arec, brec, min((uint32_t)d, steps - 1), NCP);
-  // groups of three steps; a buffer is refilled (with step s + d + 3, or
-  // again the last step once past the end) as soon as its products issue
-  for (uint32_t s = 0; s < body; s += 3) {
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      step_products(acc, R[d]);
-      load_records(R[d], arec, brec, min(s + d + 3, steps - 1), NCP);
-    }
+struct Tile {
+  static constexpr int RB = 4 / CB, NCP = 32 * CB;
+  uint32_t slot, rt, w, l, g, li, rb, cb, row0, rblk, n_rb, steps;
+  __device__ __forceinline__ Tile(uint32_t fst, uint32_t other, uint32_t n_rt) {
+    const uint32_t L = blockIdx.x, xcd = L & 7, seq = L >> 3;
+    slot = (seq / n_rt) * 8 + xcd;
+    rt = seq % n_rt;
+    w = threadIdx.x >> 6;
+    l = threadIdx.x & 63;
+    g = l >> 5;
+    li = l & 31;
+    rb = w / CB;
+    cb = w % CB;
+    row0 = rt * (32 * RB) + rb * 32;
+    n_rb = (other + 31) / 32;
+    rblk = min(row0 / 32, n_rb - 1);  // blocks wholly past the matrix read the last one; not stored
+    steps = fst / 32;
   }
-#pragma unroll
-  for (int d = 0; d < 2; ++d)
-    if (body + d < steps) step_products(acc, R[d]);
-
-  if (col < ncols) {
+  __device__ __forceinline__ const uint8_t *b_base(const uint8_t *Bp) const {
+    return Bp + chunk_offset((size_t)slot * CB + cb, steps, 0, 0, l);
+  }
+  __device__ __forceinline__ void store(const v16i_t (&acc)[15], uint64_t *out2, uint32_t other, uint32_t ncols,
+                                        const Mod &mq) const {
+    const uint32_t col = cb * 32 + li;
+    if (col >= ncols) return;
     uint64_t *dst = out2 + (size_t)slot * other * ncols + col;
+    // output (m, n) of register e: n = lane % 32, m = 8 (e / 4) + 4 g + e % 4
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       const uint32_t m = row0 + 8 * (e / 4) + 4 * g + (e % 4);
@@ -334,22 +268,115 @@ __global__ __launch_bounds__(256) void k_first_dim_mfma_pre(const uint8_t *__res
       if (m < other) dst[(size_t)m * ncols] = v;
     }
   }
+};
+
+// Operands of one step for this lane: A (chunks, or the 16 database words of
+// the row) and the B chunks.
+struct StepRaw {
+  uint4 a[8];
+  uint4 b[8];
+};
+
+__device__ __forceinline__ void load_chunks(uint4 (&x)[8], const uint8_t *base, uint32_t s) {
+  const uint8_t *p = base + (size_t)s * kStepBytes;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) x[i] = *reinterpret_cast<const uint4 *>(p + i * 1024);
+}
+__device__ __forceinline__ void load_words(uint4 (&x)[8], const uint64_t *arow, uint32_t s, uint32_t g) {
+  const uint4 *p = reinterpret_cast<const uint4 *>(arow + 32 * s + 16 * g);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) x[q] = p[q];
+}
+
+// First dimension for 2..64 queries from the database words: one register
+// buffer; the step's words become planes, then the buffer takes the next
+// step's loads while the 64 products run.
+template <int CB>
+__global__ __launch_bounds__(256) void k_first_dim_mfma(const uint64_t *__restrict__ db,
+                                                        const uint8_t *__restrict__ Bp,
+                                                        uint64_t *__restrict__ out2, uint32_t fst,
+                                                        uint32_t other, uint32_t ncols, uint32_t n_rt, Mod mq) {
+  const Tile<CB> T(fst, other, n_rt);
+  const uint32_t row = T.row0 + T.li;
+  // rows past the matrix read row 0; their outputs are not stored
+  const uint64_t *arow = db + (size_t)T.slot * fst * other + (size_t)(row < other ? row : 0) * fst;
+  const uint8_t *bb = T.b_base(Bp);
+  v16i_t acc[15];
+#pragma unroll
+  for (int d = 0; d < 15; ++d) acc[d] = (v16i_t){0};
+  StepRaw R;
+  load_words(R.a, arow, 0, T.g);
+  load_chunks(R.b, bb, 0);
+  for (uint32_t s = 0; s < T.steps; ++s) {
+    uint32_t pa0[16], pa1[16];
+    a_planes(R.a, pa0);
+    a_planes(R.a + 4, pa1);
+    uint4 B[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) B[k] = R.b[k];
+    if (s + 1 < T.steps) {
+      load_words(R.a, arow, s + 1, T.g);
+      load_chunks(R.b, bb, s + 1);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const v4i_t a = operand(pa0, pa1, i);
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        acc[i + k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, as_operand(B[k]), acc[i + k], 0, 0, 0);
+    }
+  }
+  T.store(acc, out2, other, ncols, mq);
+}
+
+// First dimension for 2..64 queries from the digit-plane copy: every operand
+// is a contiguous 1 KiB load; three register buffers, so each step's chunks
+// are loaded two steps before their products.
+template <int CB>
+__global__ __launch_bounds__(256) void k_first_dim_mfma_pre(const uint8_t *__restrict__ dbp,
+                                                            const uint8_t *__restrict__ Bp,
+                                                            uint64_t *__restrict__ out2, uint32_t fst,
+                                                            uint32_t other, uint32_t ncols, uint32_t n_rt, Mod mq) {
+  const Tile<CB> T(fst, other, n_rt);
+  const uint8_t *ab = dbp + chunk_offset((size_t)T.slot * T.n_rb + T.rblk, T.steps, 0, 0, T.l);
+  const uint8_t *bb = T.b_base(Bp);
+  v16i_t acc[15];
+#pragma unroll
+  for (int d = 0; d < 15; ++d) acc[d] = (v16i_t){0};
+  const uint32_t steps = T.steps, body = steps - steps % 3;
+  StepRaw R[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    load_chunks(R[d].a, ab, min((uint32_t)d, steps - 1));
+    load_chunks(R[d].b, bb, min((uint32_t)d, steps - 1));
+  }
+  // groups of three steps; a buffer is refilled (with step s + d + 3, or
+  // again the last step once past the end) as soon as its products issue
+  for (uint32_t s = 0; s < body; s += 3) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      step_products(acc, R[d].a, R[d].b);
+      const uint32_t n = min(s + d + 3, steps - 1);
+      load_chunks(R[d].a, ab, n);
+      load_chunks(R[d].b, bb, n);
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 2; ++d)
+    if (body + d < steps) step_products(acc, R[d].a, R[d].b);
+  T.store(acc, out2, other, ncols, mq);
 }
 
 // ---- LDS-pipelined variant ------------------------------------------------------------
 //
-// Same tiles and arithmetic as k_first_dim_mfma, but the operands travel
-// global -> LDS by direct-to-LDS loads (global_load_lds_dwordx4), three steps
-// deep, so the matrix cores are not stalled by HBM latency and no VGPRs hold
-// loads in flight.  A step's operands are 16-byte "pieces": piece p of an A
-// row block holds, for reading lane l, the two database words
-// j = 32 s + 16 g + 2 p, +1 of row (l % 32), and piece p of a B column block
-// holds bytes 16 p .. 16 p + 15 (the operand for digit p) of plane record
-// 2 s + g of column (l % 32); g = l / 32.  A piece is 1 KiB in LDS with lane
-// l's 16 bytes at 16 l, so both the DMA write and the reading ds_read_b128
-// are contiguous.
-// The RB row blocks and CB column blocks of a step (8 (RB + CB) pieces) are
-// loaded by the four waves together and shared through LDS.
+// Same tiles and arithmetic, but the operands travel global -> LDS by
+// direct-to-LDS loads (global_load_lds_dwordx4), three steps deep, so no VGPRs
+// hold loads in flight, and the RB row blocks and CB column blocks of a step
+// (8 (RB + CB) "pieces" of 1 KiB) are loaded once by the four waves together.
+// Piece p of a block holds, at 16 l, reading lane l's operand for digit p
+// (chunks), or for A without the plane copy the database words
+// j = 32 s + 16 g + 2 p, +1 of row (l % 32).  Both the DMA write and the
+// reading ds_read_b128 are contiguous.
 
 constexpr int kMfmaStages = 3;
 
@@ -373,25 +400,29 @@ struct MfmaPipe {
 };
 
 // Issue this wave's share of the loads of step s into LDS stage `stage`.
-template <int CB>
-__device__ __forceinline__ void issue_step(uint8_t *lds, int stage, uint32_t s, const uint64_t *a_base,
-                                           uint32_t fst, uint32_t row_base, uint32_t other, const uint8_t *b_base,
-                                           uint32_t w, uint32_t l) {
+// a_base: the tile's first A chunk (PRE) or the slot's database words.
+template <int CB, bool PRE>
+__device__ __forceinline__ void issue_step(uint8_t *lds, int stage, uint32_t s, const Tile<CB> &T,
+                                           const uint8_t *a_base, uint32_t fst, uint32_t other,
+                                           const uint8_t *b_base) {
   using P = MfmaPipe<CB>;
-  const uint32_t g = l >> 5, li = l & 31;
 #pragma unroll
   for (int i = 0; i < P::PER_WAVE; ++i) {
-    const uint32_t piece = w + 4 * i;  // wave-uniform
+    const uint32_t piece = T.w + 4 * i;  // wave-uniform
     uint8_t *dst = lds + stage * P::STAGE_BYTES + piece * 1024;
     const uint32_t p = piece & 7;
     const uint32_t blk = piece >> 3;
     if (blk < (uint32_t)P::RB) {
-      uint32_t row = row_base + blk * 32 + li;
-      row = row < other ? row : 0;  // masked rows read row 0; their outputs are not stored
-      lds_dma16(a_base + (size_t)row * fst + 32 * s + 16 * g + 2 * p, dst);
+      if (PRE) {
+        const uint32_t rblk = min(T.rt * P::RB + blk, T.n_rb - 1);
+        lds_dma16(a_base + chunk_offset(rblk, T.steps, s, p, T.l), dst);
+      } else {
+        uint32_t row = T.rt * (32 * P::RB) + blk * 32 + T.li;
+        row = row < other ? row : 0;  // masked rows read row 0; their outputs are not stored
+        lds_dma16(reinterpret_cast<const uint64_t *>(a_base) + (size_t)row * fst + 32 * s + 16 * T.g + 2 * p, dst);
+      }
     } else {
-      const uint32_t cb = blk - P::RB, col = cb * 32 + li;
-      lds_dma16(b_base + ((size_t)(2 * s + g) * (32 * CB) + col) * 128 + 16 * p, dst);
+      lds_dma16(b_base + chunk_offset(blk - P::RB, T.steps, s, p, T.l), dst);
     }
   }
 }
@@ -434,24 +465,21 @@ __device__ __forceinline__ void lds_read_step(uint4 (&ra)[8], uint4 (&rb)[8], ui
   rb[4] = to_uint4(y4); rb[5] = to_uint4(y5); rb[6] = to_uint4(y6); rb[7] = to_uint4(y7);
 }
 
-// PRE: db is the digit-plane copy (k_db_planes; same indexing), so a piece of
-// A is already the operand of one digit.
+// PRE: A from the digit-plane copy dbp (chunks); otherwise from the database
+// words db.
 template <int CB, bool PRE>
 __global__ __launch_bounds__(256) void k_first_dim_mfma_lds(const uint64_t *__restrict__ db,
+                                                            const uint8_t *__restrict__ dbp,
                                                             const uint8_t *__restrict__ Bp,
                                                             uint64_t *__restrict__ out2, uint32_t fst,
-                                                            uint32_t other, uint64_t num_pt, uint32_t ncols,
-                                                            uint32_t n_rt, Mod mq) {
+                                                            uint32_t other, uint32_t ncols, uint32_t n_rt, Mod mq) {
   using P = MfmaPipe<CB>;
-  constexpr int RB = P::RB, NCP = 32 * CB;
+  constexpr int RB = P::RB;
   __shared__ __attribute__((aligned(16))) uint8_t lds[kMfmaStages * P::STAGE_BYTES];
-  const uint32_t L = blockIdx.x, xcd = L & 7, seq = L >> 3;
-  const uint32_t slot = (seq / n_rt) * 8 + xcd, rt = seq % n_rt;
-  const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 5, li = l & 31;
-  const uint32_t rb = w / CB, cb = w % CB;
-  const uint32_t row_base = rt * (32 * RB);
-  const uint64_t *a_base = db + (size_t)slot * num_pt;
-  const uint8_t *b_base = Bp + (size_t)slot * (fst / 16) * NCP * 128;
+  const Tile<CB> T(fst, other, n_rt);
+  const uint8_t *a_base = PRE ? dbp + chunk_offset((size_t)T.slot * T.n_rb, T.steps, 0, 0, 0)
+                              : reinterpret_cast<const uint8_t *>(db + (size_t)T.slot * fst * other);
+  const uint8_t *b_base = Bp + chunk_offset((size_t)T.slot * CB, T.steps, 0, 0, 0);
 
   v16i_t acc[15];
 #pragma unroll
@@ -459,10 +487,10 @@ __global__ __launch_bounds__(256) void k_first_dim_mfma_lds(const uint64_t *__re
 
   // LDS byte offset of the staging area (generic -> local address space)
   const uint32_t lds_off =
-      (uint32_t)(uintptr_t)(__attribute__((address_space(3))) uint8_t *)lds + l * 16;
-  const uint32_t steps = fst / 32;
-  issue_step<CB>(lds, 0, 0, a_base, fst, row_base, other, b_base, w, l);
-  if (steps > 1) issue_step<CB>(lds, 1, 1, a_base, fst, row_base, other, b_base, w, l);
+      (uint32_t)(uintptr_t)(__attribute__((address_space(3))) uint8_t *)lds + T.l * 16;
+  const uint32_t steps = T.steps;
+  issue_step<CB, PRE>(lds, 0, 0, T, a_base, fst, other, b_base);
+  if (steps > 1) issue_step<CB, PRE>(lds, 1, 1, T, a_base, fst, other, b_base);
   for (uint32_t s = 0; s < steps; ++s) {
     // this wave's loads of step s are done once at most one later step is in flight
     if (s + 1 < steps)
@@ -473,34 +501,16 @@ __global__ __launch_bounds__(256) void k_first_dim_mfma_lds(const uint64_t *__re
     // (its reads wait for completion), so stage (s + 2) % 3 is free.  A plain
     // s_barrier: the loads of later steps stay in flight across it.
     asm volatile("s_barrier" ::: "memory");
-    if (s + 2 < steps) issue_step<CB>(lds, (s + 2) % kMfmaStages, s + 2, a_base, fst, row_base, other, b_base, w, l);
+    if (s + 2 < steps) issue_step<CB, PRE>(lds, (s + 2) % kMfmaStages, s + 2, T, a_base, fst, other, b_base);
     const uint32_t st = lds_off + (s % kMfmaStages) * P::STAGE_BYTES;
     uint4 ra[8], rbv[8];
-    lds_read_step(ra, rbv, st + rb * 8 * 1024, st + (RB + cb) * 8 * 1024);
-    uint32_t pa0[16], pa1[16];
-    if (!PRE) {
-      a_planes(ra, pa0);
-      a_planes(ra + 4, pa1);
-    }
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const v4i_t A = PRE ? b_operand(ra[i]) : operand(pa0, pa1, i);
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-        acc[i + k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, b_operand(rbv[k]), acc[i + k], 0, 0, 0);
-    }
+    lds_read_step(ra, rbv, st + T.rb * 8 * 1024, st + (RB + T.cb) * 8 * 1024);
+    if (PRE)
+      step_products(acc, ra, rbv);
+    else
+      step_products_words(acc, ra, rbv);
   }
-
-  const uint32_t row0 = row_base + rb * 32, col = cb * 32 + li;
-  if (col < ncols) {
-    uint64_t *dst = out2 + (size_t)slot * other * ncols + col;
-#pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const uint32_t m = row0 + 8 * (e / 4) + 4 * g + (e % 4);
-      const uint64_t v = diagonals_mod_q(acc, e, mq);
-      if (m < other) dst[(size_t)m * ncols] = v;
-    }
-  }
+  T.store(acc, out2, other, ncols, mq);
 }
 
 // Inverse NTT of the first-dimension outputs gathered from out2[slot][r][c]

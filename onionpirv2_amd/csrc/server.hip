@@ -293,14 +293,20 @@ int bind_keys(const onion_server *s, Lane *L, uint32_t n, const uint64_t *ids, b
   return ONION_OK;
 }
 
-// Rebuild the plane records that cover plaintexts pt_lo .. pt_lo + n - 1 of
+// Bytes of the digit-plane copy: rows padded to a multiple of 32.
+uint64_t planes_bytes(const onion_server *s) {
+  return (uint64_t)kN * ((s->P.other_dim + 31) / 32 * 32) * s->P.fst_dim * 8;
+}
+
+// Rebuild the plane chunks that cover plaintexts pt_lo .. pt_lo + n - 1 of
 // every slot from the database words (stream st; the caller holds the gate
-// exclusively).  A record is 16 plaintexts of one row; num_pt % 16 == 0
-// whenever the copy exists.
+// exclusively).  The unit is a record of 16 plaintexts of one row (fst % 32
+// == 0 whenever the copy exists, so records never straddle rows).
 void planes_update(const onion_server *s, hipStream_t st, uint64_t pt_lo, uint64_t n) {
   if (!s->dbp || !n) return;
   const uint64_t lo = pt_lo / 16, hi = (pt_lo + n + 15) / 16;
-  k_db_planes<<<dim3((uint32_t)((hi - lo + 255) / 256), kN), 256, 0, st>>>(s->db, s->dbp, s->I.num_pt, lo, hi);
+  k_db_planes<<<dim3((uint32_t)((hi - lo + 255) / 256), kN), 256, 0, st>>>(s->db, s->dbp, s->P.fst_dim,
+                                                                           s->P.other_dim, lo, hi);
 }
 
 // Allocate (and fill) or free the plane copy to match tune.db_planes.  The
@@ -316,7 +322,7 @@ int sync_planes(onion_server *s) {
     return ONION_OK;
   }
   int rc;
-  if ((rc = dalloc(&s->dbp, (size_t)s->I.num_pt * kN * 8))) return rc;
+  if ((rc = dalloc(&s->dbp, planes_bytes(s)))) return rc;
   planes_update(s, st, 0, s->I.num_pt);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(st));
@@ -404,19 +410,18 @@ bool use_mfma(const onion_server *s, const Lane *L, uint32_t nq) {
 template <int CB>
 void launch_mfma(const onion_server *s, Lane *L, const uint8_t *Bp, dim3 grid, uint32_t n_rt, uint32_t ncols) {
   const uint32_t fst = s->P.fst_dim, other = s->P.other_dim;
-  const uint64_t np = s->I.num_pt;
   const bool lds = s->tune.fd_mfma_variant == 1;
   if (s->dbp) {
     if (lds)
-      k_first_dim_mfma_lds<CB, true><<<grid, 256, 0, L->st>>>(reinterpret_cast<const uint64_t *>(s->dbp), Bp,
-                                                             L->digits, fst, other, np, ncols, n_rt, s->mq);
+      k_first_dim_mfma_lds<CB, true><<<grid, 256, 0, L->st>>>(s->db, s->dbp, Bp, L->digits, fst, other, ncols,
+                                                             n_rt, s->mq);
     else
-      k_first_dim_mfma_pre<CB><<<grid, 256, 0, L->st>>>(s->dbp, Bp, L->digits, fst, other, np, ncols, n_rt, s->mq);
+      k_first_dim_mfma_pre<CB><<<grid, 256, 0, L->st>>>(s->dbp, Bp, L->digits, fst, other, ncols, n_rt, s->mq);
   } else if (lds) {
-    k_first_dim_mfma_lds<CB, false><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, np, ncols, n_rt,
-                                                              s->mq);
+    k_first_dim_mfma_lds<CB, false><<<grid, 256, 0, L->st>>>(s->db, s->dbp, Bp, L->digits, fst, other, ncols,
+                                                              n_rt, s->mq);
   } else {
-    k_first_dim_mfma<CB><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, np, ncols, n_rt, s->mq);
+    k_first_dim_mfma<CB><<<grid, 256, 0, L->st>>>(s->db, Bp, L->digits, fst, other, ncols, n_rt, s->mq);
   }
 }
 
