@@ -64,6 +64,18 @@ def measured_traffic(kernel: str, fst: int, other: int):
     return None, None
 
 
+def mfma_kernel(tuning: dict, batch: int):
+    """(name, file key) of the matrix-core first-dimension kernel the server
+    launches for a batch under this tuning."""
+    cb = 1 if batch <= 16 else 2 if batch <= 32 else 4
+    lds = tuning.get("fd_mfma_variant") == 1
+    if tuning.get("db_planes"):
+        return (f"k_first_dim_mfma_lds<{cb}, true>", "k_first_dim_mfma_lds_pre") if lds else \
+               (f"k_first_dim_mfma_pre<{cb}>", "k_first_dim_mfma_pre")
+    return (f"k_first_dim_mfma_lds<{cb}, false>", "k_first_dim_mfma_lds") if lds else \
+           (f"k_first_dim_mfma<{cb}>", "k_first_dim_mfma")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,7 +205,8 @@ def main() -> int:
     int8_ops = 2 * 64 * (ncp * num_pt * N)  # the padded column count is what the tiles compute
     useful_ops = 2 * 64 * (2 * B * num_pt * N)
     mfma_alg_bytes = db_bytes + N * fst * ncp * 8 + N * other * 2 * B * 8
-    mfma_traffic, mfma_traffic_src = measured_traffic(f"k_first_dim_mfma_b{B}", fst, other)
+    mfma_name, mfma_key = mfma_kernel(tuning, B)
+    mfma_traffic, mfma_traffic_src = measured_traffic(f"{mfma_key}_b{B}", fst, other)
 
     # value: device time of each step (HIP events from the first kernel to the
     # last, queries already in HBM); the wall clock around answer_batch also
@@ -235,7 +248,7 @@ def main() -> int:
                          "stage_ms": {k: float(np.mean([s[k] for s in stages])) for k in stages[0]}},
         "batched_vs_single": value / single_qps_dev,
         "bit_exact": bit_exact,
-        "roofline": {"kernel": "k_first_dim_mfma", "bound": "mfma", "achieved": useful_ops / (mfma_ms * 1e-3) / 1e12,
+        "roofline": {"kernel": mfma_name, "bound": "mfma", "achieved": useful_ops / (mfma_ms * 1e-3) / 1e12,
                      "peak": INT8_PEAK_TOPS, "unit": "TOP/s (int8)",
                      "frac": useful_ops / (mfma_ms * 1e-3) / 1e12 / INT8_PEAK_TOPS,
                      "traffic": mfma_traffic, "traffic_source": mfma_traffic_src, "avg_ms": mfma_ms,

@@ -102,8 +102,9 @@ typedef struct onion_tuning {
                              the matrix-core first dimension reads operands
                              as stored; 0: free it and convert inside the
                              kernel.  Only for first dimensions that are a
-                             multiple of 32.  The server starts with 1 and
-                             falls back to 0 when the copy does not fit.    */
+                             multiple of 32.  The server starts with 1 when
+                             the copy takes at most a third of the free
+                             device memory, else with 0.                    */
 } onion_tuning;
 
 typedef struct onion_server onion_server;
@@ -181,8 +182,9 @@ int onion_gpu_set_tuning(onion_server *srv, const onion_tuning *tuning);
 /* Pick the tuning on the live database (no reference counterpart): times the
  * whole pipeline for each of the n_sizes batch sizes (1..64) with a synthetic
  * client, minimum of `reps` runs, and walks each choice in turn keeping what
- * lowers the summed device time per query.  Applies the result and writes it
- * to *chosen (may be NULL).  Waits for batches in flight. */
+ * lowers the sum over the sizes of each size's device time relative to the
+ * starting tuning's (so every size weighs alike).  Applies the result and
+ * writes it to *chosen (may be NULL).  Waits for batches in flight. */
 int onion_gpu_autotune(onion_server *srv, const uint32_t *batch_sizes, uint32_t n_sizes, uint32_t reps,
                        onion_tuning *chosen);
 

@@ -211,20 +211,36 @@ __device__ __forceinline__ void step_products_words(v16i_t (&acc)[15], const uin
   }
 }
 
-// C mod 2^128 from the 15 diagonal sums of one output, then mod q.
-__device__ __forceinline__ uint64_t diagonals_mod_q(const v16i_t (&acc)[15], int e, const Mod &mq) {
-  // P_k = sum_{d<5} S_{5k+d} 2^{8d} (|P_k| < 2^61), formed in two's complement
-  uint64_t P[3];
+// The weights 2^{8d}, d = 0..3, held in registers the optimizer cannot see
+// through, so S * 2^{8d} + P stays one v_mad_i64_i32 (a product by a known
+// power of two would become a sign extension, a 64-bit shift and an add).
+struct DigitWeights {
+  int32_t w[4];
+  __device__ __forceinline__ DigitWeights() {
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    P[k] = (uint64_t)(int64_t)acc[5 * k][e];
-#pragma unroll
-    for (int d = 1; d < 5; ++d) P[k] += (uint64_t)(int64_t)acc[5 * k + d][e] << (8 * d);
+    for (int d = 0; d < 4; ++d) {
+      w[d] = 1 << (8 * d);
+      asm volatile("" : "+v"(w[d]));
+    }
   }
-  // C = P0 + P1 2^40 + P2 2^80 mod 2^128
-  uint64_t lo = P[0], hi = (uint64_t)((int64_t)P[0] >> 63);
-  add128(lo, hi, P[1] << 40, (uint64_t)((int64_t)P[1] >> 24));
-  hi += P[2] << 16;
+};
+
+// C mod 2^128 from the 15 diagonal sums of one output, then mod q.
+__device__ __forceinline__ uint64_t diagonals_mod_q(const v16i_t (&acc)[15], int e, const Mod &mq,
+                                                    const DigitWeights &W) {
+  // P_k = sum_{d<4} S_{4k+d} 2^{8d}: |S| < 2^31, so |P_k| < 2^56
+  int64_t P[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    P[k] = acc[4 * k][e];
+#pragma unroll
+    for (int d = 1; d < 4; ++d)
+      if (4 * k + d < 15) P[k] = (int64_t)acc[4 * k + d][e] * (int64_t)W.w[d] + P[k];
+  }
+  // C = P0 + P1 2^32 + P2 2^64 + P3 2^96 mod 2^128
+  uint64_t lo = (uint64_t)P[0], hi = (uint64_t)(P[0] >> 63);
+  add128(lo, hi, (uint64_t)P[1] << 32, (uint64_t)(P[1] >> 32));
+  hi += (uint64_t)P[2] + ((uint64_t)P[3] << 32);
   return barrett128(lo, hi, mq);
 }
 
@@ -260,11 +276,12 @@ struct Tile {
     const uint32_t col = cb * 32 + li;
     if (col >= ncols) return;
     uint64_t *dst = out2 + (size_t)slot * other * ncols + col;
+    const DigitWeights W;
     // output (m, n) of register e: n = lane % 32, m = 8 (e / 4) + 4 g + e % 4
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       const uint32_t m = row0 + 8 * (e / 4) + 4 * g + (e % 4);
-      const uint64_t v = diagonals_mod_q(acc, e, mq);
+      const uint64_t v = diagonals_mod_q(acc, e, mq, W);
       if (m < other) dst[(size_t)m * ncols] = v;
     }
   }
@@ -513,33 +530,68 @@ __global__ __launch_bounds__(256) void k_first_dim_mfma_lds(const uint64_t *__re
   T.store(acc, out2, other, ncols, mq);
 }
 
-// Inverse NTT of the first-dimension outputs gathered from out2[slot][r][c]
+// Inverse NTTs of the first-dimension outputs gathered from out2[slot][r][c]
 // into fd (query b = c / 2, poly c % 2 of ciphertext r, coefficient form).
-// One workgroup per (r, c); when other % 8 == 0 the rows are dealt to XCDs
-// (r = xcd mod 8) with c fastest, so the workgroups that share a cache line
-// of out2 run on one XCD at about the same time.
-__global__ __launch_bounds__(kNttThreads) void k_ntt_inv_gather(const uint64_t *__restrict__ out2,
-                                                                uint32_t other, uint32_t ncols, uint64_t *fd,
-                                                                uint64_t fd_stride, NttTab T) {
-  __shared__ uint64_t lds[kNttLds];
-  uint32_t r, c;
-  if (other % 8 == 0) {
-    const uint32_t xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
-    r = 8 * (k / ncols) + xcd;
-    c = k % ncols;
-  } else {
-    r = blockIdx.x / ncols;
-    c = blockIdx.x % ncols;
-  }
-  const size_t sstride = (size_t)other * ncols;
-  const uint64_t *src = out2 + (size_t)r * ncols + c;
-  uint64_t x[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) x[j] = src[(size_t)(8 * threadIdx.x + j) * sstride];
-  ntt_inv(x, lds, T);
+// One workgroup per (row, group of G adjacent columns): a thread reads the G
+// columns of each of its 8 slots as one contiguous 8G-byte piece (the
+// slot-major out2 puts 1 MiB between slots, so the run length is what counts)
+// and runs the G transforms one after another.  DIFF (first multiplexer
+// level): row r < half = other / 2 pairs with row r + half, and the
+// transformed value is y - x = INTT(NTT(y) - NTT(x)), the reference's
+// coefficient-form difference mod q (INTT is linear), written over row
+// r + half.  Rows are dealt to XCDs (r = xcd mod 8, column groups fastest)
+// when the row count is a multiple of 8, so the workgroups reading the same
+// cache lines of out2 share an L2.
+// Column K of the group, then the following ones (a recursion on a constant
+// K keeps x in registers; a loop around the transform is not unrolled).
+template <int G, int K = 0>
+__device__ __forceinline__ void gather_transforms(uint64_t (&x)[G][8], uint64_t *lds, const NttTab &T, uint64_t *fd,
+                                                  uint64_t fd_stride, uint32_t r, uint32_t c0) {
+  ntt_inv(x[K], lds, T);
+  const uint32_t c = c0 + K;
   uint64_t *dst = fd + (size_t)(c >> 1) * fd_stride + ((size_t)r * 2 + (c & 1)) * kN;
 #pragma unroll
-  for (int j = 0; j < 8; ++j) dst[threadIdx.x + 256 * j] = ntt_inv_final(x[j], T);
+  for (int j = 0; j < 8; ++j) dst[threadIdx.x + 256 * j] = ntt_inv_final(x[K][j], T);
+  if constexpr (K + 1 < G) gather_transforms<G, K + 1>(x, lds, T, fd, fd_stride, r, c0);
+}
+
+template <int G, bool DIFF>
+__global__ __launch_bounds__(kNttThreads) void k_ntt_inv_gather(const uint64_t *__restrict__ out2, uint32_t other,
+                                                                uint32_t ncols, uint64_t *fd, uint64_t fd_stride,
+                                                                NttTab T) {
+  __shared__ uint64_t lds[kNttLds];
+  const uint32_t rows = DIFF ? other / 2 : other, ngroups = ncols / G;
+  uint32_t r, cg;
+  if (rows % 8 == 0) {
+    const uint32_t xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+    r = 8 * (k / ngroups) + xcd;
+    cg = k % ngroups;
+  } else {
+    r = blockIdx.x / ngroups;
+    cg = blockIdx.x % ngroups;
+  }
+  const size_t sstride = (size_t)other * ncols;
+  const uint64_t *sx = out2 + (size_t)r * ncols + cg * G;
+  const uint64_t *sy = sx + (size_t)(other / 2) * ncols;
+  uint64_t x[G][8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const size_t o = (size_t)(8 * threadIdx.x + j) * sstride;
+#pragma unroll
+    for (int k = 0; k < G; k += 2) {
+      const uint4 a = *reinterpret_cast<const uint4 *>(sx + o + k);
+      const uint64_t a0 = ((uint64_t)a.y << 32) | a.x, a1 = ((uint64_t)a.w << 32) | a.z;
+      if (DIFF) {
+        const uint4 b = *reinterpret_cast<const uint4 *>(sy + o + k);
+        x[k][j] = sub_mod(((uint64_t)b.y << 32) | b.x, a0, T.q);
+        x[k + 1][j] = sub_mod(((uint64_t)b.w << 32) | b.z, a1, T.q);
+      } else {
+        x[k][j] = a0;
+        x[k + 1][j] = a1;
+      }
+    }
+  }
+  gather_transforms<G>(x, lds, T, fd, fd_stride, DIFF ? r + other / 2 : r, cg * G);
 }
 
 }  // namespace onion

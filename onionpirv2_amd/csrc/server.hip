@@ -164,7 +164,7 @@ struct onion_server {
   uint8_t *pack = nullptr;
   // measured on MI355X at 512x1024: 8 nontemporal 16-byte loads in flight per
   // lane and 256-row blocks stream the database at 94% of a plain read kernel
-  onion_tuning tune{1024, 256, 4, 3, 0, 2, 1, 1};
+  onion_tuning tune{1024, 256, 4, 3, 0, 2, 0, 1};
   uint64_t tune_epoch = 1;
   Gate gate;
   mutable std::mutex last_mu;
@@ -323,9 +323,11 @@ int sync_planes(onion_server *s) {
   }
   int rc;
   if ((rc = dalloc(&s->dbp, planes_bytes(s)))) return rc;
-  planes_update(s, st, 0, s->I.num_pt);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(st));
+  if (s->n_written) {  // plaintexts written later convert as they arrive
+    planes_update(s, st, 0, s->I.num_pt);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+  }
   return ONION_OK;
 }
 
@@ -425,8 +427,26 @@ void launch_mfma(const onion_server *s, Lane *L, const uint8_t *Bp, dim3 grid, u
   }
 }
 
+// Inverse transforms of the slot-major outputs out2 into L->fd: all rows, or
+// (DIFF) the pair differences of the first multiplexer level.  Column groups
+// of 8, 4 or 2 (ncols = 2 nq is even).
+template <bool DIFF>
+void launch_gather(const onion_server *s, Lane *L, uint32_t nq) {
+  const uint32_t other = s->P.other_dim, ncols = 2 * nq, rows = DIFF ? other / 2 : other;
+  const uint32_t G = ncols % 8 == 0 ? 8 : ncols % 4 == 0 ? 4 : 2;
+  const uint32_t grid = rows * (ncols / G);
+  if (G == 8)
+    k_ntt_inv_gather<8, DIFF><<<grid, kNttThreads, 0, L->st>>>(L->digits, other, ncols, L->fd, s->fd_words, s->Tq);
+  else if (G == 4)
+    k_ntt_inv_gather<4, DIFF><<<grid, kNttThreads, 0, L->st>>>(L->digits, other, ncols, L->fd, s->fd_words, s->Tq);
+  else
+    k_ntt_inv_gather<2, DIFF><<<grid, kNttThreads, 0, L->st>>>(L->digits, other, ncols, L->fd, s->fd_words, s->Tq);
+}
+
+// keep_ntt: leave the outputs in out2 (NTT form, slot-major) for
+// run_mux_first instead of transforming them into L->fd.
 void run_first_dim_mfma(const onion_server *s, Lane *L, uint32_t nq, const uint64_t *sel, uint64_t sel_stride,
-                        bool time_core) {
+                        bool time_core, bool keep_ntt) {
   const uint32_t fst = s->P.fst_dim, other = s->P.other_dim, ncp = cpad_for(nq), ncols = 2 * nq;
   uint8_t *Bp = reinterpret_cast<uint8_t *>(L->Qs);
   k_sel_planes<<<dim3(fst / 8 * 2, nq), kNttThreads, 0, L->st>>>(sel, sel_stride, Bp, fst, ncp, s->Tq);
@@ -441,15 +461,15 @@ void run_first_dim_mfma(const onion_server *s, Lane *L, uint32_t nq, const uint6
   else
     launch_mfma<4>(s, L, Bp, grid, n_rt, ncols);
   if (time_core) (void)hipEventRecord(L->ev[7], L->st);
-  k_ntt_inv_gather<<<other * ncols, kNttThreads, 0, L->st>>>(L->digits, other, ncols, L->fd, s->fd_words, s->Tq);
+  if (!keep_ntt) launch_gather<false>(s, L, nq);
 }
 
 // First dimension: fst selection cts per query (coefficient form, stride
 // sel_stride) -> L->fd (other cts per query, coefficient form).
 void run_first_dim(const onion_server *s, Lane *L, uint32_t nq, const uint64_t *sel, uint64_t sel_stride,
-                   bool time_core) {
+                   bool time_core, bool keep_ntt = false) {
   if (use_mfma(s, L, nq)) {
-    run_first_dim_mfma(s, L, nq, sel, sel_stride, time_core);
+    run_first_dim_mfma(s, L, nq, sel, sel_stride, time_core, keep_ntt);
     return;
   }
   const uint32_t fst = s->P.fst_dim, other = s->P.other_dim, cpad = cpad_for(nq);
@@ -495,6 +515,21 @@ void run_mux_level(const onion_server *s, Lane *L, uint32_t nq, uint64_t n, cons
       L->digits, s->digit_words, g, s->rgsw_words, 2 * l, L->fd, s->fd_words, s->Tq, s->mq);
 }
 
+// The first multiplexer level takes the batched first dimension's NTT-form
+// outputs directly (k_ntt_inv_gather_diff + k_mux_first: four inverse
+// transforms per pair instead of six) when the matrix-core product runs and
+// the level is wide enough for the fused external product.
+bool mux_first_fused(const onion_server *s, const Lane *L, uint32_t nq) {
+  return s->I.ndim >= 2 && use_mfma(s, L, nq) && (uint64_t)(s->P.other_dim / 2) * nq >= s->tune.fuse_min;
+}
+
+void run_mux_first(const onion_server *s, Lane *L, uint32_t nq) {
+  const uint32_t other = s->P.other_dim, half = other / 2, ncols = 2 * nq;
+  launch_gather<true>(s, L, nq);
+  k_mux_first<<<half * nq, kNttThreads, 0, L->st>>>(L->digits, other, nq, ncols, L->fd, s->fd_words, L->rgsw,
+                                                    s->rgsw_words, s->P.gsw_l, s->I.base_log2, s->Tq, s->mq);
+}
+
 void run_mod_switch(const onion_server *s, Lane *L, uint32_t nq, uint64_t *cts, uint64_t stride) {
   k_mod_switch<<<dim3((2 * kN + 255) / 256, nq), 256, 0, L->st>>>(cts, stride, 2 * kN, s->ms_M, s->ms_f);
 }
@@ -510,10 +545,17 @@ void answer_device(const onion_server *s, Lane *L, uint32_t nq, bool timed) {
   if (timed) (void)hipEventRecord(L->ev[1], L->st);
   run_convert(s, L, nq, leaves + (size_t)fst * 2 * kN, s->tree_words);
   if (timed) (void)hipEventRecord(L->ev[2], L->st);
-  run_first_dim(s, L, nq, leaves, s->tree_words, timed);
+  const bool first_fused = mux_first_fused(s, L, nq);
+  run_first_dim(s, L, nq, leaves, s->tree_words, timed, first_fused);
   if (timed) (void)hipEventRecord(L->ev[3], L->st);
   uint64_t n = s->P.other_dim;
-  for (uint32_t d = 0; d + 1 < s->I.ndim; ++d) {
+  uint32_t d = 0;
+  if (first_fused) {
+    run_mux_first(s, L, nq);
+    n /= 2;
+    d = 1;
+  }
+  for (; d + 1 < s->I.ndim; ++d) {
     run_mux_level(s, L, nq, n, L->rgsw + (size_t)d * 2 * l * 2 * kN);
     n /= 2;
   }
@@ -656,8 +698,10 @@ int time_pipeline(onion_server *s, Lane *L, uint32_t n, uint32_t reps, double *m
 
 // Coordinate descent over the launch-shape choices on the live database with
 // a synthetic client (uniform key words and queries: the work does not depend
-// on the values).  Score = sum over the batch sizes of device ms per query.
-// The caller holds the gate exclusively.
+// on the values).  Score = sum over the batch sizes of the device time
+// relative to the starting choice's, so every size counts alike (in plain ms
+// per query a single query would outweigh a batch of 64 and decide choices
+// that only move the batch).  The caller holds the gate exclusively.
 int autotune_impl(onion_server *s, const uint32_t *sizes, uint32_t n_sizes, uint32_t reps, onion_tuning *out) {
   Lane *L = s->lanes[0].get();
   uint32_t maxn = 1;
@@ -695,6 +739,7 @@ int autotune_impl(onion_server *s, const uint32_t *sizes, uint32_t n_sizes, uint
     HIPCHK(hipMemcpy(L->gk_ptrs, L->h_gk.data(), maxn * sizeof(void *), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(L->rk_ptrs, L->h_rk.data(), maxn * sizeof(void *), hipMemcpyHostToDevice));
   }
+  std::vector<double> base(n_sizes, 0.0);
   auto score = [&](const onion_tuning &t, double *sc) -> int {
     s->tune = t;
     ++s->tune_epoch;
@@ -703,7 +748,8 @@ int autotune_impl(onion_server *s, const uint32_t *sizes, uint32_t n_sizes, uint
       double ms;
       int r = time_pipeline(s, L, sizes[i], reps, &ms);
       if (r) return r;
-      total += ms / sizes[i];
+      if (base[i] == 0) base[i] = std::max(ms, 1e-6);
+      total += ms / base[i];
     }
     *sc = total;
     return ONION_OK;
@@ -852,7 +898,12 @@ int onion_gpu_create(const onion_params *params, onion_server **out) {
     return fail(rc, msg);
   }
   s->pt_written.assign(I.num_pt, 0);
-  // the plane copy when it fits; without it the products convert in-kernel
+  // the plane copy when it takes at most a third of the free device memory
+  // (the rest stays for batch scratch and other databases); without it the
+  // products convert in-kernel
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && (uint64_t)free_b / 3 < planes_bytes(s))
+    s->tune.db_planes = 0;
   if ((rc = sync_planes(s)) == ONION_ENOMEM) {
     s->tune.db_planes = 0;
   } else if (rc) {

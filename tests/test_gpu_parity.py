@@ -232,6 +232,38 @@ def test_partial_upload_updates_the_plane_copy():
         assert np.array_equal(srv.tap_first_dim_batch(sel), got)
 
 
+@pytest.mark.parametrize("shape", [(64, 4), (256, 16)])
+def test_first_mux_level_from_ntt_form(shape):
+    """A batch whose first multiplexer level runs from the matrix-core
+    product's NTT-form outputs (fuse_min = 1) answers as the oracle does, and
+    as the path that transforms every first-dimension output first
+    (fuse_min = 2^30).  half = 2 and 8 take both workgroup-to-XCD mappings."""
+    from onionpirv2_amd import GpuServer
+    fst, other = shape
+    o = Oracle(fst=fst, other=other)
+    db = o.db_build(5, 0)
+    keys = []
+    for seed in (41, 42, 43):
+        sk = o.keygen(seed)
+        keys.append((seed, sk, o.galois_keys(seed, sk), o.rgsw_key(seed, sk)))
+    with GpuServer(fst, other) as srv:
+        srv.db_generate(5, 0)
+        for cid, (seed, sk, gk, rk) in enumerate(keys):
+            srv.set_galois_key(cid, gk)
+            srv.set_rgsw_key(cid, rk)
+        idx = [3, o.num_pt - 1, (o.num_pt * 2) // 3]
+        qs = np.stack([o.query(keys[i][0], 100 + i, keys[i][1], idx[i]) for i in range(3)])
+        srv.set_tuning(fd_mfma_min=2, fuse_min=1)
+        fused = srv.answer_batch([0, 1, 2], qs)
+        srv.set_tuning(fuse_min=1 << 30)
+        assert np.array_equal(srv.answer_batch([0, 1, 2], qs), fused)
+        for i in (0, 2):
+            seed, sk, gk, rk = keys[i]
+            exp, _ = o.answer(db, gk, o.rgsw_key_ntt(rk), qs[i])
+            assert np.array_equal(fused[i], exp), f"query {i}"
+            assert np.array_equal(o.decrypt_small(sk, fused[i]), o.db_plaintext(5, 0, idx[i]))
+
+
 @pytest.mark.parametrize("nq", [2, 3, 16, 17, 33, 64])
 def test_first_dim_batch_matches_oracle(cfg1, nq):
     """The batched first-dimension kernels (int8 matrix cores by default, the
