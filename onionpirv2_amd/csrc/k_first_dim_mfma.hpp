@@ -33,7 +33,7 @@
 // within the 32-block), where tile = slot * n_blocks + block.  A wave reads
 // digit i of a step as one contiguous 1 KiB load that lands in its operand
 // registers as it stands (8 cache lines per load rather than 64).
-//   B planes  written by k_sel_planes; 32-column blocks of col = 2*query +
+//   B planes  written by k_sel_chunks; 32-column blocks of col = 2*query +
 //             poly, padded to NCP columns (NCP = 32, 64 or 128)
 //   A planes  the database's digit-plane copy (k_db_planes), 32-row blocks,
 //             rows padded to a multiple of 32
@@ -98,45 +98,63 @@ __device__ __forceinline__ void a_planes(const uint4 *a, uint32_t (&out)[16]) {
   planes8(w, out);
 }
 
-// Selection vectors as digit planes: block (j-block jb of 8 leaves, poly c)
-// of query b NTTs its 8 polynomials and writes, for each slot, the half
-// (bytes 8 (jb % 2) .. +7) of the 8 chunks (step jb / 4, lane group
-// (jb / 2) % 2) of column 2b + c.  grid = (fst/8 * 2, nq); fst % 32 == 0.
-// The 64 NTT outputs of a thread stay in registers (two waves per SIMD let
-// the compiler use more than 128 VGPRs).
-template <int J>
-__device__ __forceinline__ void sel_leaves(uint64_t (&xs)[8][8], const uint64_t *src, uint64_t *lds,
-                                           const NttTab &T) {
-  // leaf J, then the following ones: a recursion on a constant J, so xs is
-  // indexed by constants (a loop here is not unrolled and puts xs in scratch)
+// Selection vectors as B chunks, in two launches so that both write
+// contiguous lines (a single pass owning 8 leaves of one column writes 8-byte
+// pieces 1 KiB apart, and measured 4.3x its bytes in write traffic).
+//   k_sel_digits  NTT of each leaf polynomial, in place, in signed-digit
+//                 form (slot order).  grid = (fst * 2, nq).
+//   k_sel_chunks  8 slots x one 32-column block x one step per workgroup:
+//                 reads each (column, j)'s 8 slots (64 contiguous bytes) into
+//                 LDS, then each thread turns 16 values of j of one column and
+//                 slot into that lane's 8 chunks; a wave stores one digit of
+//                 one slot as 1 KiB.  grid = (N / 8, NCP / 32, fst / 32).
+// Leaf j, poly c of query b is at leaves + b * leaf_stride + (2j + c) * N.
+__global__ __launch_bounds__(kNttThreads) void k_sel_digits(uint64_t *leaves, uint64_t leaf_stride, NttTab T) {
+  __shared__ uint64_t lds[kNttLds];
+  uint64_t *p = leaves + (size_t)blockIdx.y * leaf_stride + (size_t)blockIdx.x * kN;
   uint64_t x[8];
-  load_natural(x, src + (size_t)J * 2 * kN);
-  ntt_fwd(x, lds, T);
+  load_natural(x, p);
+  ntt_fwd(x, lds, T);  // every thread has read its inputs before the first barrier inside
 #pragma unroll
-  for (int r = 0; r < 8; ++r) xs[J][r] = signed_digits(ntt_fwd_final(x[r], T.q));
-  if constexpr (J + 1 < 8) sel_leaves<J + 1>(xs, src, lds, T);
+  for (int r = 0; r < 8; ++r) x[r] = signed_digits(ntt_fwd_final(x[r], T.q));
+  store_blocked(p, x);
 }
 
-__global__ __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_sel_planes(
-    const uint64_t *leaves, uint64_t leaf_stride, uint8_t *Bp, uint32_t fst, uint32_t ncp, NttTab T) {
-  __shared__ uint64_t lds[kNttLds];
-  const uint32_t jb = blockIdx.x / 2, c = blockIdx.x % 2, b = blockIdx.y;
-  uint64_t xs[8][8];
-  sel_leaves<0>(xs, leaves + (size_t)b * leaf_stride + (size_t)jb * 8 * 2 * kN + (size_t)c * kN, lds, T);
-  const uint32_t col = 2 * b + c, steps = fst / 32, n_cb = ncp / 32;
-  const uint32_t l = 32 * ((jb / 2) % 2) + col % 32;
+__global__ __launch_bounds__(256) void k_sel_chunks(const uint64_t *__restrict__ leaves, uint64_t leaf_stride,
+                                                    uint8_t *__restrict__ Bp, uint32_t fst, uint32_t ncols) {
+  __shared__ uint64_t buf[8][32][33];  // [slot][column][j], rows padded against bank conflicts
+  const uint32_t slot0 = 8 * blockIdx.x, cb = blockIdx.y, s = blockIdx.z, ncb = gridDim.y;
 #pragma unroll
-  for (int r = 0; r < 8; ++r) {
-    const uint32_t slot = 8 * threadIdx.x + r;
-    uint64_t w[8];
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t piece = threadIdx.x + 256 * k, li = piece / 32, jj = piece % 32, col = 32 * cb + li;
+    if (col < ncols) {  // padded columns stay unread; their outputs are never stored
+      const uint4 *src = reinterpret_cast<const uint4 *>(leaves + (size_t)(col / 2) * leaf_stride +
+                                                         (size_t)(2 * (32 * s + jj) + col % 2) * kN + slot0);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) w[j] = xs[j][r];
-    uint32_t o[16];
-    planes8(w, o);
-    uint8_t *dst = Bp + chunk_offset((size_t)slot * n_cb + col / 32, steps, jb / 4, 0, l) + 8 * (jb % 2);
+      for (int q = 0; q < 4; ++q) {
+        const uint4 v = src[q];
+        buf[2 * q][li][jj] = ((uint64_t)v.y << 32) | v.x;
+        buf[2 * q + 1][li][jj] = ((uint64_t)v.w << 32) | v.z;
+      }
+    }
+  }
+  __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      *reinterpret_cast<uint2 *>(dst + (size_t)k * 1024) = make_uint2(o[2 * k], o[2 * k + 1]);
+  for (int k = 0; k < 2; ++k) {
+    const uint32_t pair = threadIdx.x + 256 * k, sg = pair / 64, l = pair % 64, li = l % 32, g = l / 32;
+    uint64_t w0[8], w1[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      w0[t] = buf[sg][li][16 * g + t];
+      w1[t] = buf[sg][li][16 * g + 8 + t];
+    }
+    uint32_t p0[16], p1[16];
+    planes8(w0, p0);
+    planes8(w1, p1);
+    uint8_t *dst = Bp + chunk_offset((size_t)(slot0 + sg) * ncb + cb, fst / 32, s, 0, l);
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      *reinterpret_cast<uint4 *>(dst + (size_t)i * 1024) = make_uint4(p0[2 * i], p0[2 * i + 1], p1[2 * i], p1[2 * i + 1]);
   }
 }
 

@@ -449,7 +449,10 @@ void run_first_dim_mfma(const onion_server *s, Lane *L, uint32_t nq, const uint6
                         bool time_core, bool keep_ntt) {
   const uint32_t fst = s->P.fst_dim, other = s->P.other_dim, ncp = cpad_for(nq), ncols = 2 * nq;
   uint8_t *Bp = reinterpret_cast<uint8_t *>(L->Qs);
-  k_sel_planes<<<dim3(fst / 8 * 2, nq), kNttThreads, 0, L->st>>>(sel, sel_stride, Bp, fst, ncp, s->Tq);
+  // the leaves are transformed in place: the selection vectors have no other reader
+  uint64_t *leaves = const_cast<uint64_t *>(sel);
+  k_sel_digits<<<dim3(fst * 2, nq), kNttThreads, 0, L->st>>>(leaves, sel_stride, s->Tq);
+  k_sel_chunks<<<dim3(kN / 8, ncp / 32, fst / 32), 256, 0, L->st>>>(leaves, sel_stride, Bp, fst, ncols);
   if (time_core) (void)hipEventRecord(L->ev[6], L->st);
   const uint32_t rb = 128 / ncp;  // row blocks of 32 per workgroup
   const uint32_t n_rt = (other + 32 * rb - 1) / (32 * rb);
