@@ -139,7 +139,7 @@ def main() -> int:
     ids = np.arange(B, dtype=np.uint64)
     log(f"[bench] {B} clients keyed, queries built in {time.time() - t0:.2f} s")
 
-    tuning = srv.autotune([1, B], reps=2) if args.autotune else srv.get_tuning()
+    tuning = srv.autotune([1, B], reps=3) if args.autotune else srv.get_tuning()
     log(f"[bench] tuning {tuning}")
 
     # ---- single-query path ---------------------------------------------------------

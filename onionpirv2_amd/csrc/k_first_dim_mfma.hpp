@@ -557,7 +557,8 @@ __global__ __launch_bounds__(256) void k_first_dim_mfma_lds(const uint64_t *__re
 // level): row r < half = other / 2 pairs with row r + half, and the
 // transformed value is y - x = INTT(NTT(y) - NTT(x)), the reference's
 // coefficient-form difference mod q (INTT is linear), written over row
-// r + half.  Rows are dealt to XCDs (r = xcd mod 8, column groups fastest)
+// r + half; NTT(x) itself goes to row r in slot order, where the level's
+// k_mux_first reads it with contiguous loads.  Rows are dealt to XCDs (r = xcd mod 8, column groups fastest)
 // when the row count is a multiple of 8, so the workgroups reading the same
 // cache lines of out2 share an L2.
 // Column K of the group, then the following ones (a recursion on a constant
@@ -593,19 +594,30 @@ __global__ __launch_bounds__(kNttThreads) void k_ntt_inv_gather(const uint64_t *
   const uint64_t *sy = sx + (size_t)(other / 2) * ncols;
   uint64_t x[G][8];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const size_t o = (size_t)(8 * threadIdx.x + j) * sstride;
+  for (int k = 0; k < G; k += 2) {  // column pairs: one 16-byte load per slot
+    uint64_t a[2][8];
 #pragma unroll
-    for (int k = 0; k < G; k += 2) {
-      const uint4 a = *reinterpret_cast<const uint4 *>(sx + o + k);
-      const uint64_t a0 = ((uint64_t)a.y << 32) | a.x, a1 = ((uint64_t)a.w << 32) | a.z;
+    for (int j = 0; j < 8; ++j) {
+      const size_t o = (size_t)(8 * threadIdx.x + j) * sstride;
+      const uint4 v = *reinterpret_cast<const uint4 *>(sx + o + k);
+      a[0][j] = ((uint64_t)v.y << 32) | v.x;
+      a[1][j] = ((uint64_t)v.w << 32) | v.z;
       if (DIFF) {
-        const uint4 b = *reinterpret_cast<const uint4 *>(sy + o + k);
-        x[k][j] = sub_mod(((uint64_t)b.y << 32) | b.x, a0, T.q);
-        x[k + 1][j] = sub_mod(((uint64_t)b.w << 32) | b.z, a1, T.q);
+        const uint4 w = *reinterpret_cast<const uint4 *>(sy + o + k);
+        x[k][j] = sub_mod(((uint64_t)w.y << 32) | w.x, a[0][j], T.q);
+        x[k + 1][j] = sub_mod(((uint64_t)w.w << 32) | w.z, a[1][j], T.q);
       } else {
-        x[k][j] = a0;
-        x[k + 1][j] = a1;
+        x[k][j] = a[0][j];
+        x[k + 1][j] = a[1][j];
+      }
+    }
+    if (DIFF) {
+      // NTT(x) itself, slot order, into row r for the level's last transform:
+      // the thread's 8 slots are 64 contiguous bytes of each column
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const uint32_t c = cg * G + k + h;
+        store_blocked(fd + (size_t)(c >> 1) * fd_stride + ((size_t)r * 2 + (c & 1)) * kN, a[h]);
       }
     }
   }

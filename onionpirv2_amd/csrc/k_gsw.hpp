@@ -190,47 +190,33 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused(uint64_t *cts, uint64
   }
 }
 
-// First multiplexer level, fused, straight from the batched first dimension's
-// NTT-form outputs out2[slot][row][c] (c = 2 query + poly): for pair r of
-// query b, with v = y - x already in coefficient form at row r + half of the
-// query's cts (k_ntt_inv_gather_diff),
+// First multiplexer level, fused, for the batched first dimension's
+// NTT-form outputs: for pair r of query b (grid (half, nq)), with
+// k_ntt_inv_gather<DIFF> having left v = y - x in coefficient form at row
+// r + half and NTT(x) in slot order at row r of the query's cts,
 //   x_r = INTT(NTT(x) + ext(RGSW, v)),
 // which equals the reference's x + INTT(ext(RGSW, y - x)) mod q (INTT is
-// linear), so x's own inverse transform is folded into this one.  One
-// workgroup per (r, b), dealt to XCDs (r = xcd mod 8) when half % 8 == 0, so
-// the workgroups reading the same cache lines of out2 share an L2.
-__global__ __launch_bounds__(kNttThreads) void k_mux_first(const uint64_t *__restrict__ out2, uint32_t other,
-                                                           uint32_t nq, uint32_t ncols, uint64_t *cts,
-                                                           uint64_t cts_stride, const uint64_t *G,
-                                                           uint64_t g_stride, uint32_t l, uint32_t base_log2,
-                                                           NttTab T, Mod mq) {
+// linear), so x's own inverse transform is folded into this one.
+__global__ __launch_bounds__(kNttThreads) void k_mux_first(uint64_t *cts, uint64_t cts_stride, uint64_t half,
+                                                           const uint64_t *G, uint64_t g_stride, uint32_t l,
+                                                           uint32_t base_log2, NttTab T, Mod mq) {
   __shared__ uint64_t lds[kNttLds];
   __shared__ uint64_t park[kN];  // component 1 waits here during component 0's INTT
-  const uint32_t half = other / 2;
-  uint32_t r, b;
-  if (half % 8 == 0) {
-    const uint32_t xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
-    r = 8 * (k / nq) + xcd;
-    b = k % nq;
-  } else {
-    r = blockIdx.x / nq;
-    b = blockIdx.x % nq;
-  }
+  const uint32_t r = blockIdx.x, b = blockIdx.y;
   uint64_t *x = cts + (size_t)b * cts_stride + (size_t)r * 2 * kN;
   const uint64_t *v = x + (size_t)half * 2 * kN;
   uint64_t res[2][8];
   ext_fused_core(v, nullptr, G + (size_t)b * g_stride, l, base_log2, lds, T, mq, res);
 #pragma unroll
   for (int j = 0; j < 8; ++j) park[8 * threadIdx.x + j] = res[1][j];
-  const size_t sstride = (size_t)other * ncols;
 #pragma unroll 1
   for (uint32_t c = 0; c < 2; ++c) {
-    const uint64_t *nx = out2 + (size_t)r * ncols + 2 * b + c;  // NTT(x), slot-major
     uint64_t w[8];
+    load_blocked(w, x + (size_t)c * kN);  // NTT(x), slot order
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const uint64_t e = c == 0 ? res[0][j] : park[8 * threadIdx.x + j];
-      w[j] = add_mod(e, nx[(size_t)(8 * threadIdx.x + j) * sstride], T.q);
+      w[j] = add_mod(e, w[j], T.q);
     }
     ntt_inv(w, lds, T);
 #pragma unroll
