@@ -67,6 +67,22 @@ __device__ __forceinline__ void add128(uint64_t &lo, uint64_t &hi, uint64_t blo,
   hi += bhi + (lo < blo);
 }
 
+// (lo, hi) += a * b for a, b < 2^62 (both high words below 2^30), as three
+// 32x32 partial products: the middle pair sums below 2^63 without a carry,
+// the low product carries into hi once.
+__device__ __forceinline__ void mac128(uint64_t &lo, uint64_t &hi, uint64_t a, uint64_t b) {
+  const uint32_t al = (uint32_t)a, ah = (uint32_t)(a >> 32), bl = (uint32_t)b, bh = (uint32_t)(b >> 32);
+  const uint64_t ll = (uint64_t)al * bl;
+  const uint64_t mid = (uint64_t)al * bh + (uint64_t)ah * bl;
+  const uint64_t hh = (uint64_t)ah * bh;
+  const uint64_t t = lo + ll;
+  const uint64_t c1 = t < ll;
+  const uint64_t u = t + (mid << 32);
+  const uint64_t c2 = u < t;
+  lo = u;
+  hi += hh + (mid >> 32) + c1 + c2;
+}
+
 __device__ __forceinline__ uint64_t mulmod_barrett(uint64_t a, uint64_t b, const Mod &m) {
   uint64_t lo, hi;
   mul128(a, b, lo, hi);

@@ -45,7 +45,7 @@ __global__ __launch_bounds__(kNttThreads) void k_decomp_ntt(const uint64_t *src_
     if (b) v = sub_mod(b[e], v, T.q);
     x[r] = (v >> sh) & mask;
   }
-  ntt_fwd(x, lds, T);
+  ntt_fwd<true>(x, lds, T);  // digits < 2^base_log2 <= 2^32
 #pragma unroll
   for (int r = 0; r < 8; ++r) x[r] = ntt_fwd_final(x[r], T.q);
   store_blocked(D + (size_t)blockIdx.y * d_stride + ((size_t)ct * rows + j) * kN, x);
@@ -62,11 +62,7 @@ __device__ __forceinline__ void ext_mac8(const uint64_t *Dc, const uint64_t *G, 
     load_blocked(d, Dc + (size_t)j * kN);
     load_blocked(g, G + (size_t)j * 2 * kN + (size_t)comp * kN);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      uint64_t plo, phi;
-      mul128(d[r], g[r], plo, phi);
-      add128(lo[r], hi[r], plo, phi);
-    }
+    for (int r = 0; r < 8; ++r) mac128(lo[r], hi[r], d[r], g[r]);
   }
 #pragma unroll
   for (int r = 0; r < 8; ++r) out[r] = barrett128(lo[r], hi[r], mq);
@@ -140,7 +136,7 @@ __device__ __forceinline__ void ext_fused_core(const uint64_t *xs, const uint64_
       uint64_t d[8];
 #pragma unroll
       for (int r = 0; r < 8; ++r) d[r] = (v[r] >> sh) & mask;
-      ntt_fwd(d, lds, T);
+      ntt_fwd<true>(d, lds, T);  // digits < 2^base_log2 <= 2^32
       const uint64_t *g = G + (size_t)(poly * l + kk) * 2 * kN;
 #pragma unroll
       for (int r = 0; r < 8; ++r) d[r] = ntt_fwd_final(d[r], T.q);
@@ -149,11 +145,7 @@ __device__ __forceinline__ void ext_fused_core(const uint64_t *xs, const uint64_
         uint64_t gv[8];
         load_blocked(gv, g + (size_t)c * kN);
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
-          uint64_t plo, phi;
-          mul128(d[r], gv[r], plo, phi);
-          add128(lo[c][r], hi[c][r], plo, phi);
-        }
+        for (int r = 0; r < 8; ++r) mac128(lo[c][r], hi[c][r], d[r], gv[r]);
       }
     }
   }

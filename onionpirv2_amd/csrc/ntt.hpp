@@ -39,6 +39,20 @@ __device__ __forceinline__ void ct_bfly(uint64_t &x, uint64_t &y, uint64_t w, ui
   y = u + two_q - v;
 }
 
+// The first forward stage for inputs below 2^32 (digit polynomials): u needs
+// no range correction and the Shoup quotient floor(y ws / 2^64) needs only the
+// two products of y with the halves of ws (exact: y ws_hi + floor(y ws_lo / 2^32)
+// < 2^64), so the result is the same as ct_bfly's with fewer multiplies.
+__device__ __forceinline__ void ct_bfly_small(uint64_t &x, uint64_t &y, uint64_t w, uint64_t ws, uint64_t q,
+                                              uint64_t two_q) {
+  const uint64_t u = x;
+  const uint32_t y32 = (uint32_t)y;
+  const uint64_t t = (uint64_t)y32 * (ws >> 32) + (((uint64_t)y32 * (uint32_t)ws) >> 32);
+  const uint64_t v = (uint64_t)y32 * w - (t >> 32) * q;
+  x = u + v;
+  y = u + two_q - v;
+}
+
 __device__ __forceinline__ void gs_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t ws,
                                         uint64_t q, uint64_t two_q) {
   const uint64_t u = x, v = y;
@@ -79,12 +93,19 @@ __device__ __forceinline__ void load_tw8(Tw8 &t, const uint64_t *tab, const uint
   }
 }
 
-// forward radix-8 (three CT stages); `first` = false skips the span-H stage
+// forward radix-8 (three CT stages); `first` = false skips the span-H stage;
+// SMALL: every input is below 2^32 (the span-H stage uses ct_bfly_small)
+template <bool SMALL = false>
 __device__ __forceinline__ void fwd8(uint64_t (&x)[8], const Tw8 &t, uint64_t q, bool first) {
   const uint64_t two_q = 2 * q;
   if (first) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) ct_bfly(x[j], x[j + 4], t.w[0], t.s[0], q, two_q);
+    for (int j = 0; j < 4; ++j) {
+      if (SMALL)
+        ct_bfly_small(x[j], x[j + 4], t.w[0], t.s[0], q, two_q);
+      else
+        ct_bfly(x[j], x[j + 4], t.w[0], t.s[0], q, two_q);
+    }
   }
 #pragma unroll
   for (int j = 0; j < 8; j = (j & 1) ? j + 3 : j + 1)  // j = 0, 1, 4, 5
@@ -121,16 +142,18 @@ __device__ __forceinline__ void lds_get(const uint64_t *lds, uint64_t (&x)[8], i
   for (int j = 0; j < 8; ++j) x[j] = lds[lds_pad(b + j * stride)];
 }
 
-// Forward NTT.  Entry: natural layout, values < 4q.  Exit: blocked layout,
-// values in [0, 4q).  Uses lds[kNttLds]; begins with a barrier so the buffer
-// may have been read just before the call.
+// Forward NTT.  Entry: natural layout, values < 4q (SMALL: < 2^32, as digit
+// polynomials are).  Exit: blocked layout, values in [0, 4q).  Uses
+// lds[kNttLds]; begins with a barrier so the buffer may have been read just
+// before the call.
+template <bool SMALL = false>
 __device__ __forceinline__ void ntt_fwd(uint64_t (&x)[8], uint64_t *lds, const NttTab &T) {
   const int t = threadIdx.x;
   const uint64_t q = T.q;
   Tw8 tw;
   // pass A: spans 1024, 512, 256 (stride 256, b = t): twiddle root m = 1
   load_tw8(tw, T.rp, T.rps, 1);
-  fwd8(x, tw, q, true);
+  fwd8<SMALL>(x, tw, q, true);
   // pass B: spans 128, 64, 32 (stride 32): m = 8 + t/32
   const int bB = pass_base(t, 5);
   load_tw8(tw, T.rp, T.rps, 8 + (t >> 5));
