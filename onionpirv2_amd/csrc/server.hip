@@ -344,8 +344,8 @@ int check_db(const onion_server *s) {
 // Levels with at least tune.fuse_min nodes across the batch run fused.
 void expand_level(const onion_server *s, Lane *L, uint32_t nq, uint32_t a, uint32_t n_active) {
   if ((uint64_t)n_active * nq >= s->tune.fuse_min) {
-    k_expand_fused<<<dim3(n_active * 2, nq), kNttThreads, 0, L->st>>>(L->trees, s->tree_words, L->gk_ptrs, a,
-                                                                      s->ginv[a], s->Tq, s->Tp, s->mq, s->mp, s->ks);
+    k_expand_node<<<dim3(n_active, nq), kNttThreads, 0, L->st>>>(L->trees, s->tree_words, L->gk_ptrs, a,
+                                                                 s->ginv[a], s->Tq, s->Tp, s->mq, s->mp, s->ks);
     return;
   }
   k_expand_ks<<<dim3(n_active * 4, nq), kNttThreads, 0, L->st>>>(
