@@ -39,8 +39,11 @@
 //             rows padded to a multiple of 32
 //   A words   without the copy each lane reads the 16 database words of its
 //             row and converts them in registers
-//   output    out2[l][r][c] (uint64, canonical mod q, c < 2*nq), turned into
-//             coefficient-form ciphertexts by k_ntt_inv_gather.
+//   output    out2[r][l][c] (uint64, canonical mod q, c < 2*nq): a tile's
+//             32 columns of one (row, slot) are 256 contiguous bytes, and one
+//             row's 2048 slots are one contiguous 2 MiB run (2 KiB per slot at
+//             128 columns) for k_ntt_inv_gather, which transforms them into
+//             coefficient-form ciphertexts.
 #pragma once
 
 #include "k_first_dim.hpp"
@@ -293,14 +296,14 @@ struct Tile {
                                         const Mod &mq) const {
     const uint32_t col = cb * 32 + li;
     if (col >= ncols) return;
-    uint64_t *dst = out2 + (size_t)slot * other * ncols + col;
+    uint64_t *dst = out2 + (size_t)slot * ncols + col;
     const DigitWeights W;
     // output (m, n) of register e: n = lane % 32, m = 8 (e / 4) + 4 g + e % 4
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       const uint32_t m = row0 + 8 * (e / 4) + 4 * g + (e % 4);
       const uint64_t v = diagonals_mod_q(acc, e, mq, W);
-      if (m < other) dst[(size_t)m * ncols] = v;
+      if (m < other) dst[(size_t)m * kN * ncols] = v;
     }
   }
 };
@@ -548,19 +551,18 @@ __global__ __launch_bounds__(256) void k_first_dim_mfma_lds(const uint64_t *__re
   T.store(acc, out2, other, ncols, mq);
 }
 
-// Inverse NTTs of the first-dimension outputs gathered from out2[slot][r][c]
+// Inverse NTTs of the first-dimension outputs gathered from out2[r][slot][c]
 // into fd (query b = c / 2, poly c % 2 of ciphertext r, coefficient form).
 // One workgroup per (row, group of G adjacent columns): a thread reads the G
-// columns of each of its 8 slots as one contiguous 8G-byte piece (the
-// slot-major out2 puts 1 MiB between slots, so the run length is what counts)
-// and runs the G transforms one after another.  DIFF (first multiplexer
-// level): row r < half = other / 2 pairs with row r + half, and the
-// transformed value is y - x = INTT(NTT(y) - NTT(x)), the reference's
-// coefficient-form difference mod q (INTT is linear), written over row
-// r + half; NTT(x) itself goes to row r in slot order, where the level's
-// k_mux_first reads it with contiguous loads.  Rows are dealt to XCDs (r = xcd mod 8, column groups fastest)
-// when the row count is a multiple of 8, so the workgroups reading the same
-// cache lines of out2 share an L2.
+// columns of each of its 8 slots as one contiguous 8G-byte piece (slots are
+// 2 KiB apart at 128 columns) and runs the G transforms one after another.
+// DIFF (first multiplexer level): row r < half = other / 2 pairs with row
+// r + half, and the transformed value is y - x = INTT(NTT(y) - NTT(x)), the
+// reference's coefficient-form difference mod q (INTT is linear), written over
+// row r + half; NTT(x) itself goes to row r in slot order, where the level's
+// k_mux_first reads it with contiguous loads.  Rows are dealt to XCDs
+// (r = xcd mod 8, column groups fastest) when the row count is a multiple of
+// 8, so the workgroups reading one row's 2 MiB share an L2.
 // Column K of the group, then the following ones (a recursion on a constant
 // K keeps x in registers; a loop around the transform is not unrolled).
 template <int G, int K = 0>
@@ -589,9 +591,9 @@ __global__ __launch_bounds__(kNttThreads) void k_ntt_inv_gather(const uint64_t *
     r = blockIdx.x / ngroups;
     cg = blockIdx.x % ngroups;
   }
-  const size_t sstride = (size_t)other * ncols;
-  const uint64_t *sx = out2 + (size_t)r * ncols + cg * G;
-  const uint64_t *sy = sx + (size_t)(other / 2) * ncols;
+  const size_t sstride = ncols;
+  const uint64_t *sx = out2 + (size_t)r * kN * ncols + cg * G;
+  const uint64_t *sy = sx + (size_t)(other / 2) * kN * ncols;
   uint64_t x[G][8];
 #pragma unroll
   for (int k = 0; k < G; k += 2) {  // column pairs: one 16-byte load per slot

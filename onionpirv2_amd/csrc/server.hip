@@ -401,7 +401,7 @@ void launch_fd_bat(const onion_server *s, Lane *L, uint32_t nq) {
 
 // The int8-MFMA product serves batches of at least tune.fd_mfma_min queries
 // when fst is a multiple of 32 and the lane's digit scratch holds the
-// slot-major outputs (N x other x 2nq words).
+// first-dimension outputs (other x N x 2nq words).
 bool use_mfma(const onion_server *s, const Lane *L, uint32_t nq) {
   return nq >= 2 && s->tune.fd_mfma_min != 0 && nq >= s->tune.fd_mfma_min && s->P.fst_dim % 32 == 0 &&
          (uint64_t)L->cap * s->digit_words >= (uint64_t)kN * s->P.other_dim * 2 * nq;
@@ -427,7 +427,7 @@ void launch_mfma(const onion_server *s, Lane *L, const uint8_t *Bp, dim3 grid, u
   }
 }
 
-// Inverse transforms of the slot-major outputs out2 into L->fd: all rows, or
+// Inverse transforms of the outputs out2 into L->fd: all rows, or
 // (DIFF) the pair differences of the first multiplexer level.  Column groups
 // of 8, 4 or 2 (ncols = 2 nq is even).
 template <bool DIFF>
@@ -443,7 +443,7 @@ void launch_gather(const onion_server *s, Lane *L, uint32_t nq) {
     k_ntt_inv_gather<2, DIFF><<<grid, kNttThreads, 0, L->st>>>(L->digits, other, ncols, L->fd, s->fd_words, s->Tq);
 }
 
-// keep_ntt: leave the outputs in out2 (NTT form, slot-major) for
+// keep_ntt: leave the outputs in out2 (NTT form) for
 // run_mux_first instead of transforming them into L->fd.
 void run_first_dim_mfma(const onion_server *s, Lane *L, uint32_t nq, const uint64_t *sel, uint64_t sel_stride,
                         bool time_core, bool keep_ntt) {
