@@ -1,9 +1,13 @@
 // ntt.hpp -- negacyclic NTT / INTT of one 2048-point polynomial, computed by
 // one workgroup of 256 threads with 8 coefficients per thread in registers.
 //
-// Forward: Cooley-Tukey with bit-reversed twiddles psi^bitrev(k), Harvey lazy
-// butterflies (values in [0, 4q)); output slot e = a(psi^(2*bitrev(e)+1)),
-// the order SEAL's ntt_negacyclic_harvey produces.  Inverse: Gentleman-Sande
+// Forward: Cooley-Tukey with bit-reversed twiddles psi^bitrev(k) and lazy
+// butterflies: a Shoup product is in [0, 2q) for any 64-bit input, so each
+// stage grows the bound on the values by 2q, and with q < 2^60 a value may
+// grow to 16q before it must be brought down.  The input half of a butterfly
+// is reduced by 8q only in stages 3 and 7 (of 0..10), so the values stay
+// below 16q throughout (ntt_fwd_final brings them to [0, q)).  Output slot
+// e = a(psi^(2*bitrev(e)+1)), the order SEAL's ntt_negacyclic_harvey produces.  Inverse: Gentleman-Sande
 // with psi^-bitrev(k), values in [0, 2q); the N^-1 scale is applied by the
 // caller as it reads the result out (ntt_inv_final).
 //
@@ -30,10 +34,13 @@ constexpr int kNttLds = 2048 + 2048 / 16;  // padded LDS words per transform
 
 __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
 
-__device__ __forceinline__ void ct_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t ws,
-                                        uint64_t q, uint64_t two_q) {
+// Forward butterfly on values below 16q; CORRECT first reduces x from
+// [0, 16q) to [0, 8q).  Outputs are below (bound of u) + 2q.
+template <bool CORRECT>
+__device__ __forceinline__ void ct_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t ws, uint64_t q,
+                                        uint64_t two_q) {
   uint64_t u = x;
-  if (u >= two_q) u -= two_q;
+  if (CORRECT && u >= 4 * two_q) u -= 4 * two_q;
   const uint64_t v = mul_shoup_lazy(y, w, ws, q);
   x = u + v;
   y = u + two_q - v;
@@ -94,8 +101,9 @@ __device__ __forceinline__ void load_tw8(Tw8 &t, const uint64_t *tab, const uint
 }
 
 // forward radix-8 (three CT stages); `first` = false skips the span-H stage;
-// SMALL: every input is below 2^32 (the span-H stage uses ct_bfly_small)
-template <bool SMALL = false>
+// SMALL: every input is below 2^32 (the span-H stage uses ct_bfly_small);
+// bit s of CMASK reduces the inputs of stage s (0 = span H, 1 = H/2, 2 = H/4)
+template <int CMASK, bool SMALL = false>
 __device__ __forceinline__ void fwd8(uint64_t (&x)[8], const Tw8 &t, uint64_t q, bool first) {
   const uint64_t two_q = 2 * q;
   if (first) {
@@ -104,14 +112,15 @@ __device__ __forceinline__ void fwd8(uint64_t (&x)[8], const Tw8 &t, uint64_t q,
       if (SMALL)
         ct_bfly_small(x[j], x[j + 4], t.w[0], t.s[0], q, two_q);
       else
-        ct_bfly(x[j], x[j + 4], t.w[0], t.s[0], q, two_q);
+        ct_bfly<(CMASK & 1) != 0>(x[j], x[j + 4], t.w[0], t.s[0], q, two_q);
     }
   }
 #pragma unroll
   for (int j = 0; j < 8; j = (j & 1) ? j + 3 : j + 1)  // j = 0, 1, 4, 5
-    ct_bfly(x[j], x[j + 2], t.w[1 + (j >> 2)], t.s[1 + (j >> 2)], q, two_q);
+    ct_bfly<(CMASK & 2) != 0>(x[j], x[j + 2], t.w[1 + (j >> 2)], t.s[1 + (j >> 2)], q, two_q);
 #pragma unroll
-  for (int j = 0; j < 8; j += 2) ct_bfly(x[j], x[j + 1], t.w[3 + (j >> 1)], t.s[3 + (j >> 1)], q, two_q);
+  for (int j = 0; j < 8; j += 2)
+    ct_bfly<(CMASK & 4) != 0>(x[j], x[j + 1], t.w[3 + (j >> 1)], t.s[3 + (j >> 1)], q, two_q);
 }
 
 // inverse radix-8 (three GS stages, reverse order); `last` = false skips the span-H stage
@@ -143,7 +152,10 @@ __device__ __forceinline__ void lds_get(const uint64_t *lds, uint64_t (&x)[8], i
 }
 
 // Forward NTT.  Entry: natural layout, values < 4q (SMALL: < 2^32, as digit
-// polynomials are).  Exit: blocked layout, values in [0, 4q).  Uses
+// polynomials are).  Exit: blocked layout, values in [0, 16q) (q < 2^60).
+// Bounds by stage: 6q 8q 10q | 10q 12q 14q | 16q 10q 12q | 14q 16q, the
+// reductions being in stage 3 (pass B, span 128) and stage 7 (pass C,
+// span 8).  Uses
 // lds[kNttLds]; begins with a barrier so the buffer may have been read just
 // before the call.
 template <bool SMALL = false>
@@ -153,7 +165,7 @@ __device__ __forceinline__ void ntt_fwd(uint64_t (&x)[8], uint64_t *lds, const N
   Tw8 tw;
   // pass A: spans 1024, 512, 256 (stride 256, b = t): twiddle root m = 1
   load_tw8(tw, T.rp, T.rps, 1);
-  fwd8<SMALL>(x, tw, q, true);
+  fwd8<0, SMALL>(x, tw, q, true);
   // pass B: spans 128, 64, 32 (stride 32): m = 8 + t/32
   const int bB = pass_base(t, 5);
   load_tw8(tw, T.rp, T.rps, 8 + (t >> 5));
@@ -161,7 +173,7 @@ __device__ __forceinline__ void ntt_fwd(uint64_t (&x)[8], uint64_t *lds, const N
   lds_put(lds, x, t, 256);
   __syncthreads();
   lds_get(lds, x, bB, 32);
-  fwd8(x, tw, q, true);
+  fwd8<1>(x, tw, q, true);
   // pass C: spans 16, 8, 4 (stride 4): m = 64 + t/4
   const int bC = pass_base(t, 2);
   load_tw8(tw, T.rp, T.rps, 64 + (t >> 2));
@@ -169,14 +181,14 @@ __device__ __forceinline__ void ntt_fwd(uint64_t (&x)[8], uint64_t *lds, const N
   lds_put(lds, x, bB, 32);
   __syncthreads();
   lds_get(lds, x, bC, 4);
-  fwd8(x, tw, q, true);
+  fwd8<2>(x, tw, q, true);
   // pass D: spans 2, 1 on slots 8t .. 8t+7: m = 256 + t
   load_tw8(tw, T.rp, T.rps, 256 + t);
   __syncthreads();
   lds_put(lds, x, bC, 4);
   __syncthreads();
   lds_get(lds, x, 8 * t, 1);
-  fwd8(x, tw, q, false);
+  fwd8<0>(x, tw, q, false);
 }
 
 // Inverse NTT without the N^-1 scale.  Entry: blocked layout, values < 2q.
@@ -213,10 +225,11 @@ __device__ __forceinline__ void ntt_inv(uint64_t (&x)[8], uint64_t *lds, const N
   inv8(x, tw, q, true);
 }
 
-// Canonical value of a forward-NTT output in [0, 4q).
+// Canonical value of a forward-NTT output in [0, 16q).
 __device__ __forceinline__ uint64_t ntt_fwd_final(uint64_t v, uint64_t q) {
-  const uint64_t two_q = 2 * q;
-  if (v >= two_q) v -= two_q;
+  if (v >= 8 * q) v -= 8 * q;
+  if (v >= 4 * q) v -= 4 * q;
+  if (v >= 2 * q) v -= 2 * q;
   if (v >= q) v -= q;
   return v;
 }
