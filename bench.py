@@ -64,11 +64,13 @@ def measured_traffic(kernel: str, fst: int, other: int):
     return None, None
 
 
-def mfma_kernel(tuning: dict, batch: int):
+def mfma_kernel(tuning: dict, batch: int, fst: int):
     """(name, file key) of the matrix-core first-dimension kernel the server
     launches for a batch under this tuning."""
     cb = 1 if batch <= 16 else 2 if batch <= 32 else 4
     lds = tuning.get("fd_mfma_variant") == 1
+    if tuning.get("db_planes") and tuning.get("fd_mfma_variant") == 2 and fst % 64 == 0:
+        return f"k_first_dim_mfma_tiles<{cb}>", "k_first_dim_mfma_tiles"
     if tuning.get("db_planes"):
         return (f"k_first_dim_mfma_lds<{cb}, true>", "k_first_dim_mfma_lds_pre") if lds else \
                (f"k_first_dim_mfma_pre<{cb}>", "k_first_dim_mfma_pre")
@@ -205,7 +207,7 @@ def main() -> int:
     int8_ops = 2 * 64 * (ncp * num_pt * N)  # the padded column count is what the tiles compute
     useful_ops = 2 * 64 * (2 * B * num_pt * N)
     mfma_alg_bytes = db_bytes + N * fst * ncp * 8 + N * other * 2 * B * 8
-    mfma_name, mfma_key = mfma_kernel(tuning, B)
+    mfma_name, mfma_key = mfma_kernel(tuning, B, fst)
     mfma_traffic, mfma_traffic_src = measured_traffic(f"{mfma_key}_b{B}", fst, other)
 
     # value: device time of each step (HIP events from the first kernel to the

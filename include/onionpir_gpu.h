@@ -77,10 +77,10 @@ typedef struct onion_stage_times {
 typedef struct onion_tuning {
   uint32_t fuse_min;     /* external products (conversion, mux levels) run one
                             workgroup per ciphertext, and an expansion level
-                            one launch of one workgroup per (node, component),
-                            when a stage has at least this many ciphertexts
-                            across the batch; below it every transform gets
-                            its own workgroup                               */
+                            one launch of one workgroup per node, when a
+                            stage has at least this many ciphertexts across
+                            the batch; below it every transform gets its own
+                            workgroup                                        */
   uint32_t fd_rows_b1;   /* rows per workgroup of the single-query first-
                             dimension kernel (multiple of 32, 32..1024)       */
   uint32_t fd_tile_rows; /* thread rows of the batched first-dimension tile
@@ -95,7 +95,11 @@ typedef struct onion_tuning {
                              cores (exact signed-digit products); 0 = never  */
   uint32_t fd_mfma_variant; /* 0: operands loaded to registers ahead of their
                                step; 1: direct-to-LDS loads three steps
-                               ahead, shared by the workgroup's waves        */
+                               ahead, shared by the workgroup's waves; 2: one
+                               persistent workgroup per compute unit that
+                               reduces each tile's outputs between the next
+                               tile's products (needs the plane copy and
+                               fst % 64 == 0, else 0 runs)                   */
   uint32_t db_planes;     /* 1: keep a second device copy of the database in
                              signed-digit planes (as many bytes again as the
                              database), rebuilt on every database write, so

@@ -405,6 +405,151 @@ __global__ __launch_bounds__(256) void k_first_dim_mfma_pre(const uint8_t *__res
   T.store(acc, out2, other, ncols, mq);
 }
 
+// ---- Persistent variant ----------------------------------------------------------------
+//
+// k_first_dim_mfma_pre spends about a fifth of a tile in its epilogue with the
+// matrix cores idle (one wave per SIMD: no other wave fills the gap), and each
+// new workgroup waits for its first loads.  Here a workgroup walks tiles
+// blockIdx.x, blockIdx.x + G, ... (G = gridDim.x, a multiple of 8, so every
+// tile of a workgroup keeps its XCD and the slot-to-XCD assignment of the
+// plain variant).  Two register buffers form one stream of steps over all of
+// a workgroup's tiles (a step's loads issue as soon as the products of the
+// step two back have), so the loads of a tile's first steps are in flight
+// while the previous tile ends.  At a tile's end each wave forms C mod 2^128
+// of its 16 outputs from the accumulators (the serial part) and parks them in
+// LDS; during the next tile's steps 0..15 it reduces one parked output per
+// step (Barrett, the store), VALU work that issues between the matrix
+// products.  Outputs outside the matrix are dropped by the buffer store's
+// range check (offset 2^31 against a 2^31-byte range, or a range of 0 while
+// nothing is parked), so every step is branch-free.  Requires fst % 64 == 0
+// (an even number of steps); a third buffer would not fit beside the
+// accumulators and the epilogue.
+
+constexpr int kStageWords = 16 * 64;  // per wave: 16 outputs x 64 lanes of (lo, hi)
+
+__device__ __forceinline__ void diagonals_c128(const v16i_t (&acc)[15], int e, const DigitWeights &W, uint64_t &lo,
+                                               uint64_t &hi) {
+  int64_t P[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    P[k] = acc[4 * k][e];
+#pragma unroll
+    for (int d = 1; d < 4; ++d)
+      if (4 * k + d < 15) P[k] = (int64_t)acc[4 * k + d][e] * (int64_t)W.w[d] + P[k];
+  }
+  lo = (uint64_t)P[0];
+  hi = (uint64_t)(P[0] >> 63);
+  add128(lo, hi, (uint64_t)P[1] << 32, (uint64_t)(P[1] >> 32));
+  hi += (uint64_t)P[2] + ((uint64_t)P[3] << 32);
+}
+
+typedef unsigned v2u_t __attribute__((ext_vector_type(2)));
+
+// A step's 8 chunks from a wave-uniform base (scalar address) plus the lane's
+// byte offset.
+__device__ __forceinline__ void load_chunks_at(uint4 (&x)[8], const uint8_t *base, uint32_t s, uint32_t lane_byte) {
+  const uint8_t *p = base + (size_t)s * kStepBytes;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) x[i] = *reinterpret_cast<const uint4 *>(p + lane_byte + i * 1024);
+}
+
+// Where a wave's outputs go: row m = row0 + 8 (e / 4) + 4 g + e % 4 of slot
+// `slot`, columns cb * 32 + (lane % 32).
+struct OutTile {
+  uint32_t slot, row0;
+};
+
+template <int CB>
+__global__ __launch_bounds__(256) void k_first_dim_mfma_tiles(const uint8_t *__restrict__ dbp,
+                                                              const uint8_t *__restrict__ Bp,
+                                                              uint64_t *__restrict__ out2, uint32_t fst,
+                                                              uint32_t other, uint32_t ncols, uint32_t n_rt, Mod mq) {
+  constexpr int RB = 4 / CB;
+  __shared__ uint4 stage[4 * kStageWords];
+  const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t l = threadIdx.x & 63, g = l >> 5, li = l & 31;
+  const uint32_t rb = w / CB, cb = w % CB;
+  const uint32_t n_rb = (other + 31) / 32, steps = fst / 32, total = kN * n_rt, G = gridDim.x;
+  if (blockIdx.x >= total) return;
+  uint4 *st = stage + w * kStageWords + l;
+  // tile L: slot (L / 8 / n_rt) * 8 + L % 8, row tile (L / 8) % n_rt
+  auto geo = [&](uint32_t L) {
+    const uint32_t seq = L >> 3;
+    return OutTile{(seq / n_rt) * 8 + (L & 7), (seq % n_rt) * (32 * RB) + rb * 32};
+  };
+  // loader: tile lt, step ls; past the last tile it repeats the last step
+  uint32_t lt = blockIdx.x, ls = 0;
+  const uint8_t *la, *lb;
+  auto set_bases = [&]() {
+    const OutTile o = geo(lt);
+    la = dbp + chunk_offset((size_t)o.slot * n_rb + min(o.row0 / 32, n_rb - 1), steps, 0, 0, 0);
+    lb = Bp + chunk_offset((size_t)o.slot * CB + cb, steps, 0, 0, 0);
+  };
+  auto load_next = [&](StepRaw &r) {
+    load_chunks_at(r.a, la, ls, l * 16);
+    load_chunks_at(r.b, lb, ls, l * 16);
+    if (++ls == steps) {
+      if (lt + G < total) {
+        ls = 0;
+        lt += G;
+        set_bases();
+      } else {
+        ls = steps - 1;
+      }
+    }
+  };
+  set_bases();
+  StepRaw R[2];
+  load_next(R[0]);
+  load_next(R[1]);
+
+  // output e of the parked tile `prev`, stored only when `live`
+  const uint32_t lane_off = (4 * g * kN * ncols + li) * 8;
+  const bool col_ok = cb * 32 + li < ncols;
+  auto put = [&](const OutTile &prev, uint32_t e, bool live) {
+    const uint4 c = st[e * 64];
+    const uint64_t v = barrett128((uint64_t)c.x | (uint64_t)c.y << 32, (uint64_t)c.z | (uint64_t)c.w << 32, mq);
+    const uint32_t urow = prev.row0 + 8 * (e >> 2) + (e & 3);
+    uint64_t *base = out2 + ((size_t)urow * kN + prev.slot) * ncols + cb * 32;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(base, 0, live ? 0x80000000u : 0u, 0x00020000);
+    const uint32_t off = col_ok && urow + 4 * g < other ? lane_off : 0x80000000u;
+    v2u_t vv;
+    vv[0] = (uint32_t)v;
+    vv[1] = (uint32_t)(v >> 32);
+    __builtin_amdgcn_raw_buffer_store_b64(vv, rs, off, 0, 0);
+  };
+
+  const DigitWeights W;
+  v16i_t acc[15];
+#pragma unroll
+  for (int d = 0; d < 15; ++d) acc[d] = (v16i_t){0};
+  OutTile prev{0, 0};
+  bool have_prev = false;
+  for (uint32_t ct = blockIdx.x; ct < total; ct += G) {
+    for (uint32_t s = 0; s < steps; s += 2) {
+#pragma unroll
+      for (int d = 0; d < 2; ++d) {
+        step_products(acc, R[d].a, R[d].b);
+        put(prev, min(s + d, 15u), have_prev && s + d < 16);
+        load_next(R[d]);
+      }
+    }
+    if (have_prev)
+      for (uint32_t e = steps; e < 16; ++e) put(prev, e, true);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      uint64_t lo, hi;
+      diagonals_c128(acc, e, W, lo, hi);
+      st[e * 64] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+    }
+#pragma unroll
+    for (int k = 0; k < 15; ++k) acc[k] = (v16i_t){0};
+    prev = geo(ct);
+    have_prev = true;
+  }
+  for (uint32_t e = 0; e < 16; ++e) put(prev, e, true);
+}
+
 // ---- LDS-pipelined variant ------------------------------------------------------------
 //
 // Same tiles and arithmetic, but the operands travel global -> LDS by

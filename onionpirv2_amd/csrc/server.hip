@@ -139,6 +139,7 @@ struct onion_server {
   onion_params P{};
   onion_param_info I{};
   int device = 0;
+  uint32_t n_cu = 0;  // compute units of the device
   uint64_t w = 0;  // 2^h
   // device NTT tables: q then p, each rp rps irp irps (4N words)
   uint64_t *tabs = nullptr;
@@ -164,7 +165,7 @@ struct onion_server {
   uint8_t *pack = nullptr;
   // measured on MI355X at 512x1024: 8 nontemporal 16-byte loads in flight per
   // lane and 256-row blocks stream the database at 94% of a plain read kernel
-  onion_tuning tune{1024, 256, 4, 3, 0, 2, 0, 1};
+  onion_tuning tune{1024, 256, 4, 3, 0, 2, 2, 1};
   uint64_t tune_epoch = 1;
   Gate gate;
   mutable std::mutex last_mu;
@@ -413,7 +414,11 @@ template <int CB>
 void launch_mfma(const onion_server *s, Lane *L, const uint8_t *Bp, dim3 grid, uint32_t n_rt, uint32_t ncols) {
   const uint32_t fst = s->P.fst_dim, other = s->P.other_dim;
   const bool lds = s->tune.fd_mfma_variant == 1;
-  if (s->dbp) {
+  if (s->dbp && s->tune.fd_mfma_variant == 2 && fst % 64 == 0) {
+    // persistent: one workgroup per compute unit (a multiple of 8, at most the tile count)
+    const uint32_t total = grid.x, G = std::max(8u, std::min(total, s->n_cu) / 8 * 8);
+    k_first_dim_mfma_tiles<CB><<<G, 256, 0, L->st>>>(s->dbp, Bp, L->digits, fst, other, ncols, n_rt, s->mq);
+  } else if (s->dbp) {
     if (lds)
       k_first_dim_mfma_lds<CB, true><<<grid, 256, 0, L->st>>>(s->db, s->dbp, Bp, L->digits, fst, other, ncols,
                                                              n_rt, s->mq);
@@ -773,7 +778,7 @@ int autotune_impl(onion_server *s, const uint32_t *sizes, uint32_t n_sizes, uint
     bool active;
   };
   const std::vector<Axis> axes = {
-      {&onion_tuning::fd_mfma_variant, {0, 1}, hasb},
+      {&onion_tuning::fd_mfma_variant, {0, 1, 2}, hasb},
       {&onion_tuning::fd_mfma_min, {2, 4, 8, 16, 0}, has_small},
       {&onion_tuning::fd_tile_rows, {2, 4}, hasb},
       {&onion_tuning::fd_b1_variant, {0, 1, 2, 3}, has1},
@@ -849,6 +854,14 @@ int onion_gpu_create(const onion_params *params, onion_server **out) {
   if (hipGetDevice(&s->device) != hipSuccess) {
     delete s;
     return fail(ONION_EHIP, "no HIP device");
+  }
+  {
+    int cu = 0;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess || cu <= 0) {
+      delete s;
+      return fail(ONION_EHIP, "cannot read the device's compute unit count");
+    }
+    s->n_cu = (uint32_t)cu;
   }
   s->lanes.emplace_back();
   if ((rc = make_lane(s->lanes[0]))) {
@@ -1188,7 +1201,7 @@ int onion_gpu_set_tuning(onion_server *s, const onion_tuning *t) {
   if (t->fd_tile_rows != 2 && t->fd_tile_rows != 4) return fail(ONION_EINVAL, "fd_tile_rows must be 2 or 4");
   if (t->fd_b1_variant > 3) return fail(ONION_EINVAL, "fd_b1_variant must be 0..3");
   if (t->use_graph > 1) return fail(ONION_EINVAL, "use_graph must be 0 or 1");
-  if (t->fd_mfma_variant > 1) return fail(ONION_EINVAL, "fd_mfma_variant must be 0 or 1");
+  if (t->fd_mfma_variant > 2) return fail(ONION_EINVAL, "fd_mfma_variant must be 0, 1 or 2");
   if (t->db_planes > 1) return fail(ONION_EINVAL, "db_planes must be 0 or 1");
   Exclusive x(s->gate);
   const uint32_t had = s->tune.db_planes;

@@ -199,10 +199,10 @@ def test_first_dim_wraps_mod_2_128():
         rng = np.random.default_rng(8)
         sel2 = rng.integers(0, o.q, size=(o.fst, 2 * N), dtype=np.uint64)
         batch = np.stack([sel, sel2, sel])
-        for mfma_min, planes in ((2, 1), (2, 0), (0, 0)):
-            srv.set_tuning(fd_mfma_min=mfma_min, db_planes=planes)
+        for mfma_min, planes, var in ((2, 1, 0), (2, 1, 2), (2, 0, 0), (0, 0, 0)):
+            srv.set_tuning(fd_mfma_min=mfma_min, db_planes=planes, fd_mfma_variant=var)
             gb = srv.tap_first_dim_batch(batch)
-            assert np.array_equal(gb[0], exp) and np.array_equal(gb[2], exp), f"fd_mfma_min {mfma_min}"
+            assert np.array_equal(gb[0], exp) and np.array_equal(gb[2], exp), f"fd_mfma_min {mfma_min} var {var}"
             assert np.array_equal(gb[1], o.first_dim(db.reshape(-1), sel2))
 
 
@@ -228,7 +228,35 @@ def test_partial_upload_updates_the_plane_copy():
         got = srv.tap_first_dim_batch(sel)
         for b in range(3):
             assert np.array_equal(got[b], o.first_dim(db.reshape(-1), sel[b])), f"query {b}"
+        srv.set_tuning(fd_mfma_variant=2)
+        assert np.array_equal(srv.tap_first_dim_batch(sel), got)
         srv.set_tuning(db_planes=0)
+        assert np.array_equal(srv.tap_first_dim_batch(sel), got)
+
+
+@pytest.mark.parametrize("fst,other,nq", [(1024, 8, 3), (128, 64, 40)])
+def test_persistent_first_dim_matches_oracle(fst, other, nq):
+    """The persistent matrix-core product (fd_mfma_variant = 2) walks several
+    tiles per workgroup and reduces each tile's outputs during the next tile's
+    steps.  fst = 1024: 32 steps per tile, more than its 16 outputs;
+    fst = 128: 4 steps, the other 12 outputs reduced at the tile's end.
+    nq = 3 takes one column block of 32 (122 of its 128 columns masked) and
+    four row blocks (other = 8: 120 of 128 rows masked); nq = 40 takes four
+    column blocks and one row block.  It matches the oracle and the per-tile
+    kernel."""
+    from onionpirv2_amd import GpuServer
+    o = Oracle(fst=fst, other=other)
+    db = o.db_build(6, 0)
+    rng = np.random.default_rng(fst + nq)
+    sel = rng.integers(0, o.q, size=(nq, fst, 2 * N), dtype=np.uint64)
+    sel[0, :, :4] = o.q - 1
+    with GpuServer(fst, other) as srv:
+        srv.db_generate(6, 0)
+        srv.set_tuning(fd_mfma_min=2, fd_mfma_variant=2)
+        got = srv.tap_first_dim_batch(sel)
+        for b in sorted({0, nq - 1}):
+            assert np.array_equal(got[b], o.first_dim(db, sel[b])), f"query {b}"
+        srv.set_tuning(fd_mfma_variant=0)
         assert np.array_equal(srv.tap_first_dim_batch(sel), got)
 
 
@@ -283,7 +311,7 @@ def test_first_dim_batch_matches_oracle(cfg1, nq):
     try:
         srv.set_tuning(fd_mfma_min=0)
         assert np.array_equal(srv.tap_first_dim_batch(sel), got)
-        for var, planes in ((0, 1), (1, 1), (0, 0), (1, 0)):
+        for var, planes in ((0, 1), (1, 1), (2, 1), (0, 0), (1, 0)):
             srv.set_tuning(fd_mfma_min=2, fd_mfma_variant=var, db_planes=planes)
             assert np.array_equal(srv.tap_first_dim_batch(sel), got), f"variant {var} planes {planes}"
     finally:
@@ -327,7 +355,8 @@ def test_answer_batch_matches_single_and_oracle(cfg1, nq):
                                     dict(fd_b1_variant=1), dict(fd_b1_variant=2),
                                     dict(fd_b1_variant=3), dict(fd_mfma_min=0),
                                     dict(fd_mfma_min=6), dict(fd_mfma_variant=0),
-                                    dict(fd_mfma_variant=1), dict(db_planes=0),
+                                    dict(fd_mfma_variant=1), dict(fd_mfma_variant=2),
+                                    dict(db_planes=0),
                                     dict(db_planes=0, fd_mfma_variant=1)])
 def test_tuning_choices_are_bit_identical(cfg1, tuning):
     """Every launch-shape choice (fused vs split external products, first-dim
