@@ -532,6 +532,13 @@ __global__ __launch_bounds__(256) void k_first_dim_mfma_tiles(const uint8_t *__r
         step_products(acc, R[d].a, R[d].b);
         put(prev, min(s + d, 15u), have_prev && s + d < 16);
         load_next(R[d]);
+        // spread the step's VALU work evenly between its 64 products (grouped
+        // after a few of them it would stall the matrix core)
+#pragma unroll
+        for (int i = 0; i < 64; ++i) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+          __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // two VALU
+        }
       }
     }
     if (have_prev)
