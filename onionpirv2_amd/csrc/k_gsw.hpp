@@ -7,7 +7,8 @@
 // Split in two launches so that every digit NTT of every ciphertext of every
 // query in the batch runs in parallel:
 //   k_decomp_ntt  -- one workgroup per (ciphertext, digit): extract the digit
-//                    polynomial, NTT it, store it (canonical, NTT form);
+//                    polynomial, NTT it, store it (NTT form, below 4q:
+//                    the 128-bit sums reduce it exactly);
 //   k_ext_mac_*   -- one workgroup per (ciphertext, output component):
 //                    sum_j D_j * G_j[comp] in 128-bit, Barrett mod q, then
 //                    either store (conversion, NTT form) or INTT and add into
@@ -47,7 +48,7 @@ __global__ __launch_bounds__(kNttThreads) void k_decomp_ntt(const uint64_t *src_
   }
   ntt_fwd<true>(x, lds, T);  // digits < 2^base_log2 <= 2^32
 #pragma unroll
-  for (int r = 0; r < 8; ++r) x[r] = ntt_fwd_final(x[r], T.q);
+  for (int r = 0; r < 8; ++r) x[r] = ntt_fwd_below_4q(x[r], T.q);  // only ext_mac8 reads D
   store_blocked(D + (size_t)blockIdx.y * d_stride + ((size_t)ct * rows + j) * kN, x);
 }
 
@@ -139,7 +140,7 @@ __device__ __forceinline__ void ext_fused_core(const uint64_t *xs, const uint64_
       ntt_fwd<true>(d, lds, T);  // digits < 2^base_log2 <= 2^32
       const uint64_t *g = G + (size_t)(poly * l + kk) * 2 * kN;
 #pragma unroll
-      for (int r = 0; r < 8; ++r) d[r] = ntt_fwd_final(d[r], T.q);
+      for (int r = 0; r < 8; ++r) d[r] = ntt_fwd_below_4q(d[r], T.q);
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         uint64_t gv[8];

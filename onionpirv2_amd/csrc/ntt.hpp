@@ -234,6 +234,15 @@ __device__ __forceinline__ uint64_t ntt_fwd_final(uint64_t v, uint64_t q) {
   return v;
 }
 
+// A forward-NTT output in [0, 16q) brought below 4q (< 2^62), enough for the
+// 128-bit multiply-accumulate of external products (mac128), whose Barrett
+// reduction at the end returns the canonical residue.
+__device__ __forceinline__ uint64_t ntt_fwd_below_4q(uint64_t v, uint64_t q) {
+  if (v >= 8 * q) v -= 8 * q;
+  if (v >= 4 * q) v -= 4 * q;
+  return v;
+}
+
 // Canonical value of an inverse-NTT output in [0, 2q), scaled by N^-1.
 __device__ __forceinline__ uint64_t ntt_inv_final(uint64_t v, const NttTab &T) {
   const uint64_t r = mul_shoup_lazy(v, T.n_inv, T.n_inv_s, T.q);
