@@ -234,7 +234,7 @@ def test_partial_upload_updates_the_plane_copy():
         assert np.array_equal(srv.tap_first_dim_batch(sel), got)
 
 
-@pytest.mark.parametrize("fst,other,nq", [(1024, 8, 3), (128, 64, 40)])
+@pytest.mark.parametrize("fst,other,nq", [(1024, 8, 3), (128, 64, 40), (96, 16, 20)])
 def test_persistent_first_dim_matches_oracle(fst, other, nq):
     """The persistent matrix-core product (fd_mfma_variant = 2) walks several
     tiles per workgroup and reduces each tile's outputs during the next tile's
@@ -242,8 +242,9 @@ def test_persistent_first_dim_matches_oracle(fst, other, nq):
     fst = 128: 4 steps, the other 12 outputs reduced at the tile's end.
     nq = 3 takes one column block of 32 (122 of its 128 columns masked) and
     four row blocks (other = 8: 120 of 128 rows masked); nq = 40 takes four
-    column blocks and one row block.  It matches the oracle and the per-tile
-    kernel."""
+    column blocks and one row block.  fst = 96 (an odd number of steps) is
+    outside the persistent kernel's shapes, so the per-tile kernel runs.
+    It matches the oracle and the per-tile kernel."""
     from onionpirv2_amd import GpuServer
     o = Oracle(fst=fst, other=other)
     db = o.db_build(6, 0)
