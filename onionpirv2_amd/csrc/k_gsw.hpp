@@ -187,34 +187,138 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused(uint64_t *cts, uint64
 // NTT-form outputs: for pair r of query b (grid (half, nq)), with
 // k_ntt_inv_gather<DIFF> having left v = y - x in coefficient form at row
 // r + half and NTT(x) in slot order at row r of the query's cts,
-//   x_r = INTT(NTT(x) + ext(RGSW, v)),
-// which equals the reference's x + INTT(ext(RGSW, y - x)) mod q (INTT is
-// linear), so x's own inverse transform is folded into this one.
+//   X_r = NTT(x) + ext(RGSW, v),
+// stored in NTT form (slot order), which is NTT of the reference's
+// x + INTT(ext(RGSW, y - x)) mod q (the transforms are linear); the next
+// level (k_mux_fused_ntt) works on NTT-form ciphertexts, so no inverse
+// transform is spent here.
 __global__ __launch_bounds__(kNttThreads) void k_mux_first(uint64_t *cts, uint64_t cts_stride, uint64_t half,
                                                            const uint64_t *G, uint64_t g_stride, uint32_t l,
                                                            uint32_t base_log2, NttTab T, Mod mq) {
   __shared__ uint64_t lds[kNttLds];
-  __shared__ uint64_t park[kN];  // component 1 waits here during component 0's INTT
   const uint32_t r = blockIdx.x, b = blockIdx.y;
   uint64_t *x = cts + (size_t)b * cts_stride + (size_t)r * 2 * kN;
   const uint64_t *v = x + (size_t)half * 2 * kN;
   uint64_t res[2][8];
   ext_fused_core(v, nullptr, G + (size_t)b * g_stride, l, base_log2, lds, T, mq, res);
 #pragma unroll
-  for (int j = 0; j < 8; ++j) park[8 * threadIdx.x + j] = res[1][j];
-#pragma unroll 1
   for (uint32_t c = 0; c < 2; ++c) {
     uint64_t w[8];
     load_blocked(w, x + (size_t)c * kN);  // NTT(x), slot order
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const uint64_t e = c == 0 ? res[0][j] : park[8 * threadIdx.x + j];
-      w[j] = add_mod(e, w[j], T.q);
-    }
-    ntt_inv(w, lds, T);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) x[(size_t)c * kN + threadIdx.x + 256 * j] = ntt_inv_final(w[j], T);
+    for (int j = 0; j < 8; ++j) w[j] = add_mod(res[c][j], w[j], T.q);
+    store_blocked(x + (size_t)c * kN, w);
   }
+}
+
+// s * 2^b mod q, not fully reduced, for q = 2^60 - c: with s 2^b = h 2^60 + lo
+// (lo < 2^60) and 2^60 = c (mod q), the value h c + lo.  For s < 2^62 and
+// b <= 12, h < 2^14, so with c < 2^32 the result is below 2^60 + 2^46 < 2q.
+__device__ __forceinline__ uint64_t mul_pow2_mod(uint64_t s, uint32_t b, uint32_t c) {
+  const uint32_t h = (uint32_t)(s >> (60 - b));
+  return (uint64_t)h * c + ((s << b) & ((1ULL << 60) - 1));
+}
+
+// 128-bit multiply-accumulate of one transformed digit with RGSW row g (both
+// components; blocked slots 8t .. 8t+7).
+__device__ __forceinline__ void mac_row(uint64_t (&lo)[2][8], uint64_t (&hi)[2][8], const uint64_t (&d)[8],
+                                        const uint64_t *g) {
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    uint64_t gv[8];
+    load_blocked(gv, g + (size_t)c * kN);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) mac128(lo[c][r], hi[c][r], d[r], gv[r]);
+  }
+}
+
+// Mux fold on NTT-form ciphertexts (slot order), for pair blockIdx.x of query
+// blockIdx.y (X = cts + i*2N, Y = X + half*2N):
+//   X_c += ext(RGSW, y - x)_c,   y - x = INTT(Y - X) (canonical coefficients).
+// The same sum as the reference's x + INTT(ext(RGSW, y - x)), kept in NTT
+// form.  Of each polynomial's l digits d_j (c = sum_j d_j 2^((l-1-j)B)
+// exactly, l B >= 60 > log2 q), only d_0 .. d_{l-2} are transformed; their
+// transforms are folded by Horner's rule into S = NTT(c >> B), and the last
+// digit's transform is NTT(d_{l-1}) = (Y - X) - 2^B S (mod q).  Per pair: two
+// inverse and 2(l - 1) forward transforms, against 2l forward and two inverse
+// in coefficient form.  Requires q = 2^60 - c with c < 2^32 and B <= 12.
+__global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, uint64_t cts_stride, uint64_t half,
+                                                               const uint64_t *G, uint64_t g_stride, uint32_t l,
+                                                               uint32_t base_log2, NttTab T, Mod mq) {
+  __shared__ uint64_t lds[kNttLds];
+  __shared__ uint64_t coef[kN];  // y - x of the current polynomial; slots t + 256 j of thread t
+  __shared__ uint64_t diff[kN];  // Y - X of the current polynomial; slots 8t .. 8t+7 of thread t
+  const int t = threadIdx.x;
+  uint64_t *x = cts + (size_t)blockIdx.y * cts_stride + (size_t)blockIdx.x * 2 * kN;
+  const uint64_t *y = x + half * 2 * kN;
+  const uint64_t *Gb = G + (size_t)blockIdx.y * g_stride;
+  const uint64_t q = T.q, mask = (1ULL << base_log2) - 1;
+  const uint32_t c60 = (uint32_t)((1ULL << 60) - q);
+  uint64_t lo[2][8], hi[2][8];
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int r = 0; r < 8; ++r) lo[c][r] = hi[c][r] = 0;
+#pragma unroll 1
+  for (uint32_t poly = 0; poly < 2; ++poly) {
+    {
+      uint64_t v[8], w[8];
+      load_blocked(v, x + (size_t)poly * kN);
+      load_blocked(w, y + (size_t)poly * kN);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        v[r] = sub_mod(w[r], v[r], q);
+        diff[8 * t + r] = v[r];
+      }
+      ntt_inv(v, lds, T);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) coef[t + 256 * r] = ntt_inv_final(v[r], T);
+    }
+    uint64_t S[8];  // below 4q
+#pragma unroll
+    for (int r = 0; r < 8; ++r) S[r] = 0;
+#pragma unroll 1
+    for (uint32_t kk = 0; kk + 1 < l; ++kk) {
+      const uint32_t sh = (l - 1 - kk) * base_log2;
+      uint64_t d[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) d[r] = (coef[t + 256 * r] >> sh) & mask;
+      ntt_fwd<true>(d, lds, T);  // digits < 2^base_log2
+#pragma unroll
+      for (int r = 0; r < 8; ++r) d[r] = ntt_fwd_below_4q(d[r], q);
+      mac_row(lo, hi, d, Gb + (size_t)(poly * l + kk) * 2 * kN);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        uint64_t s = mul_pow2_mod(S[r], base_log2, c60) + d[r];  // < 2q + 4q
+        S[r] = s >= 4 * q ? s - 4 * q : s;
+      }
+    }
+    uint64_t d[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) d[r] = diff[8 * t + r] + 2 * q - mul_pow2_mod(S[r], base_log2, c60);  // < 3q
+    mac_row(lo, hi, d, Gb + (size_t)(poly * l + l - 1) * 2 * kN);
+  }
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    uint64_t w[8];
+    load_blocked(w, x + (size_t)c * kN);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) w[r] = add_mod(w[r], barrett128(lo[c][r], hi[c][r], mq), q);
+    store_blocked(x + (size_t)c * kN, w);
+  }
+}
+
+// NTT-form ciphertexts (slot order) back to canonical coefficient form, in
+// place: polynomial blockIdx.x of query blockIdx.y.  Every thread has read its
+// slots before the first barrier inside ntt_inv, and writes after the last.
+__global__ __launch_bounds__(kNttThreads) void k_cts_to_coeff(uint64_t *cts, uint64_t cts_stride, NttTab T) {
+  __shared__ uint64_t lds[kNttLds];
+  uint64_t *p = cts + (size_t)blockIdx.y * cts_stride + (size_t)blockIdx.x * kN;
+  uint64_t x[8];
+  load_blocked(x, p);
+  ntt_inv(x, lds, T);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) p[threadIdx.x + 256 * j] = ntt_inv_final(x[j], T);
 }
 
 // Conversion bottom rows, fused: row l + c%l of RGSW c/l = ext(RGSW(s) key, leaf c).

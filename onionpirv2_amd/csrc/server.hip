@@ -508,14 +508,39 @@ void run_first_dim(const onion_server *s, Lane *L, uint32_t nq, const uint64_t *
   k_ntt_inv<<<other * 2 * nq, kNttThreads, 0, L->st>>>(L->fd, L->fd, s->Tq);
 }
 
-// One multiplexer level over n cts per query in fd, with RGSW `g` (per-query stride).
-void run_mux_level(const onion_server *s, Lane *L, uint32_t nq, uint64_t n, const uint64_t *g) {
+// n NTT-form cts per query in fd back to coefficient form.
+void cts_to_coeff(const onion_server *s, Lane *L, uint32_t nq, uint64_t n) {
+  k_cts_to_coeff<<<dim3((uint32_t)(2 * n), nq), kNttThreads, 0, L->st>>>(L->fd, s->fd_words, s->Tq);
+}
+
+// k_mux_fused_ntt's arithmetic: q = 2^60 - c with c < 2^32, digits of at most
+// 12 bits, at least two of them.
+bool mux_ntt_ok(const onion_server *s) {
+  return s->I.q < (1ULL << 60) && (1ULL << 60) - s->I.q < (1ULL << 32) && s->I.base_log2 <= 12 && s->P.gsw_l >= 2;
+}
+
+// One multiplexer level over n cts per query in fd, with RGSW `g` (per-query
+// stride).  *ntt_form: the cts are in NTT form (slot order); a wide level
+// keeps them so, a narrow one (the split kernels) first brings them back to
+// coefficient form.
+void run_mux_level(const onion_server *s, Lane *L, uint32_t nq, uint64_t n, const uint64_t *g,
+                   bool *ntt_form = nullptr) {
   const uint32_t l = s->P.gsw_l;
   const uint64_t half = n / 2;
+  const bool in_ntt = ntt_form && *ntt_form;
   if (half * nq >= s->tune.fuse_min) {
-    k_mux_fused<<<dim3((uint32_t)half, nq), kNttThreads, 0, L->st>>>(L->fd, s->fd_words, half, g, s->rgsw_words,
-                                                                    l, s->I.base_log2, s->Tq, s->mq);
+    if (in_ntt)
+      k_mux_fused_ntt<<<dim3((uint32_t)half, nq), kNttThreads, 0, L->st>>>(L->fd, s->fd_words, half, g,
+                                                                          s->rgsw_words, l, s->I.base_log2, s->Tq,
+                                                                          s->mq);
+    else
+      k_mux_fused<<<dim3((uint32_t)half, nq), kNttThreads, 0, L->st>>>(L->fd, s->fd_words, half, g, s->rgsw_words,
+                                                                      l, s->I.base_log2, s->Tq, s->mq);
     return;
+  }
+  if (in_ntt) {
+    cts_to_coeff(s, L, nq, n);
+    *ntt_form = false;
   }
   k_decomp_ntt<<<dim3((uint32_t)(half * 2 * l), nq), kNttThreads, 0, L->st>>>(
       L->fd, L->fd + half * 2 * kN, s->fd_words, L->digits, s->digit_words, l, s->I.base_log2, s->Tq);
@@ -524,11 +549,13 @@ void run_mux_level(const onion_server *s, Lane *L, uint32_t nq, uint64_t n, cons
 }
 
 // The first multiplexer level takes the batched first dimension's NTT-form
-// outputs directly (k_ntt_inv_gather_diff + k_mux_first: four inverse
-// transforms per pair instead of six) when the matrix-core product runs and
-// the level is wide enough for the fused external product.
+// outputs directly (k_ntt_inv_gather<8, true> + k_mux_first: two inverse
+// transforms per pair instead of six) and leaves NTT-form cts for the wide
+// levels after it, when the matrix-core product runs and the level is wide
+// enough for the fused external product.
 bool mux_first_fused(const onion_server *s, const Lane *L, uint32_t nq) {
-  return s->I.ndim >= 2 && use_mfma(s, L, nq) && (uint64_t)(s->P.other_dim / 2) * nq >= s->tune.fuse_min;
+  return s->I.ndim >= 2 && use_mfma(s, L, nq) && mux_ntt_ok(s) &&
+         (uint64_t)(s->P.other_dim / 2) * nq >= s->tune.fuse_min;
 }
 
 void run_mux_first(const onion_server *s, Lane *L, uint32_t nq) {
@@ -558,15 +585,18 @@ void answer_device(const onion_server *s, Lane *L, uint32_t nq, bool timed) {
   if (timed) (void)hipEventRecord(L->ev[3], L->st);
   uint64_t n = s->P.other_dim;
   uint32_t d = 0;
+  bool ntt_form = false;
   if (first_fused) {
     run_mux_first(s, L, nq);
     n /= 2;
     d = 1;
+    ntt_form = true;
   }
   for (; d + 1 < s->I.ndim; ++d) {
-    run_mux_level(s, L, nq, n, L->rgsw + (size_t)d * 2 * l * 2 * kN);
+    run_mux_level(s, L, nq, n, L->rgsw + (size_t)d * 2 * l * 2 * kN, &ntt_form);
     n /= 2;
   }
+  if (ntt_form) cts_to_coeff(s, L, nq, n);
   if (timed) (void)hipEventRecord(L->ev[4], L->st);
   run_mod_switch(s, L, nq, L->fd, s->fd_words);
   if (timed) (void)hipEventRecord(L->ev[5], L->st);

@@ -266,7 +266,11 @@ def test_first_mux_level_from_ntt_form(shape):
     """A batch whose first multiplexer level runs from the matrix-core
     product's NTT-form outputs (fuse_min = 1) answers as the oracle does, and
     as the path that transforms every first-dimension output first
-    (fuse_min = 2^30).  half = 2 and 8 take both workgroup-to-XCD mappings."""
+    (fuse_min = 2^30).  half = 2 and 8 take both workgroup-to-XCD mappings.
+    The levels after it keep NTT-form ciphertexts while they are wide
+    (k_mux_fused_ntt) and return to coefficient form for the narrow split
+    kernels: fuse_min = 5 / 10 / 20 put that switch after the first, second
+    or third level."""
     from onionpirv2_amd import GpuServer
     fst, other = shape
     o = Oracle(fst=fst, other=other)
@@ -286,6 +290,9 @@ def test_first_mux_level_from_ntt_form(shape):
         fused = srv.answer_batch([0, 1, 2], qs)
         srv.set_tuning(fuse_min=1 << 30)
         assert np.array_equal(srv.answer_batch([0, 1, 2], qs), fused)
+        for fm in (5, 10, 20):
+            srv.set_tuning(fuse_min=fm)
+            assert np.array_equal(srv.answer_batch([0, 1, 2], qs), fused), f"fuse_min {fm}"
         for i in (0, 2):
             seed, sk, gk, rk = keys[i]
             exp, _ = o.answer(db, gk, o.rgsw_key_ntt(rk), qs[i])
