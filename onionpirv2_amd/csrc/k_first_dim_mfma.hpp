@@ -704,17 +704,14 @@ __global__ __launch_bounds__(256) void k_first_dim_mfma_lds(const uint64_t *__re
 }
 
 // Inverse NTTs of the first-dimension outputs gathered from out2[r][slot][c]
-// into fd (query b = c / 2, poly c % 2 of ciphertext r, coefficient form).
+// into fd (query b = c / 2, poly c % 2 of ciphertext r, coefficient form),
+// for batches whose first multiplexer level does not read out2 itself.
 // One workgroup per (row, group of G adjacent columns): a thread reads the G
 // columns of each of its 8 slots as one contiguous 8G-byte piece (slots are
-// 2 KiB apart at 128 columns) and runs the G transforms one after another.
-// DIFF (first multiplexer level): row r < half = other / 2 pairs with row
-// r + half, and the transformed value is y - x = INTT(NTT(y) - NTT(x)), the
-// reference's coefficient-form difference mod q (INTT is linear), written over
-// row r + half; NTT(x) itself goes to row r in slot order, where the level's
-// k_mux_first reads it with contiguous loads.  Rows are dealt to XCDs
-// (r = xcd mod 8, column groups fastest) when the row count is a multiple of
-// 8, so the workgroups reading one row's 2 MiB share an L2.
+// 1 KiB apart at 128 columns) and runs the G transforms one after another.
+// Rows are dealt to XCDs (r = xcd mod 8, column groups fastest) when the row
+// count is a multiple of 8, so the workgroups reading one row's 2 MiB share an
+// L2.
 // Column K of the group, then the following ones (a recursion on a constant
 // K keeps x in registers; a loop around the transform is not unrolled).
 template <int G, int K = 0>
@@ -728,14 +725,14 @@ __device__ __forceinline__ void gather_transforms(uint64_t (&x)[G][8], uint64_t 
   if constexpr (K + 1 < G) gather_transforms<G, K + 1>(x, lds, T, fd, fd_stride, r, c0);
 }
 
-template <int G, bool DIFF>
+template <int G>
 __global__ __launch_bounds__(kNttThreads) void k_ntt_inv_gather(const uint64_t *__restrict__ out2, uint32_t other,
                                                                 uint32_t ncols, uint64_t *fd, uint64_t fd_stride,
                                                                 NttTab T) {
   __shared__ uint64_t lds[kNttLds];
-  const uint32_t rows = DIFF ? other / 2 : other, ngroups = ncols / G;
+  const uint32_t ngroups = ncols / G;
   uint32_t r, cg;
-  if (rows % 8 == 0) {
+  if (other % 8 == 0) {
     const uint32_t xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
     r = 8 * (k / ngroups) + xcd;
     cg = k % ngroups;
@@ -743,39 +740,18 @@ __global__ __launch_bounds__(kNttThreads) void k_ntt_inv_gather(const uint64_t *
     r = blockIdx.x / ngroups;
     cg = blockIdx.x % ngroups;
   }
-  const size_t sstride = ncols;
-  const uint64_t *sx = out2 + (size_t)r * kN * ncols + cg * G;
-  const uint64_t *sy = sx + (size_t)(other / 2) * kN * ncols;
+  const uint64_t *src = out2 + (size_t)r * kN * ncols + cg * G;
   uint64_t x[G][8];
 #pragma unroll
   for (int k = 0; k < G; k += 2) {  // column pairs: one 16-byte load per slot
-    uint64_t a[2][8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const size_t o = (size_t)(8 * threadIdx.x + j) * sstride;
-      const uint4 v = *reinterpret_cast<const uint4 *>(sx + o + k);
-      a[0][j] = ((uint64_t)v.y << 32) | v.x;
-      a[1][j] = ((uint64_t)v.w << 32) | v.z;
-      if (DIFF) {
-        const uint4 w = *reinterpret_cast<const uint4 *>(sy + o + k);
-        x[k][j] = sub_mod(((uint64_t)w.y << 32) | w.x, a[0][j], T.q);
-        x[k + 1][j] = sub_mod(((uint64_t)w.w << 32) | w.z, a[1][j], T.q);
-      } else {
-        x[k][j] = a[0][j];
-        x[k + 1][j] = a[1][j];
-      }
-    }
-    if (DIFF) {
-      // NTT(x) itself, slot order, into row r for the level's last transform:
-      // the thread's 8 slots are 64 contiguous bytes of each column
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const uint32_t c = cg * G + k + h;
-        store_blocked(fd + (size_t)(c >> 1) * fd_stride + ((size_t)r * 2 + (c & 1)) * kN, a[h]);
-      }
+      const uint4 v = *reinterpret_cast<const uint4 *>(src + (size_t)(8 * threadIdx.x + j) * ncols + k);
+      x[k][j] = ((uint64_t)v.y << 32) | v.x;
+      x[k + 1][j] = ((uint64_t)v.w << 32) | v.z;
     }
   }
-  gather_transforms<G>(x, lds, T, fd, fd_stride, DIFF ? r + other / 2 : r, cg * G);
+  gather_transforms<G>(x, lds, T, fd, fd_stride, r, cg * G);
 }
 
 }  // namespace onion

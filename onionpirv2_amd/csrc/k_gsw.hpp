@@ -183,40 +183,24 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused(uint64_t *cts, uint64
   }
 }
 
-// First multiplexer level, fused, for the batched first dimension's
-// NTT-form outputs: for pair r of query b (grid (half, nq)), with
-// k_ntt_inv_gather<DIFF> having left v = y - x in coefficient form at row
-// r + half and NTT(x) in slot order at row r of the query's cts,
-//   X_r = NTT(x) + ext(RGSW, v),
-// stored in NTT form (slot order), which is NTT of the reference's
-// x + INTT(ext(RGSW, y - x)) mod q (the transforms are linear); the next
-// level (k_mux_fused_ntt) works on NTT-form ciphertexts, so no inverse
-// transform is spent here.
-__global__ __launch_bounds__(kNttThreads) void k_mux_first(uint64_t *cts, uint64_t cts_stride, uint64_t half,
-                                                           const uint64_t *G, uint64_t g_stride, uint32_t l,
-                                                           uint32_t base_log2, NttTab T, Mod mq) {
-  __shared__ uint64_t lds[kNttLds];
-  const uint32_t r = blockIdx.x, b = blockIdx.y;
-  uint64_t *x = cts + (size_t)b * cts_stride + (size_t)r * 2 * kN;
-  const uint64_t *v = x + (size_t)half * 2 * kN;
-  uint64_t res[2][8];
-  ext_fused_core(v, nullptr, G + (size_t)b * g_stride, l, base_log2, lds, T, mq, res);
-#pragma unroll
-  for (uint32_t c = 0; c < 2; ++c) {
-    uint64_t w[8];
-    load_blocked(w, x + (size_t)c * kN);  // NTT(x), slot order
-#pragma unroll
-    for (int j = 0; j < 8; ++j) w[j] = add_mod(res[c][j], w[j], T.q);
-    store_blocked(x + (size_t)c * kN, w);
+// An RGSW for the NTT-form multiplexer: within each polynomial's block of l
+// rows, row j < l - 1 becomes G_j - 2^((l-1-j)B) G_{l-1} (mod q), in place
+// (row l - 1, which the others read, is left as it is).  Then, with
+// C = NTT(c) and c = sum_j d_j 2^((l-1-j)B) exactly,
+//   sum_j NTT(d_j) G_j = sum_{j<l-1} NTT(d_j) G'_j + C G_{l-1}  (mod q),
+// because NTT(d_{l-1}) = C - sum_{j<l-1} 2^((l-1-j)B) NTT(d_j): the last
+// digit needs no transform.  grid = (2 l N / 256, nq); g: the level's RGSW of
+// query 0, per-query stride g_stride.
+__global__ __launch_bounds__(256) void k_rgsw_fold_last(uint64_t *G, uint64_t g_stride, uint32_t l,
+                                                        uint32_t base_log2, Mod mq) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;  // (poly, comp, e) over 2 * 2 * N
+  const uint32_t poly = i / (2 * kN), rest = i % (2 * kN);
+  uint64_t *blk = G + (size_t)blockIdx.y * g_stride + (size_t)poly * l * 2 * kN + rest;
+  const uint64_t last = blk[(size_t)(l - 1) * 2 * kN];
+  for (uint32_t j = 0; j + 1 < l; ++j) {
+    const uint64_t pw = 1ULL << ((l - 1 - j) * base_log2);  // below 2^60 - B < q
+    blk[(size_t)j * 2 * kN] = sub_mod(blk[(size_t)j * 2 * kN], mulmod_barrett(last, pw, mq), mq.q);
   }
-}
-
-// s * 2^b mod q, not fully reduced, for q = 2^60 - c: with s 2^b = h 2^60 + lo
-// (lo < 2^60) and 2^60 = c (mod q), the value h c + lo.  For s < 2^62 and
-// b <= 12, h < 2^14, so with c < 2^32 the result is below 2^60 + 2^46 < 2q.
-__device__ __forceinline__ uint64_t mul_pow2_mod(uint64_t s, uint32_t b, uint32_t c) {
-  const uint32_t h = (uint32_t)(s >> (60 - b));
-  return (uint64_t)h * c + ((s << b) & ((1ULL << 60) - 1));
 }
 
 // 128-bit multiply-accumulate of one transformed digit with RGSW row g (both
@@ -232,79 +216,120 @@ __device__ __forceinline__ void mac_row(uint64_t (&lo)[2][8], uint64_t (&hi)[2][
   }
 }
 
-// Mux fold on NTT-form ciphertexts (slot order), for pair blockIdx.x of query
-// blockIdx.y (X = cts + i*2N, Y = X + half*2N):
-//   X_c += ext(RGSW, y - x)_c,   y - x = INTT(Y - X) (canonical coefficients).
-// The same sum as the reference's x + INTT(ext(RGSW, y - x)), kept in NTT
-// form.  Of each polynomial's l digits d_j (c = sum_j d_j 2^((l-1-j)B)
-// exactly, l B >= 60 > log2 q), only d_0 .. d_{l-2} are transformed; their
-// transforms are folded by Horner's rule into S = NTT(c >> B), and the last
-// digit's transform is NTT(d_{l-1}) = (Y - X) - 2^B S (mod q).  Per pair: two
-// inverse and 2(l - 1) forward transforms, against 2l forward and two inverse
-// in coefficient form.  Requires q = 2^60 - c with c < 2^32 and B <= 12.
+// Mux fold on NTT-form ciphertexts (slot order):
+//   X_c += ext(RGSW, y - x)_c,   y - x = INTT(Y - X) (canonical coefficients),
+// the same sum as the reference's x + INTT(ext(RGSW, y - x)), kept in NTT
+// form.  G is the level's RGSW after k_rgsw_fold_last, so of each
+// polynomial's l digits only d_0 .. d_{l-2} are transformed and the last
+// row multiplies C = Y - X itself.  Per pair: two inverse and 2(l - 1)
+// forward transforms, against 2l forward and two inverse in coefficient form.
+//
+// FIRST = false: pair (blockIdx.x, blockIdx.x + half) of query blockIdx.y in
+// cts (per-query stride cts_stride), result in place.
+// FIRST = true (the first level, straight from the matrix-core product's
+// NTT-form outputs out2[row][slot][column], column = 2 query + poly, ncols
+// columns): pair (r, r + half) of query b, result in cts row r.  The 64
+// queries of one row pair run on one XCD one after another (grid half * nq,
+// block index mod 8 = XCD), so the pair's two 2 MiB rows come from HBM once
+// and every query's 8-byte loads at the 1 KiB slot stride hit that L2.
+template <bool FIRST>
 __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, uint64_t cts_stride, uint64_t half,
+                                                               const uint64_t *out2, uint32_t ncols, uint32_t nq,
                                                                const uint64_t *G, uint64_t g_stride, uint32_t l,
                                                                uint32_t base_log2, NttTab T, Mod mq) {
   __shared__ uint64_t lds[kNttLds];
-  __shared__ uint64_t coef[kN];  // y - x of the current polynomial; slots t + 256 j of thread t
-  __shared__ uint64_t diff[kN];  // Y - X of the current polynomial; slots 8t .. 8t+7 of thread t
+  __shared__ uint64_t coef[kN];     // y - x of the current polynomial; slots t + 256 j of thread t
+  __shared__ uint64_t diff[2][kN];  // Y - X of both polynomials; slots 8t .. 8t+7 of thread t
   const int t = threadIdx.x;
-  uint64_t *x = cts + (size_t)blockIdx.y * cts_stride + (size_t)blockIdx.x * 2 * kN;
-  const uint64_t *y = x + half * 2 * kN;
-  const uint64_t *Gb = G + (size_t)blockIdx.y * g_stride;
+  uint32_t r, b;
+  if (!FIRST) {
+    r = blockIdx.x;
+    b = blockIdx.y;
+  } else if (half % 8 == 0) {
+    const uint32_t k = blockIdx.x >> 3;
+    r = 8 * (k / nq) + (blockIdx.x & 7);
+    b = k % nq;
+  } else {
+    r = blockIdx.x / nq;
+    b = blockIdx.x % nq;
+  }
+  uint64_t *x = cts + (size_t)b * cts_stride + (size_t)r * 2 * kN;
+  const uint64_t *ox = out2 + (size_t)r * kN * ncols + 2 * b;  // FIRST: X at ox[slot * ncols + poly]
+  const uint64_t *oy = ox + (size_t)half * kN * ncols;
+  const uint64_t *Gb = G + (size_t)b * g_stride;
   const uint64_t q = T.q, mask = (1ULL << base_log2) - 1;
-  const uint32_t c60 = (uint32_t)((1ULL << 60) - q);
+  // Y - X of both polynomials (canonical NTT form) into diff
+  if (FIRST) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const size_t o = (size_t)(8 * t + j) * ncols;
+      const uint4 u = *reinterpret_cast<const uint4 *>(ox + o);
+      const uint4 w = *reinterpret_cast<const uint4 *>(oy + o);
+      diff[0][8 * t + j] = sub_mod(((uint64_t)w.y << 32) | w.x, ((uint64_t)u.y << 32) | u.x, q);
+      diff[1][8 * t + j] = sub_mod(((uint64_t)w.w << 32) | w.z, ((uint64_t)u.w << 32) | u.z, q);
+    }
+  } else {
+    const uint64_t *y = x + half * 2 * kN;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      uint64_t u[8], w[8];
+      load_blocked(u, x + (size_t)c * kN);
+      load_blocked(w, y + (size_t)c * kN);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) diff[c][8 * t + j] = sub_mod(w[j], u[j], q);
+    }
+  }
   uint64_t lo[2][8], hi[2][8];
 #pragma unroll
   for (int c = 0; c < 2; ++c)
 #pragma unroll
-    for (int r = 0; r < 8; ++r) lo[c][r] = hi[c][r] = 0;
+    for (int j = 0; j < 8; ++j) lo[c][j] = hi[c][j] = 0;
 #pragma unroll 1
   for (uint32_t poly = 0; poly < 2; ++poly) {
     {
-      uint64_t v[8], w[8];
-      load_blocked(v, x + (size_t)poly * kN);
-      load_blocked(w, y + (size_t)poly * kN);
+      uint64_t v[8];
 #pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        v[r] = sub_mod(w[r], v[r], q);
-        diff[8 * t + r] = v[r];
-      }
+      for (int j = 0; j < 8; ++j) v[j] = diff[poly][8 * t + j];
       ntt_inv(v, lds, T);
 #pragma unroll
-      for (int r = 0; r < 8; ++r) coef[t + 256 * r] = ntt_inv_final(v[r], T);
+      for (int j = 0; j < 8; ++j) coef[t + 256 * j] = ntt_inv_final(v[j], T);
     }
-    uint64_t S[8];  // below 4q
-#pragma unroll
-    for (int r = 0; r < 8; ++r) S[r] = 0;
 #pragma unroll 1
     for (uint32_t kk = 0; kk + 1 < l; ++kk) {
       const uint32_t sh = (l - 1 - kk) * base_log2;
       uint64_t d[8];
 #pragma unroll
-      for (int r = 0; r < 8; ++r) d[r] = (coef[t + 256 * r] >> sh) & mask;
+      for (int j = 0; j < 8; ++j) d[j] = (coef[t + 256 * j] >> sh) & mask;
       ntt_fwd<true>(d, lds, T);  // digits < 2^base_log2
 #pragma unroll
-      for (int r = 0; r < 8; ++r) d[r] = ntt_fwd_below_4q(d[r], q);
+      for (int j = 0; j < 8; ++j) d[j] = ntt_fwd_below_4q(d[j], q);
       mac_row(lo, hi, d, Gb + (size_t)(poly * l + kk) * 2 * kN);
-#pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        uint64_t s = mul_pow2_mod(S[r], base_log2, c60) + d[r];  // < 2q + 4q
-        S[r] = s >= 4 * q ? s - 4 * q : s;
-      }
     }
     uint64_t d[8];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) d[r] = diff[8 * t + r] + 2 * q - mul_pow2_mod(S[r], base_log2, c60);  // < 3q
+    for (int j = 0; j < 8; ++j) d[j] = diff[poly][8 * t + j];
     mac_row(lo, hi, d, Gb + (size_t)(poly * l + l - 1) * 2 * kN);
+  }
+  // X + E into cts row r (FIRST reads X from out2 again).  The fence keeps
+  // these loads after the loop: hoisted, they would stay live through it.
+  asm volatile("" ::: "memory");
+  uint64_t w[2][8];
+  if (FIRST) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint4 u = *reinterpret_cast<const uint4 *>(ox + (size_t)(8 * t + j) * ncols);
+      w[0][j] = ((uint64_t)u.y << 32) | u.x;
+      w[1][j] = ((uint64_t)u.w << 32) | u.z;
+    }
+  } else {
+    load_blocked(w[0], x);
+    load_blocked(w[1], x + kN);
   }
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
-    uint64_t w[8];
-    load_blocked(w, x + (size_t)c * kN);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) w[r] = add_mod(w[r], barrett128(lo[c][r], hi[c][r], mq), q);
-    store_blocked(x + (size_t)c * kN, w);
+    for (int j = 0; j < 8; ++j) w[c][j] = add_mod(w[c][j], barrett128(lo[c][j], hi[c][j], mq), q);
+    store_blocked(x + (size_t)c * kN, w[c]);
   }
 }
 

@@ -432,20 +432,18 @@ void launch_mfma(const onion_server *s, Lane *L, const uint8_t *Bp, dim3 grid, u
   }
 }
 
-// Inverse transforms of the outputs out2 into L->fd: all rows, or
-// (DIFF) the pair differences of the first multiplexer level.  Column groups
-// of 8, 4 or 2 (ncols = 2 nq is even).
-template <bool DIFF>
+// Inverse transforms of the outputs out2 into L->fd (coefficient form).
+// Column groups of 8, 4 or 2 (ncols = 2 nq is even).
 void launch_gather(const onion_server *s, Lane *L, uint32_t nq) {
-  const uint32_t other = s->P.other_dim, ncols = 2 * nq, rows = DIFF ? other / 2 : other;
+  const uint32_t other = s->P.other_dim, ncols = 2 * nq;
   const uint32_t G = ncols % 8 == 0 ? 8 : ncols % 4 == 0 ? 4 : 2;
-  const uint32_t grid = rows * (ncols / G);
+  const uint32_t grid = other * (ncols / G);
   if (G == 8)
-    k_ntt_inv_gather<8, DIFF><<<grid, kNttThreads, 0, L->st>>>(L->digits, other, ncols, L->fd, s->fd_words, s->Tq);
+    k_ntt_inv_gather<8><<<grid, kNttThreads, 0, L->st>>>(L->digits, other, ncols, L->fd, s->fd_words, s->Tq);
   else if (G == 4)
-    k_ntt_inv_gather<4, DIFF><<<grid, kNttThreads, 0, L->st>>>(L->digits, other, ncols, L->fd, s->fd_words, s->Tq);
+    k_ntt_inv_gather<4><<<grid, kNttThreads, 0, L->st>>>(L->digits, other, ncols, L->fd, s->fd_words, s->Tq);
   else
-    k_ntt_inv_gather<2, DIFF><<<grid, kNttThreads, 0, L->st>>>(L->digits, other, ncols, L->fd, s->fd_words, s->Tq);
+    k_ntt_inv_gather<2><<<grid, kNttThreads, 0, L->st>>>(L->digits, other, ncols, L->fd, s->fd_words, s->Tq);
 }
 
 // keep_ntt: leave the outputs in out2 (NTT form) for
@@ -469,7 +467,7 @@ void run_first_dim_mfma(const onion_server *s, Lane *L, uint32_t nq, const uint6
   else
     launch_mfma<4>(s, L, Bp, grid, n_rt, ncols);
   if (time_core) (void)hipEventRecord(L->ev[7], L->st);
-  if (!keep_ntt) launch_gather<false>(s, L, nq);
+  if (!keep_ntt) launch_gather(s, L, nq);
 }
 
 // First dimension: fst selection cts per query (coefficient form, stride
@@ -513,26 +511,30 @@ void cts_to_coeff(const onion_server *s, Lane *L, uint32_t nq, uint64_t n) {
   k_cts_to_coeff<<<dim3((uint32_t)(2 * n), nq), kNttThreads, 0, L->st>>>(L->fd, s->fd_words, s->Tq);
 }
 
-// k_mux_fused_ntt's arithmetic: q = 2^60 - c with c < 2^32, digits of at most
-// 12 bits, at least two of them.
-bool mux_ntt_ok(const onion_server *s) {
-  return s->I.q < (1ULL << 60) && (1ULL << 60) - s->I.q < (1ULL << 32) && s->I.base_log2 <= 12 && s->P.gsw_l >= 2;
+// k_mux_fused_ntt transforms digits of at most 32 bits (ntt_fwd<true>) and
+// needs at least two of them.
+bool mux_ntt_ok(const onion_server *s) { return s->I.base_log2 <= 32 && s->P.gsw_l >= 2; }
+
+// The level's RGSW `g` (per query) folded for k_mux_fused_ntt.
+void fold_last(const onion_server *s, Lane *L, uint32_t nq, uint64_t *g) {
+  k_rgsw_fold_last<<<dim3(4 * kN / 256, nq), 256, 0, L->st>>>(g, s->rgsw_words, s->P.gsw_l, s->I.base_log2, s->mq);
 }
 
 // One multiplexer level over n cts per query in fd, with RGSW `g` (per-query
 // stride).  *ntt_form: the cts are in NTT form (slot order); a wide level
 // keeps them so, a narrow one (the split kernels) first brings them back to
 // coefficient form.
-void run_mux_level(const onion_server *s, Lane *L, uint32_t nq, uint64_t n, const uint64_t *g,
+void run_mux_level(const onion_server *s, Lane *L, uint32_t nq, uint64_t n, uint64_t *g,
                    bool *ntt_form = nullptr) {
   const uint32_t l = s->P.gsw_l;
   const uint64_t half = n / 2;
   const bool in_ntt = ntt_form && *ntt_form;
   if (half * nq >= s->tune.fuse_min) {
-    if (in_ntt)
-      k_mux_fused_ntt<<<dim3((uint32_t)half, nq), kNttThreads, 0, L->st>>>(L->fd, s->fd_words, half, g,
-                                                                          s->rgsw_words, l, s->I.base_log2, s->Tq,
-                                                                          s->mq);
+    if (in_ntt) {
+      fold_last(s, L, nq, g);
+      k_mux_fused_ntt<false><<<dim3((uint32_t)half, nq), kNttThreads, 0, L->st>>>(
+          L->fd, s->fd_words, half, nullptr, 0, nq, g, s->rgsw_words, l, s->I.base_log2, s->Tq, s->mq);
+    }
     else
       k_mux_fused<<<dim3((uint32_t)half, nq), kNttThreads, 0, L->st>>>(L->fd, s->fd_words, half, g, s->rgsw_words,
                                                                       l, s->I.base_log2, s->Tq, s->mq);
@@ -549,10 +551,10 @@ void run_mux_level(const onion_server *s, Lane *L, uint32_t nq, uint64_t n, cons
 }
 
 // The first multiplexer level takes the batched first dimension's NTT-form
-// outputs directly (k_ntt_inv_gather<8, true> + k_mux_first: two inverse
-// transforms per pair instead of six) and leaves NTT-form cts for the wide
-// levels after it, when the matrix-core product runs and the level is wide
-// enough for the fused external product.
+// outputs directly (k_mux_fused_ntt<true> reads both rows of a pair from
+// out2: two inverse transforms per pair instead of the reference's six) and
+// leaves NTT-form cts for the wide levels after it, when the matrix-core
+// product runs and the level is wide enough for the fused external product.
 bool mux_first_fused(const onion_server *s, const Lane *L, uint32_t nq) {
   return s->I.ndim >= 2 && use_mfma(s, L, nq) && mux_ntt_ok(s) &&
          (uint64_t)(s->P.other_dim / 2) * nq >= s->tune.fuse_min;
@@ -560,9 +562,10 @@ bool mux_first_fused(const onion_server *s, const Lane *L, uint32_t nq) {
 
 void run_mux_first(const onion_server *s, Lane *L, uint32_t nq) {
   const uint32_t half = s->P.other_dim / 2;
-  launch_gather<true>(s, L, nq);
-  k_mux_first<<<dim3(half, nq), kNttThreads, 0, L->st>>>(L->fd, s->fd_words, half, L->rgsw, s->rgsw_words,
-                                                         s->P.gsw_l, s->I.base_log2, s->Tq, s->mq);
+  fold_last(s, L, nq, L->rgsw);
+  k_mux_fused_ntt<true><<<half * nq, kNttThreads, 0, L->st>>>(L->fd, s->fd_words, half, L->digits, 2 * nq, nq,
+                                                              L->rgsw, s->rgsw_words, s->P.gsw_l, s->I.base_log2,
+                                                              s->Tq, s->mq);
 }
 
 void run_mod_switch(const onion_server *s, Lane *L, uint32_t nq, uint64_t *cts, uint64_t stride) {
