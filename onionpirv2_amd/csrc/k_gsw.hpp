@@ -198,7 +198,7 @@ __global__ __launch_bounds__(256) void k_rgsw_fold_last(uint64_t *G, uint64_t g_
   uint64_t *blk = G + (size_t)blockIdx.y * g_stride + (size_t)poly * l * 2 * kN + rest;
   const uint64_t last = blk[(size_t)(l - 1) * 2 * kN];
   for (uint32_t j = 0; j + 1 < l; ++j) {
-    const uint64_t pw = 1ULL << ((l - 1 - j) * base_log2);  // below 2^60 - B < q
+    const uint64_t pw = 1ULL << ((l - 1 - j) * base_log2);  // (l - 1) B < 60: below q
     blk[(size_t)j * 2 * kN] = sub_mod(blk[(size_t)j * 2 * kN], mulmod_barrett(last, pw, mq), mq.q);
   }
 }

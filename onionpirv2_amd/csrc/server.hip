@@ -512,8 +512,12 @@ void cts_to_coeff(const onion_server *s, Lane *L, uint32_t nq, uint64_t n) {
 }
 
 // k_mux_fused_ntt transforms digits of at most 32 bits (ntt_fwd<true>) and
-// needs at least two of them.
-bool mux_ntt_ok(const onion_server *s) { return s->I.base_log2 <= 32 && s->P.gsw_l >= 2; }
+// needs at least two of them; k_rgsw_fold_last's gadget powers
+// 2^((l-1-j)B) stay below 2^60 > q.
+bool mux_ntt_ok(const onion_server *s) {
+  const uint32_t l = s->P.gsw_l, b = s->I.base_log2;
+  return b <= 32 && l >= 2 && (uint64_t)(l - 1) * b < 60;
+}
 
 // The level's RGSW `g` (per query) folded for k_mux_fused_ntt.
 void fold_last(const onion_server *s, Lane *L, uint32_t nq, uint64_t *g) {
