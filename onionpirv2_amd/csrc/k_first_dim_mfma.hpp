@@ -126,7 +126,10 @@ __global__ __launch_bounds__(kNttThreads) void k_sel_digits(uint64_t *leaves, ui
 __global__ __launch_bounds__(256) void k_sel_chunks(const uint64_t *__restrict__ leaves, uint64_t leaf_stride,
                                                     uint8_t *__restrict__ Bp, uint32_t fst, uint32_t ncols) {
   __shared__ uint64_t buf[8][32][33];  // [slot][column][j], rows padded against bank conflicts
-  const uint32_t slot0 = 8 * blockIdx.x, cb = blockIdx.y, s = blockIdx.z, ncb = gridDim.y;
+  // slot groups 2m and 2m + 1 read the two halves of the same 128-byte lines:
+  // blocks L and L + 8 (same XCD, dispatched together) take them
+  const uint32_t L = blockIdx.x, sgi = (L >> 4) * 16 + (L & 7) * 2 + ((L >> 3) & 1);
+  const uint32_t slot0 = 8 * sgi, cb = blockIdx.y, s = blockIdx.z, ncb = gridDim.y;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const uint32_t piece = threadIdx.x + 256 * k, li = piece / 32, jj = piece % 32, col = 32 * cb + li;
