@@ -235,6 +235,36 @@ def test_decomposition_recomposes_exactly():
                 assert sum(int(d[poly * l + j, i]) * gad[j] for j in range(l)) == int(ct[poly * o.N + i])
 
 
+def test_external_product_with_folded_last_digit():
+    """The identity the device's NTT-form multiplexer relies on: with each
+    polynomial's RGSW rows j < l-1 replaced by G_j - 2^((l-1-j)B) G_{l-1}
+    (mod q), the last digit's transform is never needed,
+      sum_j NTT(d_j) G_j = sum_{j<l-1} NTT(d_j) G'_j + NTT(c) G_{l-1}  (mod q),
+    because c = sum_j d_j 2^((l-1-j)B) exactly.  Checked against the oracle's
+    external product on a random ciphertext, in Python integers."""
+    o = Oracle()
+    q, N, l, B = o.q, o.N, o.l, o.base_log2
+    sk = o.keygen(43)
+    g = o.rgsw(43, 5, sk, 1).reshape(2 * l, 2, N).astype(object)
+    rng = np.random.default_rng(12)
+    ct = rng.integers(0, q, size=2 * N, dtype=np.uint64)
+    ct[:2] = [q - 1, 0]
+    exp = u64(2 * N)
+    o.L.orc_external_product(o.c, ptr(g.astype(np.uint64).reshape(-1).copy()), l, B, ptr(ct.copy()), ptr(exp))
+    d = u64(2 * l * N)
+    o.L.orc_decompose(o.c, ptr(ct.copy()), l, B, ptr(d))
+    d = d.reshape(2 * l, N)
+    acc = np.zeros((2, N), dtype=object)
+    for poly in range(2):
+        last = g[poly * l + l - 1]
+        for j in range(l - 1):
+            folded = (g[poly * l + j] - (2 ** ((l - 1 - j) * B)) * last) % q
+            acc += o.ntt(d[poly * l + j]).astype(object) * folded
+        acc += o.ntt(ct[poly * N:(poly + 1) * N]).astype(object) * last
+    got = (acc % q).astype(np.uint64).reshape(-1)
+    assert np.array_equal(got, exp)
+
+
 def test_external_product_selects():
     o = Oracle()
     sk = o.keygen(41)
