@@ -41,9 +41,10 @@
 //             row and converts them in registers
 //   output    out2[r][l][c] (uint64, canonical mod q, c < 2*nq): a tile's
 //             32 columns of one (row, slot) are 256 contiguous bytes, and one
-//             row's 2048 slots are one contiguous 2 MiB run (2 KiB per slot at
-//             128 columns) for k_ntt_inv_gather, which transforms them into
-//             coefficient-form ciphertexts.
+//             row's 2048 slots are one contiguous 2 MiB run (1 KiB per slot at
+//             128 columns), read by the first multiplexer level
+//             (k_mux_fused_ntt<true>, in k_gsw.hpp) or by k_ntt_inv_gather,
+//             which transforms the rows into coefficient-form ciphertexts.
 #pragma once
 
 #include "k_first_dim.hpp"

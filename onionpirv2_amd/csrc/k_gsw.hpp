@@ -13,6 +13,10 @@
 //                    sum_j D_j * G_j[comp] in 128-bit, Barrett mod q, then
 //                    either store (conversion, NTT form) or INTT and add into
 //                    the mux accumulator (other-dimension fold).
+// Fused variants run one workgroup per ciphertext (k_mux_fused,
+// k_convert_fused); k_mux_fused_ntt folds pairs of NTT-form ciphertexts with
+// a level RGSW prepared by k_rgsw_fold_last, and k_cts_to_coeff returns
+// NTT-form ciphertexts to coefficient form.
 // grid.y is the query index; every buffer has a per-query stride.
 #pragma once
 
