@@ -45,7 +45,9 @@ __device__ __forceinline__ uint64_t galois_gather(const uint64_t *in, int j, uin
 
 // trees: query b's tree at trees + b*tree_stride (2w cts, slot s at s*2N)
 // keys:  query b's Galois keys at key_ptrs[b]; this level's key at + level*4N,
-//        laid out [comp][limb q,p][N], NTT form
+//        laid out [comp][limb q,p][N], NTT form, each limb already scaled by
+//        N^-1 (k_key_scale at upload), so the inverse transforms of the key
+//        products need no scaling of their own
 // xbuf:  query b's X at xbuf + b*x_stride, [node - first][comp][limb][N]
 __global__ __launch_bounds__(kNttThreads) void k_expand_ks(const uint64_t *trees, uint64_t tree_stride,
                                                            const uint64_t *const *key_ptrs, uint32_t level,
@@ -70,7 +72,7 @@ __global__ __launch_bounds__(kNttThreads) void k_expand_ks(const uint64_t *trees
   ntt_inv(x, lds, T);
   uint64_t *dst = xbuf + (size_t)blockIdx.y * x_stride + ((size_t)(nl * 2 + comp) * 2 + limb) * kN;
 #pragma unroll
-  for (int j = 0; j < 8; ++j) dst[threadIdx.x + 256 * j] = ntt_inv_final(x[j], T);
+  for (int j = 0; j < 8; ++j) dst[threadIdx.x + 256 * j] = ntt_inv_canonical(x[j], T.q);
 }
 
 // The rounding division of the key switch and the children for one
@@ -133,14 +135,15 @@ __device__ __forceinline__ void expand_node_comp(int comp, const uint64_t (&pq)[
   for (int j = 0; j < 8; ++j) x[j] = mulmod_barrett(pq[8 * t + j], kv[j], mq);
   ntt_inv(x, lds, Tq);
 #pragma unroll
-  for (int j = 0; j < 8; ++j) xq[j] = ntt_inv_final(x[j], Tq);
+  for (int j = 0; j < 8; ++j) xq[j] = ntt_inv_canonical(x[j], Tq.q);
   load_blocked(kv, key + (size_t)(2 * comp + 1) * kN);
 #pragma unroll
   for (int j = 0; j < 8; ++j) x[j] = mulmod_barrett(pp[8 * t + j], kv[j], mp);
   ntt_inv(x, lds, Tp);
 #pragma unroll
   for (int j = 0; j < 8; ++j)
-    expand_child(left, right, c0, comp, t + 256 * j, xq[j], ntt_inv_final(x[j], Tp), k, ginv, Tq.q, Tp.q, ks);
+    expand_child(left, right, c0, comp, t + 256 * j, xq[j], ntt_inv_canonical(x[j], Tp.q), k, ginv, Tq.q, Tp.q,
+                 ks);
 }
 
 // One level in one launch, one workgroup per node: both components share the
@@ -182,6 +185,16 @@ __global__ __launch_bounds__(kNttThreads) void k_expand_node(uint64_t *trees, ui
   expand_node_comp(0, park[0], park[1], key, lds, left, right, c0, k, ginv, Tq, Tp, mq, mp, ks);
   asm volatile("" ::: "memory");
   expand_node_comp(1, park[0], park[1], key, lds, left, right, c0, k, ginv, Tq, Tp, mq, mp, ks);
+}
+
+// Galois keys as stored: every value of limb q (p) times N^-1 mod q (mod p).
+// Layout [level][comp][limb][N]; grid = n_words / 256.
+__global__ __launch_bounds__(256) void k_key_scale(uint64_t *keys, uint64_t n_words, uint64_t nq_inv, uint64_t np_inv,
+                                                   Mod mq, Mod mp) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_words) return;
+  const bool limb_p = (i / kN) & 1;
+  keys[i] = limb_p ? mulmod_barrett(keys[i], np_inv, mp) : mulmod_barrett(keys[i], nq_inv, mq);
 }
 
 }  // namespace onion

@@ -243,6 +243,10 @@ __device__ __forceinline__ uint64_t ntt_fwd_below_4q(uint64_t v, uint64_t q) {
   return v;
 }
 
+// Canonical value of an inverse-NTT output in [0, 2q), not scaled (for inputs
+// that carry the N^-1 factor already).
+__device__ __forceinline__ uint64_t ntt_inv_canonical(uint64_t v, uint64_t q) { return v >= q ? v - q : v; }
+
 // Canonical value of an inverse-NTT output in [0, 2q), scaled by N^-1.
 __device__ __forceinline__ uint64_t ntt_inv_final(uint64_t v, const NttTab &T) {
   const uint64_t r = mul_shoup_lazy(v, T.n_inv, T.n_inv_s, T.q);

@@ -1115,6 +1115,9 @@ int onion_gpu_set_galois_key(onion_server *s, uint64_t id, const uint64_t *keys,
   }
   ck.gk_levels = (uint32_t)(n / level_words);
   HIPCHK(hipMemcpyAsync(ck.gk, keys, n * 8, hipMemcpyHostToDevice, st));
+  // stored times N^-1 per limb: the expansion's inverse transforms then need no scaling
+  k_key_scale<<<(uint32_t)((n + 255) / 256), 256, 0, st>>>(ck.gk, n, s->Tq.n_inv, s->Tp.n_inv, s->mq, s->mp);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(st));
   return ONION_OK;
 }
