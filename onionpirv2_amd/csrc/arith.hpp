@@ -67,9 +67,9 @@ __device__ __forceinline__ void add128(uint64_t &lo, uint64_t &hi, uint64_t blo,
   hi += bhi + (lo < blo);
 }
 
-// (lo, hi) += a * b for a, b < 2^62 (both high words below 2^30), as three
-// 32x32 partial products: the middle pair sums below 2^63 without a carry,
-// the low product carries into hi once.
+// (lo, hi) += a * b for a < 2^63 and b < 2^60 (high words below 2^31 and
+// 2^28), as three 32x32 partial products: the middle pair sums below
+// 2^63 + 2^60 without a carry, the low product carries into hi once.
 __device__ __forceinline__ void mac128(uint64_t &lo, uint64_t &hi, uint64_t a, uint64_t b) {
   const uint32_t al = (uint32_t)a, ah = (uint32_t)(a >> 32), bl = (uint32_t)b, bh = (uint32_t)(b >> 32);
   const uint64_t ll = (uint64_t)al * bl;

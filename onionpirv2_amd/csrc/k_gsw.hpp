@@ -7,7 +7,7 @@
 // Split in two launches so that every digit NTT of every ciphertext of every
 // query in the batch runs in parallel:
 //   k_decomp_ntt  -- one workgroup per (ciphertext, digit): extract the digit
-//                    polynomial, NTT it, store it (NTT form, below 4q:
+//                    polynomial, NTT it, store it (NTT form, below 8q:
 //                    the 128-bit sums reduce it exactly);
 //   k_ext_mac_*   -- one workgroup per (ciphertext, output component):
 //                    sum_j D_j * G_j[comp] in 128-bit, Barrett mod q, then
@@ -52,7 +52,7 @@ __global__ __launch_bounds__(kNttThreads) void k_decomp_ntt(const uint64_t *src_
   }
   ntt_fwd<true>(x, lds, T);  // digits < 2^base_log2 <= 2^32
 #pragma unroll
-  for (int r = 0; r < 8; ++r) x[r] = ntt_fwd_below_4q(x[r], T.q);  // only ext_mac8 reads D
+  for (int r = 0; r < 8; ++r) x[r] = ntt_fwd_below_8q(x[r], T.q);  // only ext_mac8 reads D
   store_blocked(D + (size_t)blockIdx.y * d_stride + ((size_t)ct * rows + j) * kN, x);
 }
 
@@ -144,7 +144,7 @@ __device__ __forceinline__ void ext_fused_core(const uint64_t *xs, const uint64_
       ntt_fwd<true>(d, lds, T);  // digits < 2^base_log2 <= 2^32
       const uint64_t *g = G + (size_t)(poly * l + kk) * 2 * kN;
 #pragma unroll
-      for (int r = 0; r < 8; ++r) d[r] = ntt_fwd_below_4q(d[r], T.q);
+      for (int r = 0; r < 8; ++r) d[r] = ntt_fwd_below_8q(d[r], T.q);
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         uint64_t gv[8];
@@ -306,7 +306,7 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
       for (int j = 0; j < 8; ++j) d[j] = (coef[t + 256 * j] >> sh) & mask;
       ntt_fwd<true>(d, lds, T);  // digits < 2^base_log2
 #pragma unroll
-      for (int j = 0; j < 8; ++j) d[j] = ntt_fwd_below_4q(d[j], q);
+      for (int j = 0; j < 8; ++j) d[j] = ntt_fwd_below_8q(d[j], q);
       mac_row(lo, hi, d, Gb + (size_t)(poly * l + kk) * 2 * kN);
     }
     uint64_t d[8];

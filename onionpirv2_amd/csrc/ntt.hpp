@@ -234,14 +234,11 @@ __device__ __forceinline__ uint64_t ntt_fwd_final(uint64_t v, uint64_t q) {
   return v;
 }
 
-// A forward-NTT output in [0, 16q) brought below 4q (< 2^62), enough for the
-// 128-bit multiply-accumulate of external products (mac128), whose Barrett
-// reduction at the end returns the canonical residue.
-__device__ __forceinline__ uint64_t ntt_fwd_below_4q(uint64_t v, uint64_t q) {
-  if (v >= 8 * q) v -= 8 * q;
-  if (v >= 4 * q) v -= 4 * q;
-  return v;
-}
+// A forward-NTT output in [0, 16q) brought below 8q (< 2^63), enough for the
+// 128-bit multiply-accumulate of external products (mac128 with a canonical
+// second operand), whose Barrett reduction at the end returns the canonical
+// residue.
+__device__ __forceinline__ uint64_t ntt_fwd_below_8q(uint64_t v, uint64_t q) { return v >= 8 * q ? v - 8 * q : v; }
 
 // Canonical value of an inverse-NTT output in [0, 2q), not scaled (for inputs
 // that carry the N^-1 factor already).
