@@ -261,12 +261,13 @@ def test_persistent_first_dim_matches_oracle(fst, other, nq):
         assert np.array_equal(srv.tap_first_dim_batch(sel), got)
 
 
-@pytest.mark.parametrize("shape", [(64, 4), (256, 16)])
+@pytest.mark.parametrize("shape", [(64, 2), (64, 4), (256, 16)])
 def test_first_mux_level_from_ntt_form(shape):
     """A batch whose first multiplexer level runs from the matrix-core
     product's NTT-form outputs (fuse_min = 1) answers as the oracle does, and
     as the path that transforms every first-dimension output first
-    (fuse_min = 2^30).  half = 2 and 8 take both workgroup-to-XCD mappings.
+    (fuse_min = 2^30).  half = 1, 2 and 8 take both workgroup-to-XCD mappings;
+    at other = 2 the first level is the only one.
     The levels after it keep NTT-form ciphertexts while they are wide
     (k_mux_fused_ntt) and return to coefficient form for the narrow split
     kernels: fuse_min = 5 / 10 / 20 put that switch after the first, second
