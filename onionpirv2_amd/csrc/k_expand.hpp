@@ -29,6 +29,8 @@ struct KsConst {
   uint64_t half_p_mod_q;  // floor(p/2) mod q
   uint64_t p_inv;         // p^-1 mod q
   uint64_t p_inv_s;       // Shoup companion of p_inv
+  uint64_t np_inv;        // N p^-1 mod q (k_expand_leaf_ntt)
+  uint64_t np_inv_s;      // Shoup companion of np_inv
 };
 
 // Galois permutation sigma_g of a coefficient-form polynomial (SEAL
@@ -185,6 +187,111 @@ __global__ __launch_bounds__(kNttThreads) void k_expand_node(uint64_t *trees, ui
   expand_node_comp(0, park[0], park[1], key, lds, left, right, c0, k, ginv, Tq, Tp, mq, mp, ks);
   asm volatile("" ::: "memory");
   expand_node_comp(1, park[0], park[1], key, lds, left, right, c0, k, ginv, Tq, Tp, mq, mp, ks);
+}
+
+// slot of NTT(a) holding sigma_g(a)'s slot s: slot s holds a(psi^o(s)) with
+// o(s) = 2 bitrev(s) + 1, and sigma_g(a)(psi^o) = a(psi^(g o)), so the source
+// is the slot with o = g o(s) mod 2N (odd: its (o - 1) / 2 is o >> 1)
+__device__ __forceinline__ uint32_t galois_slot(uint32_t s, uint32_t g) {
+  const uint32_t o = 2 * (__brev(s) >> 21) + 1;  // 11-bit reversal (N = 2048)
+  return __brev(((g * o) & (2 * kN - 1)) >> 1) >> 21;
+}
+
+// One component of a last-level node in NTT form (k_expand_leaf_ntt).  P: the
+// thread's slots of NTT_p(sigma(c1)); Q[0], Q[1]: NTT_q(c0), NTT_q(c1).
+__device__ __forceinline__ void leaf_ntt_comp(int comp, const uint64_t (&P)[8], const uint64_t (&Q)[2][kN],
+                                              const uint64_t *key, const uint64_t *xk, uint32_t g, uint64_t *lds,
+                                              uint64_t *left, uint64_t *right, const NttTab &Tq, const NttTab &Tp,
+                                              const Mod &mq, const Mod &mp, const KsConst &ks) {
+  const int t = threadIdx.x;
+  const uint64_t q = Tq.q, p = Tp.q;
+  uint64_t x[8], kv[8];
+  // the p-limb product in coefficient form: the rounding term of the key switch
+  load_blocked(kv, key + (size_t)(2 * comp + 1) * kN);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) x[j] = mulmod_barrett(P[j], kv[j], mp);
+  ntt_inv(x, lds, Tp);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    // expand_child's division, rearranged: KS = (X_q - corr) p^-1 with
+    // corr = ((X_p + floor(p/2)) mod p) mod q - floor(p/2) mod q, taken
+    // here times N^-1 (the q-limb key carries N^-1)
+    uint64_t last = ntt_inv_canonical(x[j], p) + ks.half_p;
+    if (last >= p) last -= p;
+    const uint64_t lq = last >= q ? last - q : last;  // p < 2q
+    x[j] = mul_shoup_lazy(sub_mod(lq, ks.half_p_mod_q, q), Tq.n_inv, Tq.n_inv_s, q);  // < 2q
+  }
+  ntt_fwd(x, lds, Tq);
+  // slot pairs 8t + j, 8t + j + 1: 16-byte loads and stores
+  const uint4 *kq = reinterpret_cast<const uint4 *>(key + (size_t)(2 * comp) * kN + 8 * t);
+  uint4 *lo = reinterpret_cast<uint4 *>(left + (size_t)comp * kN + 8 * t);
+  uint4 *ro = reinterpret_cast<uint4 *>(right + (size_t)comp * kN + 8 * t);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint4 kp = kq[i];
+    uint64_t lv[2], rv[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int j = 2 * i + h;
+      const uint32_t s = 8 * t + j, src = galois_slot(s, g);
+      const uint64_t kj = h ? ((uint64_t)kp.w << 32) | kp.z : ((uint64_t)kp.y << 32) | kp.x;
+      // N^-1 NTT(X_q) = NTT_q(sigma(c1)) * key_q (as stored)
+      const uint64_t xq = mulmod_barrett(Q[1][src], kj, mq);
+      uint64_t ksv = mul_shoup_lazy(sub_mod(xq, ntt_fwd_final(x[j], q), q), ks.np_inv, ks.np_inv_s, q);
+      if (ksv >= q) ksv -= q;
+      // c'_0 = sigma(c0) + KS_0, c'_1 = KS_1
+      const uint64_t cp = comp == 0 ? add_mod(Q[0][src], ksv, q) : ksv;
+      const uint64_t a = Q[comp][s];
+      lv[h] = add_mod(a, cp, q);
+      const uint64_t r = mul_shoup_lazy(sub_mod(a, cp, q), xk[s], xk[kN + s], q);  // times x^-k
+      rv[h] = r >= q ? r - q : r;
+    }
+    lo[i] = make_uint4((uint32_t)lv[0], (uint32_t)(lv[0] >> 32), (uint32_t)lv[1], (uint32_t)(lv[1] >> 32));
+    ro[i] = make_uint4((uint32_t)rv[0], (uint32_t)(rv[0] >> 32), (uint32_t)rv[1], (uint32_t)(rv[1] >> 32));
+  }
+}
+
+// The last expansion level with NTT-form children (slot order, canonical), one
+// workgroup per node: the form the selection vectors and the conversion's top
+// rows take, so their forward transforms disappear.  In NTT form sigma_g is a
+// slot permutation (galois_slot) and x^-k the factor psi^(-k o(s)) of slot s
+// (xk: N factors, then N Shoup companions).  A node transforms c0, c1 (mod q)
+// and sigma(c1) (mod p) once each; the q-limb key products need no transform;
+// each component's p-limb product returns to coefficient form for the
+// rounding division and its correction is transformed forward: 3 + 2 + 2
+// transforms, where k_expand_node takes 6 and the children's selection
+// vectors 4 more.  The node's own residues in coefficient form go unchanged
+// into the rounding, so the children equal k_expand_node's, transformed.
+__global__ __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_expand_leaf_ntt(uint64_t *trees, uint64_t tree_stride,
+                                                                 const uint64_t *const *key_ptrs, uint32_t level,
+                                                                 uint32_t g, uint32_t ginv, const uint64_t *xk,
+                                                                 NttTab Tq, NttTab Tp, Mod mq, Mod mp, KsConst ks) {
+  __shared__ uint64_t lds[kNttLds];
+  __shared__ uint64_t Q[2][kN];  // NTT_q(c0), NTT_q(c1), canonical; read at permuted slots
+  const uint32_t nl = blockIdx.x, k = 1u << level;
+  const uint64_t node = (uint64_t)k + nl;
+  uint64_t *tree = trees + (size_t)blockIdx.y * tree_stride;
+  const uint64_t *c = tree + node * 2 * kN;
+  const uint64_t *key = key_ptrs[blockIdx.y] + (size_t)level * 4 * kN;
+  const int t = threadIdx.x;
+  uint64_t x[8];
+#pragma unroll 1
+  for (int comp = 0; comp < 2; ++comp) {
+    load_natural(x, c + (size_t)comp * kN);
+    ntt_fwd(x, lds, Tq);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) Q[comp][8 * t + j] = ntt_fwd_final(x[j], Tq.q);
+  }
+  // sigma(c1) mod p; its residues < q < p serve unchanged
+  uint64_t P[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) P[j] = galois_gather(c + kN, t + 256 * j, ginv, Tq.q);
+  ntt_fwd(P, lds, Tp);  // its barriers also order the Q writes before the reads below
+  // the children are other tree slots: every read of c precedes these writes
+  uint64_t *left = tree + (2 * node) * 2 * kN, *right = tree + (2 * node + 1) * 2 * kN;
+  leaf_ntt_comp(0, P, Q, key, xk, g, lds, left, right, Tq, Tp, mq, mp, ks);
+  asm volatile("" ::: "memory");
+  leaf_ntt_comp(1, P, Q, key, xk, g, lds, left, right, Tq, Tp, mq, mp, ks);
 }
 
 // Galois keys as stored: every value of limb q (p) times N^-1 mod q (mod p).

@@ -33,19 +33,26 @@ __device__ __forceinline__ uint64_t mad32(uint32_t a, uint32_t b, uint64_t c) {
 }
 
 // Selection vectors: NTT of poly c of leaf j of query b (grid = fst*2 x B),
-// scattered as limbs into Qs[l][j][2b + c].
+// scattered as limbs into Qs[l][j][2b + c]; NTT_IN: the leaves are in NTT
+// form already (slot order, canonical).
+template <bool NTT_IN>
 __global__ __launch_bounds__(kNttThreads) void k_sel_ntt(const uint64_t *leaves, uint64_t leaf_stride,
                                                          uint32_t *Qs, uint32_t fst, uint32_t cpad,
                                                          NttTab T) {
   __shared__ uint64_t lds[kNttLds];
   const uint32_t j = blockIdx.x / 2, c = blockIdx.x % 2, b = blockIdx.y;
+  const uint64_t *src = leaves + (size_t)b * leaf_stride + (size_t)j * 2 * kN + (size_t)c * kN;
   uint64_t x[8];
-  load_natural(x, leaves + (size_t)b * leaf_stride + (size_t)j * 2 * kN + (size_t)c * kN);
-  ntt_fwd(x, lds, T);
+  if (NTT_IN) {
+    load_blocked(x, src);
+  } else {
+    load_natural(x, src);
+    ntt_fwd(x, lds, T);
+  }
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
     const int e = 8 * threadIdx.x + r;
-    const uint64_t v = ntt_fwd_final(x[r], T.q);
+    const uint64_t v = NTT_IN ? x[r] : ntt_fwd_final(x[r], T.q);
     uint2 lim;
     lim.x = (uint32_t)v & kMask30;
     lim.y = (uint32_t)(v >> 30);

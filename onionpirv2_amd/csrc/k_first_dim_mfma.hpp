@@ -106,7 +106,9 @@ __device__ __forceinline__ void a_planes(const uint4 *a, uint32_t (&out)[16]) {
 // contiguous lines (a single pass owning 8 leaves of one column writes 8-byte
 // pieces 1 KiB apart, and measured 4.3x its bytes in write traffic).
 //   k_sel_digits  NTT of each leaf polynomial, in place, in signed-digit
-//                 form (slot order).  grid = (fst * 2, nq).
+//                 form (slot order).  grid = (fst * 2, nq).  Skipped when
+//                 the leaves are in NTT form already (k_expand_leaf_ntt):
+//                 k_sel_chunks<true> then takes canonical residues.
 //   k_sel_chunks  8 slots x one 32-column block x one step per workgroup:
 //                 reads each (column, j)'s 8 slots (64 contiguous bytes) into
 //                 LDS, then each thread turns 16 values of j of one column and
@@ -124,6 +126,7 @@ __global__ __launch_bounds__(kNttThreads) void k_sel_digits(uint64_t *leaves, ui
   store_blocked(p, x);
 }
 
+template <bool CANON>
 __global__ __launch_bounds__(256) void k_sel_chunks(const uint64_t *__restrict__ leaves, uint64_t leaf_stride,
                                                     uint8_t *__restrict__ Bp, uint32_t fst, uint32_t ncols) {
   __shared__ uint64_t buf[8][32][33];  // [slot][column][j], rows padded against bank conflicts
@@ -140,8 +143,9 @@ __global__ __launch_bounds__(256) void k_sel_chunks(const uint64_t *__restrict__
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const uint4 v = src[q];
-        buf[2 * q][li][jj] = ((uint64_t)v.y << 32) | v.x;
-        buf[2 * q + 1][li][jj] = ((uint64_t)v.w << 32) | v.z;
+        const uint64_t a = ((uint64_t)v.y << 32) | v.x, b = ((uint64_t)v.w << 32) | v.z;
+        buf[2 * q][li][jj] = CANON ? signed_digits(a) : a;
+        buf[2 * q + 1][li][jj] = CANON ? signed_digits(b) : b;
       }
     }
   }

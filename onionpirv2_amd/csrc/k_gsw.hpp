@@ -368,7 +368,9 @@ __global__ __launch_bounds__(kNttThreads) void k_convert_fused(const uint64_t *l
 }
 
 // Conversion rows 0..l-1: NTT of both polys of leaf c into
-//   out + (c / l) * 2l * 2N + (c % l) * 2N + poly * N   (per query)
+//   out + (c / l) * 2l * 2N + (c % l) * 2N + poly * N   (per query);
+// NTT_IN: the leaves are in NTT form already (a copy).
+template <bool NTT_IN>
 __global__ __launch_bounds__(kNttThreads) void k_rgsw_top_rows(const uint64_t *leaves, uint64_t leaf_stride,
                                                                uint64_t *out, uint64_t out_stride,
                                                                uint32_t l, NttTab T) {
@@ -379,10 +381,14 @@ __global__ __launch_bounds__(kNttThreads) void k_rgsw_top_rows(const uint64_t *l
   uint64_t *dst = out + (size_t)blockIdx.y * out_stride + (ct / l) * (2 * l) * 2 * kN + (ct % l) * 2 * kN +
                   (size_t)poly * kN;
   uint64_t x[8];
-  load_natural(x, src);
-  ntt_fwd(x, lds, T);
+  if (NTT_IN) {
+    load_blocked(x, src);
+  } else {
+    load_natural(x, src);
+    ntt_fwd(x, lds, T);
 #pragma unroll
-  for (int r = 0; r < 8; ++r) x[r] = ntt_fwd_final(x[r], T.q);
+    for (int r = 0; r < 8; ++r) x[r] = ntt_fwd_final(x[r], T.q);
+  }
   store_blocked(dst, x);
 }
 

@@ -141,13 +141,15 @@ struct onion_server {
   int device = 0;
   uint32_t n_cu = 0;  // compute units of the device
   uint64_t w = 0;  // 2^h
-  // device NTT tables: q then p, each rp rps irp irps (4N words)
+  // device NTT tables: q then p, each rp rps irp irps (4N words); then the
+  // last expansion level's x^-k slot factors and their Shoup companions (2N)
   uint64_t *tabs = nullptr;
+  const uint64_t *xk = nullptr;
   NttTab Tq{}, Tp{};
   Mod mq{}, mp{};
   KsConst ks{};
   LiftConst lift{};
-  std::vector<uint32_t> ginv;  // inverse of each level's Galois element mod 2N
+  std::vector<uint32_t> gal, ginv;  // each level's Galois element N/2^a + 1, its inverse mod 2N
   uint64_t ms_M = 0;
   uint32_t ms_f = 0;
   // database, and its signed-digit plane copy (tune.db_planes)
@@ -342,36 +344,56 @@ int check_db(const onion_server *s) {
 // ---- pipeline stages (nq queries; query b's ciphertext at tree b, slot 1) ------------
 
 // Nodes 2^a .. 2^a + n_active - 1 of every query's tree; X scratch in digits.
-// Levels with at least tune.fuse_min nodes across the batch run fused.
-void expand_level(const onion_server *s, Lane *L, uint32_t nq, uint32_t a, uint32_t n_active) {
+// Levels with at least tune.fuse_min nodes across the batch run fused; the
+// last level (`leaf`), fused, writes NTT-form children.  Returns whether the
+// children are in NTT form.
+bool expand_level(const onion_server *s, Lane *L, uint32_t nq, uint32_t a, uint32_t n_active, bool leaf = false) {
   if ((uint64_t)n_active * nq >= s->tune.fuse_min) {
+    if (leaf) {
+      k_expand_leaf_ntt<<<dim3(n_active, nq), kNttThreads, 0, L->st>>>(
+          L->trees, s->tree_words, L->gk_ptrs, a, s->gal[a], s->ginv[a], s->xk, s->Tq, s->Tp, s->mq, s->mp, s->ks);
+      return true;
+    }
     k_expand_node<<<dim3(n_active, nq), kNttThreads, 0, L->st>>>(L->trees, s->tree_words, L->gk_ptrs, a,
                                                                  s->ginv[a], s->Tq, s->Tp, s->mq, s->mp, s->ks);
-    return;
+    return false;
   }
   k_expand_ks<<<dim3(n_active * 4, nq), kNttThreads, 0, L->st>>>(
       L->trees, s->tree_words, L->gk_ptrs, a, s->ginv[a], L->digits, s->digit_words, s->Tq, s->Tp, s->mq, s->mp);
   k_expand_combine<<<dim3(n_active, nq), 256, 0, L->st>>>(L->trees, s->tree_words, L->digits, s->digit_words, a,
                                                           s->ginv[a], s->I.q, s->I.p, s->ks);
+  return false;
 }
 
-void run_expand(const onion_server *s, Lane *L, uint32_t nq) {
+// The expansion tree of each query; returns whether the leaves are in NTT form.
+bool run_expand(const onion_server *s, Lane *L, uint32_t nq) {
   const uint32_t h = s->I.expan_height, u = s->I.useful_cnt;
+  bool leaves_ntt = false;
   for (uint32_t a = 0; a < h; ++a) {
     // active nodes at level a: those whose left-most leaf is < u
     const uint64_t span = s->w >> a;  // leaves under one node
     const uint32_t n_active = (uint32_t)((u + span - 1) / span);
-    expand_level(s, L, nq, a, n_active);
+    leaves_ntt = expand_level(s, L, nq, a, n_active, a + 1 == h);
   }
+  return leaves_ntt;
 }
 
 // Conversion of the (ndim-1)*l RGSW leaves starting at conv (stride per query).
-void run_convert(const onion_server *s, Lane *L, uint32_t nq, const uint64_t *conv, uint64_t conv_stride) {
+// NTT-form leaves (leaves_ntt) are the top rows as they are, and are then
+// brought back to coefficient form, in place, for the digit decomposition.
+void run_convert(const onion_server *s, Lane *L, uint32_t nq, uint64_t *conv, uint64_t conv_stride,
+                 bool leaves_ntt = false) {
   const uint32_t nd = s->I.ndim - 1, l = s->P.gsw_l, lk = s->P.gsw_l_key;
   if (nd == 0) return;
   const uint32_t n_ct = nd * l;
-  k_rgsw_top_rows<<<dim3(n_ct * 2, nq), kNttThreads, 0, L->st>>>(conv, conv_stride, L->rgsw, s->rgsw_words, l,
-                                                                 s->Tq);
+  if (leaves_ntt) {
+    k_rgsw_top_rows<true><<<dim3(n_ct * 2, nq), kNttThreads, 0, L->st>>>(conv, conv_stride, L->rgsw,
+                                                                         s->rgsw_words, l, s->Tq);
+    k_cts_to_coeff<<<dim3(n_ct * 2, nq), kNttThreads, 0, L->st>>>(conv, conv_stride, s->Tq);
+  } else {
+    k_rgsw_top_rows<false><<<dim3(n_ct * 2, nq), kNttThreads, 0, L->st>>>(conv, conv_stride, L->rgsw,
+                                                                          s->rgsw_words, l, s->Tq);
+  }
   if ((uint64_t)n_ct * nq >= s->tune.fuse_min) {
     k_convert_fused<<<dim3(n_ct, nq), kNttThreads, 0, L->st>>>(conv, conv_stride, L->rk_ptrs, lk,
                                                                s->I.base_log2_key, L->rgsw, s->rgsw_words, l,
@@ -447,15 +469,20 @@ void launch_gather(const onion_server *s, Lane *L, uint32_t nq) {
 }
 
 // keep_ntt: leave the outputs in out2 (NTT form) for
-// run_mux_first instead of transforming them into L->fd.
-void run_first_dim_mfma(const onion_server *s, Lane *L, uint32_t nq, const uint64_t *sel, uint64_t sel_stride,
-                        bool time_core, bool keep_ntt) {
+// run_mux_first instead of transforming them into L->fd.  sel_ntt: the
+// selection vectors are in NTT form already.
+void run_first_dim_mfma(const onion_server *s, Lane *L, uint32_t nq, uint64_t *sel, uint64_t sel_stride,
+                        bool time_core, bool keep_ntt, bool sel_ntt) {
   const uint32_t fst = s->P.fst_dim, other = s->P.other_dim, ncp = cpad_for(nq), ncols = 2 * nq;
   uint8_t *Bp = reinterpret_cast<uint8_t *>(L->Qs);
-  // the leaves are transformed in place: the selection vectors have no other reader
-  uint64_t *leaves = const_cast<uint64_t *>(sel);
-  k_sel_digits<<<dim3(fst * 2, nq), kNttThreads, 0, L->st>>>(leaves, sel_stride, s->Tq);
-  k_sel_chunks<<<dim3(kN / 8, ncp / 32, fst / 32), 256, 0, L->st>>>(leaves, sel_stride, Bp, fst, ncols);
+  const dim3 chunk_grid(kN / 8, ncp / 32, fst / 32);
+  if (sel_ntt) {
+    k_sel_chunks<true><<<chunk_grid, 256, 0, L->st>>>(sel, sel_stride, Bp, fst, ncols);
+  } else {
+    // the leaves are transformed in place: the selection vectors have no other reader
+    k_sel_digits<<<dim3(fst * 2, nq), kNttThreads, 0, L->st>>>(sel, sel_stride, s->Tq);
+    k_sel_chunks<false><<<chunk_grid, 256, 0, L->st>>>(sel, sel_stride, Bp, fst, ncols);
+  }
   if (time_core) (void)hipEventRecord(L->ev[6], L->st);
   const uint32_t rb = 128 / ncp;  // row blocks of 32 per workgroup
   const uint32_t n_rt = (other + 32 * rb - 1) / (32 * rb);
@@ -470,17 +497,21 @@ void run_first_dim_mfma(const onion_server *s, Lane *L, uint32_t nq, const uint6
   if (!keep_ntt) launch_gather(s, L, nq);
 }
 
-// First dimension: fst selection cts per query (coefficient form, stride
-// sel_stride) -> L->fd (other cts per query, coefficient form).
-void run_first_dim(const onion_server *s, Lane *L, uint32_t nq, const uint64_t *sel, uint64_t sel_stride,
-                   bool time_core, bool keep_ntt = false) {
+// First dimension: fst selection cts per query (coefficient form, or NTT
+// form with sel_ntt; stride sel_stride) -> L->fd (other cts per query,
+// coefficient form).
+void run_first_dim(const onion_server *s, Lane *L, uint32_t nq, uint64_t *sel, uint64_t sel_stride,
+                   bool time_core, bool keep_ntt = false, bool sel_ntt = false) {
   if (use_mfma(s, L, nq)) {
-    run_first_dim_mfma(s, L, nq, sel, sel_stride, time_core, keep_ntt);
+    run_first_dim_mfma(s, L, nq, sel, sel_stride, time_core, keep_ntt, sel_ntt);
     return;
   }
   const uint32_t fst = s->P.fst_dim, other = s->P.other_dim, cpad = cpad_for(nq);
   const uint64_t num_pt = s->I.num_pt;
-  k_sel_ntt<<<dim3(fst * 2, nq), kNttThreads, 0, L->st>>>(sel, sel_stride, L->Qs, fst, cpad, s->Tq);
+  if (sel_ntt)
+    k_sel_ntt<true><<<dim3(fst * 2, nq), kNttThreads, 0, L->st>>>(sel, sel_stride, L->Qs, fst, cpad, s->Tq);
+  else
+    k_sel_ntt<false><<<dim3(fst * 2, nq), kNttThreads, 0, L->st>>>(sel, sel_stride, L->Qs, fst, cpad, s->Tq);
   if (time_core) (void)hipEventRecord(L->ev[6], L->st);
   if (nq == 1) {
     if (fst % 16 == 0) {
@@ -581,14 +612,14 @@ void run_mod_switch(const onion_server *s, Lane *L, uint32_t nq, uint64_t *cts, 
 // database-streaming kernel (never inside a graph capture).
 void answer_device(const onion_server *s, Lane *L, uint32_t nq, bool timed) {
   const uint32_t fst = s->P.fst_dim, l = s->P.gsw_l;
-  const uint64_t *leaves = L->trees + s->w * 2 * kN;
+  uint64_t *leaves = L->trees + s->w * 2 * kN;
   if (timed) (void)hipEventRecord(L->ev[0], L->st);
-  run_expand(s, L, nq);
+  const bool leaves_ntt = run_expand(s, L, nq);
   if (timed) (void)hipEventRecord(L->ev[1], L->st);
-  run_convert(s, L, nq, leaves + (size_t)fst * 2 * kN, s->tree_words);
+  run_convert(s, L, nq, leaves + (size_t)fst * 2 * kN, s->tree_words, leaves_ntt);
   if (timed) (void)hipEventRecord(L->ev[2], L->st);
   const bool first_fused = mux_first_fused(s, L, nq);
-  run_first_dim(s, L, nq, leaves, s->tree_words, timed, first_fused);
+  run_first_dim(s, L, nq, leaves, s->tree_words, timed, first_fused, leaves_ntt);
   if (timed) (void)hipEventRecord(L->ev[3], L->st);
   uint64_t n = s->P.other_dim;
   uint32_t d = 0;
@@ -909,11 +940,21 @@ int onion_gpu_create(const onion_params *params, onion_server **out) {
 
   // NTT tables and constants
   host::Tables tq = host::make_tables(I.q, kN), tp = host::make_tables(I.p, kN);
-  if ((rc = dalloc(&s->tabs, 8 * N))) { onion_gpu_destroy(s); return rc; }
-  std::vector<uint64_t> h(8 * N);
+  if ((rc = dalloc(&s->tabs, 10 * N))) { onion_gpu_destroy(s); return rc; }
+  std::vector<uint64_t> h(10 * N);
   const std::vector<uint64_t> *src[8] = {&tq.rp, &tq.rps, &tq.irp, &tq.irps, &tp.rp, &tp.rps, &tp.irp, &tp.irps};
   for (int i = 0; i < 8; ++i) memcpy(h.data() + i * N, src[i]->data(), N * 8);
-  if (hipMemcpy(s->tabs, h.data(), 8 * N * 8, hipMemcpyHostToDevice) != hipSuccess) {
+  if (I.expan_height > 0) {
+    // x^-k in NTT form, k = 2^(h-1): slot s (holding a(psi^o), o = 2 bitrev(s) + 1) times psi^(-k o)
+    const uint64_t base = host::invmod(host::powmod(tq.psi, uint64_t(1) << (I.expan_height - 1), I.q), I.q);
+    for (uint32_t e = 0; e < N; ++e) {
+      const uint64_t f = host::powmod(base, 2 * host::bitrev(e, 11) + 1, I.q);
+      h[8 * N + e] = f;
+      h[9 * N + e] = host::shoup(f, I.q);
+    }
+  }
+  s->xk = s->tabs + 8 * N;
+  if (hipMemcpy(s->tabs, h.data(), 10 * N * 8, hipMemcpyHostToDevice) != hipSuccess) {
     onion_gpu_destroy(s);
     return fail(ONION_EHIP, "table upload failed");
   }
@@ -927,6 +968,8 @@ int onion_gpu_create(const onion_params *params, onion_server **out) {
   s->ks.half_p_mod_q = s->ks.half_p % I.q;
   s->ks.p_inv = host::invmod(I.p % I.q, I.q);
   s->ks.p_inv_s = host::shoup(s->ks.p_inv, I.q);
+  s->ks.np_inv = host::mulmod(N % I.q, s->ks.p_inv, I.q);
+  s->ks.np_inv_s = host::shoup(s->ks.np_inv, I.q);
   s->lift = LiftConst{I.t, (I.t + 1) / 2, I.q - I.t};
   for (uint32_t a = 0; a < I.expan_height; ++a) {
     const uint64_t g = N / (uint64_t(1) << a) + 1;
@@ -934,6 +977,7 @@ int onion_gpu_create(const onion_params *params, onion_server **out) {
     uint64_t gi = 1;
     for (uint64_t x = 1; x < 2 * N; x += 2)
       if ((x * g) % (2 * N) == 1) { gi = x; break; }
+    s->gal.push_back((uint32_t)g);
     s->ginv.push_back((uint32_t)gi);
   }
   host::mod_switch_scale(I.q, I.small_q, s->ms_M, s->ms_f);
@@ -1287,7 +1331,8 @@ int onion_gpu_tap_expand(onion_server *s, uint64_t id, const uint64_t *query, ui
   int rc;
   if ((rc = ensure_scratch(s, L, 1)) || (rc = bind_keys(s, L, 1, &id, false))) return rc;
   HIPCHK(hipMemcpyAsync(L->trees + 2 * kN, query, 2 * kN * 8, hipMemcpyHostToDevice, L->st));
-  run_expand(s, L, 1);
+  if (run_expand(s, L, 1))  // leaves in coefficient form, as the reference's expansion returns them
+    k_cts_to_coeff<<<dim3(2 * s->I.useful_cnt, 1), kNttThreads, 0, L->st>>>(L->trees + s->w * 2 * kN, 0, s->Tq);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(leaves, L->trees + s->w * 2 * kN, (size_t)s->I.useful_cnt * 2 * kN * 8,
                         hipMemcpyDeviceToHost, L->st));

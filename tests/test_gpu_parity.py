@@ -301,6 +301,43 @@ def test_first_mux_level_from_ntt_form(shape):
             assert np.array_equal(o.decrypt_small(sk, fused[i]), o.db_plaintext(5, 0, idx[i]))
 
 
+@pytest.mark.parametrize("shape", [(32, 4), (64, 8), (256, 16), (1024, 2)])
+def test_expansion_leaves_in_ntt_form(shape):
+    """A fused last expansion level (fuse_min = 1) writes NTT-form leaves:
+    the Galois substitution as a slot permutation and x^-k as a slot factor.
+    Tree heights 6, 7, 9 and 11 (k = 32 .. 1024 at the last level).  The
+    expansion tap (leaves returned to coefficient form), a single answer, and
+    batches through the matrix-core and the VALU first dimension (NTT-form
+    selection vectors, NTT-form conversion top rows) all give the oracle's
+    bits, and the same bits as coefficient-form leaves (fuse_min = 2^30)."""
+    from onionpirv2_amd import GpuServer
+    fst, other = shape
+    o = Oracle(fst=fst, other=other)
+    db = o.db_build(8, 0)
+    keys = []
+    for seed in (51, 52):
+        sk = o.keygen(seed)
+        keys.append((seed, sk, o.galois_keys(seed, sk), o.rgsw_key(seed, sk)))
+    idx = [o.num_pt - 1, o.num_pt // 3]
+    qs = np.stack([o.query(keys[i][0], 200 + i, keys[i][1], idx[i]) for i in range(2)])
+    exp = [o.answer(db, keys[i][2], o.rgsw_key_ntt(keys[i][3]), qs[i])[0] for i in range(2)]
+    with GpuServer(fst, other) as srv:
+        srv.db_generate(8, 0)
+        for cid, (seed, sk, gk, rk) in enumerate(keys):
+            srv.set_galois_key(cid, gk)
+            srv.set_rgsw_key(cid, rk)
+        srv.set_tuning(fuse_min=1)
+        assert np.array_equal(srv.tap_expand(0, qs[0]), o.expand(qs[0], keys[0][2]))
+        assert np.array_equal(srv.answer(1, qs[1]), exp[1])
+        for mfma_min in (2, 0):
+            srv.set_tuning(fd_mfma_min=mfma_min)
+            got = srv.answer_batch([0, 1], qs)
+            for i in range(2):
+                assert np.array_equal(got[i], exp[i]), f"fd_mfma_min {mfma_min}, query {i}"
+        srv.set_tuning(fuse_min=1 << 30, fd_mfma_min=2)
+        assert np.array_equal(srv.answer_batch([0, 1], qs), got)
+
+
 @pytest.mark.parametrize("nq", [2, 3, 16, 17, 33, 64])
 def test_first_dim_batch_matches_oracle(cfg1, nq):
     """The batched first-dimension kernels (int8 matrix cores by default, the
