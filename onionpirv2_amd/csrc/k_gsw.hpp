@@ -27,6 +27,9 @@ namespace onion {
 // Digits of ciphertext c: x = src_a[c] (or src_b[c] - src_a[c] mod q when
 // src_b is given, the "y - x" of the mux).  Digit j = poly*l + (l-1-p) holds
 // (x >> p*B) & (2^B - 1).  D layout per query: [ct][2l][N] (NTT form).
+// FOLD (a gadget matrix prepared by k_rgsw_fold_last): the last digit's row
+// holds NTT(x) itself.
+template <bool FOLD = false>
 __global__ __launch_bounds__(kNttThreads) void k_decomp_ntt(const uint64_t *src_a,
                                                             const uint64_t *src_b, uint64_t src_stride,
                                                             uint64_t *D, uint64_t d_stride,
@@ -42,15 +45,19 @@ __global__ __launch_bounds__(kNttThreads) void k_decomp_ntt(const uint64_t *src_
   const size_t qoff = (size_t)blockIdx.y * src_stride;
   const uint64_t *a = src_a + qoff + ct * 2 * kN + (size_t)poly * kN;
   const uint64_t *b = src_b ? src_b + qoff + ct * 2 * kN + (size_t)poly * kN : nullptr;
+  const bool whole = FOLD && pp == 0;
   uint64_t x[8];
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
     const int e = threadIdx.x + 256 * r;
     uint64_t v = a[e];
     if (b) v = sub_mod(b[e], v, T.q);
-    x[r] = (v >> sh) & mask;
+    x[r] = whole ? v : (v >> sh) & mask;
   }
-  ntt_fwd<true>(x, lds, T);  // digits < 2^base_log2 <= 2^32
+  if (whole)
+    ntt_fwd(x, lds, T);
+  else
+    ntt_fwd<true>(x, lds, T);  // digits < 2^base_log2 <= 2^32
 #pragma unroll
   for (int r = 0; r < 8; ++r) x[r] = ntt_fwd_below_8q(x[r], T.q);  // only ext_mac8 reads D
   store_blocked(D + (size_t)blockIdx.y * d_stride + ((size_t)ct * rows + j) * kN, x);
@@ -112,14 +119,35 @@ __global__ __launch_bounds__(kNttThreads) void k_ext_mac_intt_add(const uint64_t
   }
 }
 
+// 128-bit multiply-accumulate of one transformed digit with RGSW row g (both
+// components; blocked slots 8t .. 8t+7).
+__device__ __forceinline__ void mac_row(uint64_t (&lo)[2][8], uint64_t (&hi)[2][8], const uint64_t (&d)[8],
+                                        const uint64_t *g) {
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    uint64_t gv[8];
+    load_blocked(gv, g + (size_t)c * kN);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) mac128(lo[c][r], hi[c][r], d[r], gv[r]);
+  }
+}
+
 // ---- fused variant: one workgroup per (ciphertext, query) ---------------------------
 // All 2l digit NTTs of one ciphertext run back to back in the workgroup and
 // both output components accumulate in registers, so digits never leave the
 // CU.  Chosen when a level has enough ciphertexts to fill the device.
 // Result: res[comp][r] = blocked slot 8t + r of sum_j D_j * G_j[comp] mod q.
+// FOLD: G was prepared by k_rgsw_fold_last, so the last digit's row takes
+// NTT(x) itself.  NTT_IN (conversion of NTT-form leaves, ys unused): xs is in
+// NTT form; each polynomial is copied to top + poly * N as it is (the RGSW's
+// top row), returns to coefficient form in registers for its digits, and is
+// read again for the last row.
+template <bool FOLD = false, bool NTT_IN = false>
 __device__ __forceinline__ void ext_fused_core(const uint64_t *xs, const uint64_t *ys, const uint64_t *G,
                                                uint32_t l, uint32_t base_log2, uint64_t *lds,
-                                               const NttTab &T, const Mod &mq, uint64_t (&res)[2][8]) {
+                                               const NttTab &T, const Mod &mq, uint64_t (&res)[2][8],
+                                               uint64_t *top = nullptr) {
+  static_assert(FOLD || !NTT_IN, "NTT-form input needs the folded gadget matrix");
   const uint64_t mask = (1ULL << base_log2) - 1;
   uint64_t lo[2][8], hi[2][8];
 #pragma unroll
@@ -128,30 +156,41 @@ __device__ __forceinline__ void ext_fused_core(const uint64_t *xs, const uint64_
     for (int r = 0; r < 8; ++r) lo[c][r] = hi[c][r] = 0;
 #pragma unroll 1
   for (uint32_t poly = 0; poly < 2; ++poly) {
+    const uint64_t *g_last = G + (size_t)(poly * l + l - 1) * 2 * kN;
     uint64_t v[8];
+    if (NTT_IN) {
+      load_blocked(v, xs + (size_t)poly * kN);  // canonical, slot order
+      store_blocked(top + (size_t)poly * kN, v);
+      ntt_inv(v, lds, T);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int e = threadIdx.x + 256 * r;
-      const uint64_t xa = xs[(size_t)poly * kN + e];
-      v[r] = ys ? sub_mod(ys[(size_t)poly * kN + e], xa, T.q) : xa;
+      for (int r = 0; r < 8; ++r) v[r] = ntt_inv_final(v[r], T);  // coefficient t + 256 r
+    } else {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int e = threadIdx.x + 256 * r;
+        const uint64_t xa = xs[(size_t)poly * kN + e];
+        v[r] = ys ? sub_mod(ys[(size_t)poly * kN + e], xa, T.q) : xa;
+      }
     }
 #pragma unroll 1
-    for (uint32_t kk = 0; kk < l; ++kk) {
+    for (uint32_t kk = 0; kk < (FOLD ? l - 1 : l); ++kk) {
       const uint32_t sh = (l - 1 - kk) * base_log2;
       uint64_t d[8];
 #pragma unroll
       for (int r = 0; r < 8; ++r) d[r] = (v[r] >> sh) & mask;
       ntt_fwd<true>(d, lds, T);  // digits < 2^base_log2 <= 2^32
-      const uint64_t *g = G + (size_t)(poly * l + kk) * 2 * kN;
 #pragma unroll
       for (int r = 0; r < 8; ++r) d[r] = ntt_fwd_below_8q(d[r], T.q);
+      mac_row(lo, hi, d, G + (size_t)(poly * l + kk) * 2 * kN);
+    }
+    if (NTT_IN) {  // the last row takes NTT(x): the leaf as read
+      load_blocked(v, xs + (size_t)poly * kN);
+      mac_row(lo, hi, v, g_last);
+    } else if (FOLD) {
+      ntt_fwd(v, lds, T);
 #pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        uint64_t gv[8];
-        load_blocked(gv, g + (size_t)c * kN);
-#pragma unroll
-        for (int r = 0; r < 8; ++r) mac128(lo[c][r], hi[c][r], d[r], gv[r]);
-      }
+      for (int r = 0; r < 8; ++r) v[r] = ntt_fwd_below_8q(v[r], T.q);
+      mac_row(lo, hi, v, g_last);
     }
   }
 #pragma unroll
@@ -193,8 +232,9 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused(uint64_t *cts, uint64
 // C = NTT(c) and c = sum_j d_j 2^((l-1-j)B) exactly,
 //   sum_j NTT(d_j) G_j = sum_{j<l-1} NTT(d_j) G'_j + C G_{l-1}  (mod q),
 // because NTT(d_{l-1}) = C - sum_{j<l-1} 2^((l-1-j)B) NTT(d_j): the last
-// digit needs no transform.  grid = (2 l N / 256, nq); g: the level's RGSW of
-// query 0, per-query stride g_stride.
+// digit needs no transform.  grid = (4 N / 256, nq): one thread per
+// (polynomial block, component, slot); G: the RGSW of query 0, per-query
+// stride g_stride.
 __global__ __launch_bounds__(256) void k_rgsw_fold_last(uint64_t *G, uint64_t g_stride, uint32_t l,
                                                         uint32_t base_log2, Mod mq) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;  // (poly, comp, e) over 2 * 2 * N
@@ -204,19 +244,6 @@ __global__ __launch_bounds__(256) void k_rgsw_fold_last(uint64_t *G, uint64_t g_
   for (uint32_t j = 0; j + 1 < l; ++j) {
     const uint64_t pw = 1ULL << ((l - 1 - j) * base_log2);  // (l - 1) B < 60: below q
     blk[(size_t)j * 2 * kN] = sub_mod(blk[(size_t)j * 2 * kN], mulmod_barrett(last, pw, mq), mq.q);
-  }
-}
-
-// 128-bit multiply-accumulate of one transformed digit with RGSW row g (both
-// components; blocked slots 8t .. 8t+7).
-__device__ __forceinline__ void mac_row(uint64_t (&lo)[2][8], uint64_t (&hi)[2][8], const uint64_t (&d)[8],
-                                        const uint64_t *g) {
-#pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    uint64_t gv[8];
-    load_blocked(gv, g + (size_t)c * kN);
-#pragma unroll
-    for (int r = 0; r < 8; ++r) mac128(lo[c][r], hi[c][r], d[r], gv[r]);
   }
 }
 
@@ -351,7 +378,10 @@ __global__ __launch_bounds__(kNttThreads) void k_cts_to_coeff(uint64_t *cts, uin
 }
 
 // Conversion bottom rows, fused: row l + c%l of RGSW c/l = ext(RGSW(s) key, leaf c).
-__global__ __launch_bounds__(kNttThreads) void k_convert_fused(const uint64_t *leaves, uint64_t leaf_stride,
+// FOLD: the key was prepared by k_rgsw_fold_last.  NTT_IN: the leaves are in
+// NTT form, and this kernel also writes the top row c%l (the leaf itself).
+template <bool FOLD, bool NTT_IN>
+__global__ __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_convert_fused(const uint64_t *leaves, uint64_t leaf_stride,
                                                                const uint64_t *const *g_ptrs, uint32_t lk,
                                                                uint32_t base_log2_key, uint64_t *out,
                                                                uint64_t out_stride, uint32_t l_out, NttTab T,
@@ -359,10 +389,11 @@ __global__ __launch_bounds__(kNttThreads) void k_convert_fused(const uint64_t *l
   __shared__ uint64_t lds[kNttLds];
   const uint64_t ct = blockIdx.x;
   const uint64_t *src = leaves + (size_t)blockIdx.y * leaf_stride + ct * 2 * kN;
+  uint64_t *rgsw = out + (size_t)blockIdx.y * out_stride + (ct / l_out) * (2 * l_out) * 2 * kN;
+  uint64_t *dst = rgsw + (l_out + ct % l_out) * 2 * kN;
   uint64_t res[2][8];
-  ext_fused_core(src, nullptr, g_ptrs[blockIdx.y], lk, base_log2_key, lds, T, mq, res);
-  uint64_t *dst = out + (size_t)blockIdx.y * out_stride + (ct / l_out) * (2 * l_out) * 2 * kN +
-                  (l_out + ct % l_out) * 2 * kN;
+  ext_fused_core<FOLD, NTT_IN>(src, nullptr, g_ptrs[blockIdx.y], lk, base_log2_key, lds, T, mq, res,
+                               rgsw + (ct % l_out) * 2 * kN);
   store_blocked(dst, res[0]);
   store_blocked(dst + kN, res[1]);
 }

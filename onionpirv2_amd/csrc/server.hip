@@ -149,6 +149,9 @@ struct onion_server {
   Mod mq{}, mp{};
   KsConst ks{};
   LiftConst lift{};
+  // the stored RGSW(s) keys are folded (k_rgsw_fold_last) for the conversion;
+  // fixed by q, p and l_key, which every server sharing a key store agrees on
+  bool conv_fold = false;
   std::vector<uint32_t> gal, ginv;  // each level's Galois element N/2^a + 1, its inverse mod 2N
   uint64_t ms_M = 0;
   uint32_t ms_f = 0;
@@ -378,14 +381,29 @@ bool run_expand(const onion_server *s, Lane *L, uint32_t nq) {
   return leaves_ntt;
 }
 
+// k_convert_fused<FOLD, NTT_IN> for nq queries.
+template <bool FOLD, bool NTT_IN>
+void launch_convert_fused(const onion_server *s, Lane *L, uint32_t nq, uint32_t n_ct, const uint64_t *conv,
+                          uint64_t conv_stride) {
+  k_convert_fused<FOLD, NTT_IN><<<dim3(n_ct, nq), kNttThreads, 0, L->st>>>(
+      conv, conv_stride, L->rk_ptrs, s->P.gsw_l_key, s->I.base_log2_key, L->rgsw, s->rgsw_words, s->P.gsw_l,
+      s->Tq, s->mq);
+}
+
 // Conversion of the (ndim-1)*l RGSW leaves starting at conv (stride per query).
-// NTT-form leaves (leaves_ntt) are the top rows as they are, and are then
-// brought back to coefficient form, in place, for the digit decomposition.
+// NTT-form leaves (leaves_ntt) are the top rows as they are; the fused kernel
+// copies them and returns them to coefficient form in registers, the split
+// kernels after k_cts_to_coeff has done so in place.
 void run_convert(const onion_server *s, Lane *L, uint32_t nq, uint64_t *conv, uint64_t conv_stride,
                  bool leaves_ntt = false) {
   const uint32_t nd = s->I.ndim - 1, l = s->P.gsw_l, lk = s->P.gsw_l_key;
   if (nd == 0) return;
   const uint32_t n_ct = nd * l;
+  const bool fused = (uint64_t)n_ct * nq >= s->tune.fuse_min;
+  if (fused && leaves_ntt && s->conv_fold) {
+    launch_convert_fused<true, true>(s, L, nq, n_ct, conv, conv_stride);
+    return;
+  }
   if (leaves_ntt) {
     k_rgsw_top_rows<true><<<dim3(n_ct * 2, nq), kNttThreads, 0, L->st>>>(conv, conv_stride, L->rgsw,
                                                                          s->rgsw_words, l, s->Tq);
@@ -394,14 +412,20 @@ void run_convert(const onion_server *s, Lane *L, uint32_t nq, uint64_t *conv, ui
     k_rgsw_top_rows<false><<<dim3(n_ct * 2, nq), kNttThreads, 0, L->st>>>(conv, conv_stride, L->rgsw,
                                                                           s->rgsw_words, l, s->Tq);
   }
-  if ((uint64_t)n_ct * nq >= s->tune.fuse_min) {
-    k_convert_fused<<<dim3(n_ct, nq), kNttThreads, 0, L->st>>>(conv, conv_stride, L->rk_ptrs, lk,
-                                                               s->I.base_log2_key, L->rgsw, s->rgsw_words, l,
-                                                               s->Tq, s->mq);
+  if (fused) {
+    if (s->conv_fold)
+      launch_convert_fused<true, false>(s, L, nq, n_ct, conv, conv_stride);
+    else
+      launch_convert_fused<false, false>(s, L, nq, n_ct, conv, conv_stride);
     return;
   }
-  k_decomp_ntt<<<dim3(n_ct * 2 * lk, nq), kNttThreads, 0, L->st>>>(conv, nullptr, conv_stride, L->digits,
-                                                                   s->digit_words, lk, s->I.base_log2_key, s->Tq);
+  const dim3 dgrid(n_ct * 2 * lk, nq);
+  if (s->conv_fold)
+    k_decomp_ntt<true><<<dgrid, kNttThreads, 0, L->st>>>(conv, nullptr, conv_stride, L->digits, s->digit_words, lk,
+                                                         s->I.base_log2_key, s->Tq);
+  else
+    k_decomp_ntt<false><<<dgrid, kNttThreads, 0, L->st>>>(conv, nullptr, conv_stride, L->digits, s->digit_words,
+                                                          lk, s->I.base_log2_key, s->Tq);
   k_ext_mac_store<<<dim3(n_ct * 2, nq), kNttThreads, 0, L->st>>>(L->digits, s->digit_words, L->rk_ptrs, 2 * lk,
                                                                  L->rgsw, s->rgsw_words, l, s->mq);
 }
@@ -579,7 +603,7 @@ void run_mux_level(const onion_server *s, Lane *L, uint32_t nq, uint64_t n, uint
     cts_to_coeff(s, L, nq, n);
     *ntt_form = false;
   }
-  k_decomp_ntt<<<dim3((uint32_t)(half * 2 * l), nq), kNttThreads, 0, L->st>>>(
+  k_decomp_ntt<false><<<dim3((uint32_t)(half * 2 * l), nq), kNttThreads, 0, L->st>>>(
       L->fd, L->fd + half * 2 * kN, s->fd_words, L->digits, s->digit_words, l, s->I.base_log2, s->Tq);
   k_ext_mac_intt_add<<<dim3((uint32_t)(half * 2), nq), kNttThreads, 0, L->st>>>(
       L->digits, s->digit_words, g, s->rgsw_words, 2 * l, L->fd, s->fd_words, s->Tq, s->mq);
@@ -971,6 +995,9 @@ int onion_gpu_create(const onion_params *params, onion_server **out) {
   s->ks.np_inv = host::mulmod(N % I.q, s->ks.p_inv, I.q);
   s->ks.np_inv_s = host::shoup(s->ks.np_inv, I.q);
   s->lift = LiftConst{I.t, (I.t + 1) / 2, I.q - I.t};
+  // k_rgsw_fold_last's gadget powers 2^((l_key-1)B) stay below q, and the
+  // digits below 2^32 (the digit transforms' first stage)
+  s->conv_fold = P.gsw_l_key >= 2 && I.base_log2_key <= 32 && (uint64_t)(P.gsw_l_key - 1) * I.base_log2_key < 60;
   for (uint32_t a = 0; a < I.expan_height; ++a) {
     const uint64_t g = N / (uint64_t(1) << a) + 1;
     // g is odd, so it is invertible mod 2N
@@ -1186,6 +1213,8 @@ int onion_gpu_set_rgsw_key(onion_server *s, uint64_t id, const uint64_t *rows, s
   HIPCHK(hipMemcpyAsync(ck.rk, rows, n * 8, hipMemcpyHostToDevice, st));
   // GSWEval::gsw_ntt_negacyclic_harvey: every half-row to NTT form
   k_ntt_fwd<<<(uint32_t)(n / kN), kNttThreads, 0, st>>>(ck.rk, ck.rk, s->Tq);
+  if (s->conv_fold)  // stored folded: the conversion multiplies the last row by the leaf itself
+    k_rgsw_fold_last<<<dim3(4 * kN / 256, 1), 256, 0, st>>>(ck.rk, 0, s->P.gsw_l_key, s->I.base_log2_key, s->mq);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(st));
   return ONION_OK;
