@@ -80,7 +80,9 @@ typedef struct onion_tuning {
                             one launch of one workgroup per node, when a
                             stage has at least this many ciphertexts across
                             the batch; below it every transform gets its own
-                            workgroup                                        */
+                            workgroup.  A fused last expansion level writes
+                            its leaves in NTT form, and the first dimension
+                            and the conversion then skip their transforms   */
   uint32_t fd_rows_b1;   /* rows per workgroup of the single-query first-
                             dimension kernel (multiple of 32, 32..1024)       */
   uint32_t fd_tile_rows; /* thread rows of the batched first-dimension tile
