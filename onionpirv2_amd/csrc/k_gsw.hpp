@@ -289,16 +289,23 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
   const uint64_t *oy = ox + (size_t)half * kN * ncols;
   const uint64_t *Gb = G + (size_t)b * g_stride;
   const uint64_t q = T.q, mask = (1ULL << base_log2) - 1;
-  // Y - X of both polynomials (canonical NTT form) into diff
+  // Y - X of both polynomials (canonical NTT form) into diff; FIRST also
+  // parks X in its destination row (contiguous, this thread's own slots) for
+  // the final sum, instead of reading out2 at the slot stride again
   if (FIRST) {
+    uint64_t xs[2][8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const size_t o = (size_t)(8 * t + j) * ncols;
       const uint4 u = *reinterpret_cast<const uint4 *>(ox + o);
       const uint4 w = *reinterpret_cast<const uint4 *>(oy + o);
-      diff[0][8 * t + j] = sub_mod(((uint64_t)w.y << 32) | w.x, ((uint64_t)u.y << 32) | u.x, q);
-      diff[1][8 * t + j] = sub_mod(((uint64_t)w.w << 32) | w.z, ((uint64_t)u.w << 32) | u.z, q);
+      xs[0][j] = ((uint64_t)u.y << 32) | u.x;
+      xs[1][j] = ((uint64_t)u.w << 32) | u.z;
+      diff[0][8 * t + j] = sub_mod(((uint64_t)w.y << 32) | w.x, xs[0][j], q);
+      diff[1][8 * t + j] = sub_mod(((uint64_t)w.w << 32) | w.z, xs[1][j], q);
     }
+    store_blocked(x, xs[0]);
+    store_blocked(x + kN, xs[1]);
   } else {
     const uint64_t *y = x + half * 2 * kN;
 #pragma unroll
@@ -341,21 +348,12 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
     for (int j = 0; j < 8; ++j) d[j] = diff[poly][8 * t + j];
     mac_row(lo, hi, d, Gb + (size_t)(poly * l + l - 1) * 2 * kN);
   }
-  // X + E into cts row r (FIRST reads X from out2 again).  The fence keeps
-  // these loads after the loop: hoisted, they would stay live through it.
+  // X + E into cts row r.  The fence keeps these loads after the loop:
+  // hoisted, they would stay live through it.
   asm volatile("" ::: "memory");
   uint64_t w[2][8];
-  if (FIRST) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const uint4 u = *reinterpret_cast<const uint4 *>(ox + (size_t)(8 * t + j) * ncols);
-      w[0][j] = ((uint64_t)u.y << 32) | u.x;
-      w[1][j] = ((uint64_t)u.w << 32) | u.z;
-    }
-  } else {
-    load_blocked(w[0], x);
-    load_blocked(w[1], x + kN);
-  }
+  load_blocked(w[0], x);
+  load_blocked(w[1], x + kN);
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
 #pragma unroll
