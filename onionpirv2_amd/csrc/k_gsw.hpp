@@ -262,7 +262,7 @@ __global__ __launch_bounds__(256) void k_rgsw_fold_last(uint64_t *G, uint64_t g_
 // columns): pair (r, r + half) of query b, result in cts row r.  The 64
 // queries of one row pair run on one XCD one after another (grid half * nq,
 // block index mod 8 = XCD), so the pair's two 2 MiB rows come from HBM once
-// and every query's 8-byte loads at the 1 KiB slot stride hit that L2.
+// and every query's 16-byte loads at the 1 KiB slot stride hit that L2.
 template <bool FIRST>
 __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, uint64_t cts_stride, uint64_t half,
                                                                const uint64_t *out2, uint32_t ncols, uint32_t nq,
