@@ -42,8 +42,8 @@ extern "C" {
 typedef struct onion_params {
   uint32_t poly_degree; /* N; the device path serves N = 2048                    */
   uint32_t plain_bits;  /* PlainMod: t = smallest prime above 2^(plain_bits-1)   */
-  uint32_t gsw_l;       /* l of the data RGSW (GSW_L)                             */
-  uint32_t gsw_l_key;   /* l of the key RGSW (GSW_L_KEY)                          */
+  uint32_t gsw_l;       /* l of the data RGSW (GSW_L), 1..16                      */
+  uint32_t gsw_l_key;   /* l of the key RGSW (GSW_L_KEY), 1..16                   */
   uint32_t fst_dim;     /* first dimension size                                   */
   uint32_t other_dim;   /* product of the further dimensions (power of two)      */
 } onion_params;

@@ -56,3 +56,12 @@ def test_invalid_shapes_are_rejected():
         derive_params(256, 48)  # other_dim not a power of two
     with pytest.raises(OnionValueError):
         derive_params(2100, 4)  # 2^ceil(log2(2110)) > N: expansion height above log2 N
+    # 2 l products of a transformed digit (< 8q) and a gadget value (< q) sum in
+    # 128 bits: 32 stay below 256 q^2 < 2^128, 34 may not
+    assert derive_params(64, 4, gsw_l=16, gsw_l_key=16)["base_log2_key"] == 4
+    with pytest.raises(OnionValueError):
+        derive_params(64, 4, gsw_l_key=17)
+    with pytest.raises(OnionValueError):
+        derive_params(64, 4, gsw_l=17)
+    o = Oracle(fst=64, other=4)
+    assert 256 * o.q * o.q < 1 << 128 < 288 * o.q * o.q

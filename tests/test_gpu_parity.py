@@ -301,6 +301,45 @@ def test_first_mux_level_from_ntt_form(shape):
             assert np.array_equal(o.decrypt_small(sk, fused[i]), o.db_plaintext(5, 0, idx[i]))
 
 
+@pytest.mark.parametrize("l,l_key", [(3, 4), (13, 11), (5, 16)])
+def test_other_gadget_lengths(l, l_key):
+    """Gadget lengths other than (5, 10) decide which folds apply: the
+    multiplexer's (l - 1) B < 60 holds for l = 3 (B = 20) and 5, not for
+    l = 13 (B = 5: 60), which keeps coefficient-form levels; the conversion
+    key's holds for l_key = 4 (B = 15), not for 11 (B = 6) or 16 (B = 4, and
+    32 products per 128-bit sum, the most the bound allows), which take the
+    unfolded conversion of NTT-form leaves (top rows copied, then returned to
+    coefficient form in place).  Fused (fuse_min = 1) and split shapes give the
+    oracle's bits for the expansion, conversion, a single answer and a batch."""
+    from onionpirv2_amd import GpuServer
+    fst, other = 64, 4
+    o = Oracle(fst=fst, other=other, l=l, l_key=l_key)
+    db = o.db_build(9, 0)
+    keys = []
+    for seed in (61, 62):
+        sk = o.keygen(seed)
+        keys.append((seed, sk, o.galois_keys(seed, sk), o.rgsw_key(seed, sk)))
+    idx = [o.num_pt - 1, 70]
+    qs = np.stack([o.query(keys[i][0], 300 + i, keys[i][1], idx[i]) for i in range(2)])
+    rkn = [o.rgsw_key_ntt(k[3]) for k in keys]
+    exp = [o.answer(db, keys[i][2], rkn[i], qs[i])[0] for i in range(2)]
+    leaves = o.expand(qs[0], keys[0][2])
+    conv = o.convert(leaves, rkn[0])
+    with GpuServer(fst, other, gsw_l=l, gsw_l_key=l_key) as srv:
+        srv.db_generate(9, 0)
+        for cid, (seed, sk, gk, rk) in enumerate(keys):
+            srv.set_galois_key(cid, gk)
+            srv.set_rgsw_key(cid, rk)
+        for fm in (1, 1 << 30):
+            srv.set_tuning(fuse_min=fm, fd_mfma_min=2)
+            assert np.array_equal(srv.tap_expand(0, qs[0]), leaves), f"fuse_min {fm}"
+            assert np.array_equal(srv.tap_convert(0, leaves), conv), f"fuse_min {fm}"
+            assert np.array_equal(srv.answer(1, qs[1]), exp[1]), f"fuse_min {fm}"
+            got = srv.answer_batch([0, 1], qs)
+            for i in range(2):
+                assert np.array_equal(got[i], exp[i]), f"fuse_min {fm}, query {i}"
+
+
 @pytest.mark.parametrize("shape", [(32, 4), (64, 8), (256, 16), (1024, 2)])
 def test_expansion_leaves_in_ntt_form(shape):
     """A fused last expansion level (fuse_min = 1) writes NTT-form leaves:
