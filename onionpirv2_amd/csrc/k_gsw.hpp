@@ -263,36 +263,40 @@ __global__ __launch_bounds__(256) void k_rgsw_fold_last(uint64_t *G, uint64_t g_
 // queries of one row pair run on one XCD one after another (grid half * nq,
 // block index mod 8 = XCD), so the pair's two 2 MiB rows come from HBM once
 // and every query's 16-byte loads at the 1 KiB slot stride hit that L2.
-template <bool FIRST>
+template <bool FIRST, int Q = 1>
 __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, uint64_t cts_stride, uint64_t half,
                                                                const uint64_t *out2, uint32_t ncols, uint32_t nq,
                                                                const uint64_t *G, uint64_t g_stride, uint32_t l,
                                                                uint32_t base_log2, NttTab T, Mod mq) {
+  static_assert(Q == 1 || (FIRST && Q == 2), "query pairs only at the first level");
   __shared__ uint64_t lds[kNttLds];
   __shared__ uint64_t coef[kN];     // y - x of the current polynomial; slots t + 256 j of thread t
-  __shared__ uint64_t diff[2][kN];  // Y - X of both polynomials; slots 8t .. 8t+7 of thread t
+  __shared__ uint64_t diff[2][kN];  // Y - X of both polynomials (of query b)
   const int t = threadIdx.x;
-  uint32_t r, b;
+  const uint32_t nqw = nq / Q;  // workgroups per row pair
+  uint32_t r, b;                // row pair (r, r + half); first query b
   if (!FIRST) {
     r = blockIdx.x;
     b = blockIdx.y;
   } else if (half % 8 == 0) {
     const uint32_t k = blockIdx.x >> 3;
-    r = 8 * (k / nq) + (blockIdx.x & 7);
-    b = k % nq;
+    r = 8 * (k / nqw) + (blockIdx.x & 7);
+    b = Q * (k % nqw);
   } else {
-    r = blockIdx.x / nq;
-    b = blockIdx.x % nq;
+    r = blockIdx.x / nqw;
+    b = Q * (blockIdx.x % nqw);
   }
-  uint64_t *x = cts + (size_t)b * cts_stride + (size_t)r * 2 * kN;
-  const uint64_t *ox = out2 + (size_t)r * kN * ncols + 2 * b;  // FIRST: X at ox[slot * ncols + poly]
+  const uint64_t *ox = out2 + (size_t)r * kN * ncols + 2 * b;  // FIRST: X of query b at ox[slot * ncols + poly]
   const uint64_t *oy = ox + (size_t)half * kN * ncols;
-  const uint64_t *Gb = G + (size_t)b * g_stride;
+  // Q = 2: Y - X of query b + 1, parked in its row r + half, which no level
+  // after the first reads (the levels fold rows below half)
+  uint64_t *dpark = cts + (size_t)(b + Q - 1) * cts_stride + (size_t)(r + half) * 2 * kN;
   const uint64_t q = T.q, mask = (1ULL << base_log2) - 1;
-  // Y - X of both polynomials (canonical NTT form) into diff; FIRST also
-  // parks X in its destination row (contiguous, this thread's own slots) for
-  // the final sum, instead of reading out2 at the slot stride again
-  if (FIRST) {
+  // Y - X (canonical NTT form) into diff; FIRST also parks X in its
+  // destination row for the final sum, instead of reading out2 at the slot
+  // stride again
+  if (FIRST && Q == 1) {
+    uint64_t *x = cts + (size_t)b * cts_stride + (size_t)r * 2 * kN;
     uint64_t xs[2][8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -306,7 +310,33 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
     }
     store_blocked(x, xs[0]);
     store_blocked(x + kN, xs[1]);
+  } else if (FIRST) {
+    // lanes 2i and 2i + 1 read queries b and b + 1 of one slot: the 32
+    // contiguous bytes of each row, so a wave load touches 32 lines, not 64
+    const uint32_t h = t & 1;
+    uint64_t *xh = cts + (size_t)(b + h) * cts_stride + (size_t)r * 2 * kN;
+#pragma unroll 4
+    for (int i = 0; i < kN / 128; ++i) {
+      const uint32_t sl = 128 * i + (t >> 1);
+      const size_t o = (size_t)sl * ncols + 2 * h;
+      const uint4 u = *reinterpret_cast<const uint4 *>(ox + o);
+      const uint4 w = *reinterpret_cast<const uint4 *>(oy + o);
+      const uint64_t x0 = ((uint64_t)u.y << 32) | u.x, x1 = ((uint64_t)u.w << 32) | u.z;
+      const uint64_t d0 = sub_mod(((uint64_t)w.y << 32) | w.x, x0, q);
+      const uint64_t d1 = sub_mod(((uint64_t)w.w << 32) | w.z, x1, q);
+      xh[sl] = x0;
+      xh[kN + sl] = x1;
+      if (h == 0) {
+        diff[0][sl] = d0;
+        diff[1][sl] = d1;
+      } else {
+        dpark[sl] = d0;
+        dpark[kN + sl] = d1;
+      }
+    }
+    __syncthreads();  // parks and diff are read below in other threads' slot order
   } else {
+    const uint64_t *x = cts + (size_t)b * cts_stride + (size_t)r * 2 * kN;
     const uint64_t *y = x + half * 2 * kN;
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
@@ -317,48 +347,60 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
       for (int j = 0; j < 8; ++j) diff[c][8 * t + j] = sub_mod(w[j], u[j], q);
     }
   }
-  uint64_t lo[2][8], hi[2][8];
-#pragma unroll
-  for (int c = 0; c < 2; ++c)
-#pragma unroll
-    for (int j = 0; j < 8; ++j) lo[c][j] = hi[c][j] = 0;
 #pragma unroll 1
-  for (uint32_t poly = 0; poly < 2; ++poly) {
-    {
-      uint64_t v[8];
+  for (int qi = 0; qi < Q; ++qi) {
+    uint64_t *x = cts + (size_t)(b + qi) * cts_stride + (size_t)r * 2 * kN;
+    const uint64_t *Gb = G + (size_t)(b + qi) * g_stride;
+    // this thread's slots 8t .. 8t+7 of Y - X
+    auto y_minus_x = [&](uint32_t poly, uint64_t(&v)[8]) {
+      if (Q == 2 && qi == 1) {
+        load_blocked(v, dpark + (size_t)poly * kN);
+      } else {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) v[j] = diff[poly][8 * t + j];
-      ntt_inv(v, lds, T);
+        for (int j = 0; j < 8; ++j) v[j] = diff[poly][8 * t + j];
+      }
+    };
+    uint64_t lo[2][8], hi[2][8];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) coef[t + 256 * j] = ntt_inv_final(v[j], T);
-    }
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) lo[c][j] = hi[c][j] = 0;
 #pragma unroll 1
-    for (uint32_t kk = 0; kk + 1 < l; ++kk) {
-      const uint32_t sh = (l - 1 - kk) * base_log2;
+    for (uint32_t poly = 0; poly < 2; ++poly) {
+      {
+        uint64_t v[8];
+        y_minus_x(poly, v);
+        ntt_inv(v, lds, T);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) coef[t + 256 * j] = ntt_inv_final(v[j], T);
+      }
+#pragma unroll 1
+      for (uint32_t kk = 0; kk + 1 < l; ++kk) {
+        const uint32_t sh = (l - 1 - kk) * base_log2;
+        uint64_t d[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) d[j] = (coef[t + 256 * j] >> sh) & mask;
+        ntt_fwd<true>(d, lds, T);  // digits < 2^base_log2
+#pragma unroll
+        for (int j = 0; j < 8; ++j) d[j] = ntt_fwd_below_8q(d[j], q);
+        mac_row(lo, hi, d, Gb + (size_t)(poly * l + kk) * 2 * kN);
+      }
       uint64_t d[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) d[j] = (coef[t + 256 * j] >> sh) & mask;
-      ntt_fwd<true>(d, lds, T);  // digits < 2^base_log2
-#pragma unroll
-      for (int j = 0; j < 8; ++j) d[j] = ntt_fwd_below_8q(d[j], q);
-      mac_row(lo, hi, d, Gb + (size_t)(poly * l + kk) * 2 * kN);
+      y_minus_x(poly, d);
+      mac_row(lo, hi, d, Gb + (size_t)(poly * l + l - 1) * 2 * kN);
     }
-    uint64_t d[8];
+    // X + E into cts row r.  The fence keeps these loads after the loop:
+    // hoisted, they would stay live through it.
+    asm volatile("" ::: "memory");
+    uint64_t w[2][8];
+    load_blocked(w[0], x);
+    load_blocked(w[1], x + kN);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) d[j] = diff[poly][8 * t + j];
-    mac_row(lo, hi, d, Gb + (size_t)(poly * l + l - 1) * 2 * kN);
-  }
-  // X + E into cts row r.  The fence keeps these loads after the loop:
-  // hoisted, they would stay live through it.
-  asm volatile("" ::: "memory");
-  uint64_t w[2][8];
-  load_blocked(w[0], x);
-  load_blocked(w[1], x + kN);
+    for (int c = 0; c < 2; ++c) {
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) w[c][j] = add_mod(w[c][j], barrett128(lo[c][j], hi[c][j], mq), q);
-    store_blocked(x + (size_t)c * kN, w[c]);
+      for (int j = 0; j < 8; ++j) w[c][j] = add_mod(w[c][j], barrett128(lo[c][j], hi[c][j], mq), q);
+      store_blocked(x + (size_t)c * kN, w[c]);
+    }
   }
 }
 

@@ -619,12 +619,19 @@ bool mux_first_fused(const onion_server *s, const Lane *L, uint32_t nq) {
          (uint64_t)(s->P.other_dim / 2) * nq >= s->tune.fuse_min;
 }
 
+// An even batch runs one workgroup per row pair and two queries (reading the
+// 32 bytes of both per slot and row); an odd one, one per query.
 void run_mux_first(const onion_server *s, Lane *L, uint32_t nq) {
   const uint32_t half = s->P.other_dim / 2;
   fold_last(s, L, nq, L->rgsw);
-  k_mux_fused_ntt<true><<<half * nq, kNttThreads, 0, L->st>>>(L->fd, s->fd_words, half, L->digits, 2 * nq, nq,
-                                                              L->rgsw, s->rgsw_words, s->P.gsw_l, s->I.base_log2,
-                                                              s->Tq, s->mq);
+  if (nq % 2 == 0)
+    k_mux_fused_ntt<true, 2><<<half * nq / 2, kNttThreads, 0, L->st>>>(
+        L->fd, s->fd_words, half, L->digits, 2 * nq, nq, L->rgsw, s->rgsw_words, s->P.gsw_l, s->I.base_log2, s->Tq,
+        s->mq);
+  else
+    k_mux_fused_ntt<true, 1><<<half * nq, kNttThreads, 0, L->st>>>(
+        L->fd, s->fd_words, half, L->digits, 2 * nq, nq, L->rgsw, s->rgsw_words, s->P.gsw_l, s->I.base_log2, s->Tq,
+        s->mq);
 }
 
 void run_mod_switch(const onion_server *s, Lane *L, uint32_t nq, uint64_t *cts, uint64_t stride) {

@@ -271,7 +271,8 @@ def test_first_mux_level_from_ntt_form(shape):
     The levels after it keep NTT-form ciphertexts while they are wide
     (k_mux_fused_ntt) and return to coefficient form for the narrow split
     kernels: fuse_min = 5 / 10 / 20 put that switch after the first, second
-    or third level."""
+    or third level.  The batch of 3 takes the first level one query per
+    workgroup, a batch of 4 two queries per workgroup."""
     from onionpirv2_amd import GpuServer
     fst, other = shape
     o = Oracle(fst=fst, other=other)
@@ -299,6 +300,13 @@ def test_first_mux_level_from_ntt_form(shape):
             exp, _ = o.answer(db, gk, o.rgsw_key_ntt(rk), qs[i])
             assert np.array_equal(fused[i], exp), f"query {i}"
             assert np.array_equal(o.decrypt_small(sk, fused[i]), o.db_plaintext(5, 0, idx[i]))
+        # an even batch takes the first level two queries per workgroup
+        q4 = o.query(keys[0][0], 104, keys[0][1], 7)
+        exp4, _ = o.answer(db, keys[0][2], o.rgsw_key_ntt(keys[0][3]), q4)
+        srv.set_tuning(fuse_min=1)
+        even = srv.answer_batch([0, 1, 2, 0], np.concatenate([qs, q4[None]]))
+        assert np.array_equal(even[:3], fused)
+        assert np.array_equal(even[3], exp4)
 
 
 @pytest.mark.parametrize("l,l_key", [(3, 4), (13, 11), (5, 16)])
