@@ -169,8 +169,11 @@ struct onion_server {
   size_t io_words = 0;
   uint8_t *pack = nullptr;
   // measured on MI355X at 512x1024: 8 nontemporal 16-byte loads in flight per
-  // lane and 256-row blocks stream the database at 94% of a plain read kernel
-  onion_tuning tune{1024, 256, 4, 3, 0, 2, 2, 1};
+  // lane and 256-row blocks stream the database at 94% of a plain read kernel;
+  // fuse_min 512 runs a single query's first multiplexer level (512 pairs)
+  // fused on the first dimension's NTT-form outputs, 1.4% faster than split
+  // (batches of 64 take the same time either way)
+  onion_tuning tune{512, 256, 4, 3, 0, 2, 2, 1};
   uint64_t tune_epoch = 1;
   Gate gate;
   mutable std::mutex last_mu;
@@ -558,7 +561,7 @@ void run_first_dim(const onion_server *s, Lane *L, uint32_t nq, uint64_t *sel, u
     launch_fd_bat<128>(s, L, nq);
   }
   if (time_core) (void)hipEventRecord(L->ev[7], L->st);
-  k_ntt_inv<<<other * 2 * nq, kNttThreads, 0, L->st>>>(L->fd, L->fd, s->Tq);
+  if (!keep_ntt) k_ntt_inv<<<other * 2 * nq, kNttThreads, 0, L->st>>>(L->fd, L->fd, s->Tq);
 }
 
 // n NTT-form cts per query in fd back to coefficient form.
@@ -619,6 +622,14 @@ bool mux_first_fused(const onion_server *s, const Lane *L, uint32_t nq) {
          (uint64_t)(s->P.other_dim / 2) * nq >= s->tune.fuse_min;
 }
 
+// The VALU first-dimension kernels (one query, or batches off the matrix
+// cores) also leave NTT-form outputs in fd: a wide first level folds them as
+// they are (k_mux_fused_ntt<false>) instead of transforming them all back.
+bool first_dim_keeps_ntt(const onion_server *s, const Lane *L, uint32_t nq) {
+  return s->I.ndim >= 2 && !use_mfma(s, L, nq) && mux_ntt_ok(s) &&
+         (uint64_t)(s->P.other_dim / 2) * nq >= s->tune.fuse_min;
+}
+
 // An even batch runs one workgroup per row pair and two queries (reading the
 // 32 bytes of both per slot and row); an odd one, one per query.
 void run_mux_first(const onion_server *s, Lane *L, uint32_t nq) {
@@ -649,12 +660,12 @@ void answer_device(const onion_server *s, Lane *L, uint32_t nq, bool timed) {
   if (timed) (void)hipEventRecord(L->ev[1], L->st);
   run_convert(s, L, nq, leaves + (size_t)fst * 2 * kN, s->tree_words, leaves_ntt);
   if (timed) (void)hipEventRecord(L->ev[2], L->st);
-  const bool first_fused = mux_first_fused(s, L, nq);
-  run_first_dim(s, L, nq, leaves, s->tree_words, timed, first_fused, leaves_ntt);
+  const bool first_fused = mux_first_fused(s, L, nq), fd_ntt = first_dim_keeps_ntt(s, L, nq);
+  run_first_dim(s, L, nq, leaves, s->tree_words, timed, first_fused || fd_ntt, leaves_ntt);
   if (timed) (void)hipEventRecord(L->ev[3], L->st);
   uint64_t n = s->P.other_dim;
   uint32_t d = 0;
-  bool ntt_form = false;
+  bool ntt_form = fd_ntt;
   if (first_fused) {
     run_mux_first(s, L, nq);
     n /= 2;
@@ -882,7 +893,7 @@ int autotune_impl(onion_server *s, const uint32_t *sizes, uint32_t n_sizes, uint
       {&onion_tuning::fd_tile_rows, {2, 4}, hasb},
       {&onion_tuning::fd_b1_variant, {0, 1, 2, 3}, has1},
       {&onion_tuning::fd_rows_b1, {64, 128, 256, 512}, has1},
-      {&onion_tuning::fuse_min, {1, 256, 1024, 4096, 1u << 30}, true},
+      {&onion_tuning::fuse_min, {1, 256, 512, 1024, 4096, 1u << 30}, true},
       {&onion_tuning::use_graph, {0, 1}, true},
   };
   for (const Axis &ax : axes) {
