@@ -15,7 +15,8 @@
 // deep, so a level's latency is two transforms, not six.  Wide levels take
 // one launch, k_expand_node (one workgroup per node, six transforms deep),
 // which keeps X out of memory and transforms sigma(c1) once per limb for both
-// components.
+// components; a wide last level takes k_expand_leaf_ntt, which writes the
+// leaves in NTT form (the form every reader of the leaves transforms them to).
 // Batched: blockIdx.y selects the query; each query has its own tree, key and
 // X scratch.
 #pragma once

@@ -259,10 +259,11 @@ __global__ __launch_bounds__(256) void k_rgsw_fold_last(uint64_t *G, uint64_t g_
 // cts (per-query stride cts_stride), result in place.
 // FIRST = true (the first level, straight from the matrix-core product's
 // NTT-form outputs out2[row][slot][column], column = 2 query + poly, ncols
-// columns): pair (r, r + half) of query b, result in cts row r.  The 64
-// queries of one row pair run on one XCD one after another (grid half * nq,
-// block index mod 8 = XCD), so the pair's two 2 MiB rows come from HBM once
-// and every query's 16-byte loads at the 1 KiB slot stride hit that L2.
+// columns): pair (r, r + half) of query b, result in cts row r; with Q = 2,
+// of queries b and b + 1 (an even batch).  The workgroups of one row pair
+// run on one XCD one after another (grid half * nq / Q, block index mod 8 =
+// XCD), so the pair's two 2 MiB rows come from HBM once and the 16-byte loads
+// at the 1 KiB slot stride hit that L2.
 template <bool FIRST, int Q = 1>
 __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, uint64_t cts_stride, uint64_t half,
                                                                const uint64_t *out2, uint32_t ncols, uint32_t nq,
