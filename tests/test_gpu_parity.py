@@ -148,7 +148,8 @@ def test_unknown_client_raises(cfg1):
 
 def test_small_shape_generic_kernel_and_upload():
     """fst = 12 (not a multiple of 16) takes the generic first-dim kernel;
-    the database arrives through the plaintext upload path."""
+    the database arrives through the plaintext upload path.  Split and fused
+    first multiplexer levels."""
     from onionpirv2_amd import GpuServer
     o = Oracle(fst=12, other=4)
     rng = np.random.default_rng(9)
@@ -171,6 +172,10 @@ def test_small_shape_generic_kernel_and_upload():
             got = srv.answer(1, q)
             assert np.array_equal(got, exp)
             assert np.array_equal(o.decrypt_small(sk, got), pts[idx])
+        # fuse_min = 1: the generic kernel's NTT-form outputs go straight into
+        # a fused first multiplexer level
+        srv.set_tuning(fuse_min=1)
+        assert np.array_equal(srv.answer(1, q), exp)
 
 
 def test_first_dim_wraps_mod_2_128():
