@@ -15,8 +15,8 @@
 // deep, so a level's latency is two transforms, not six.  Wide levels take
 // one launch, k_expand_node (one workgroup per node, six transforms deep),
 // which keeps X out of memory and transforms sigma(c1) once per limb for both
-// components; a wide last level takes k_expand_leaf_ntt, which writes the
-// leaves in NTT form (the form every reader of the leaves transforms them to).
+// components; from the first wide level on, k_expand_ntt writes NTT-form
+// children (the form every reader of the leaves transforms them to).
 // Batched: blockIdx.y selects the query; each query has its own tree, key and
 // X scratch.
 #pragma once
@@ -30,7 +30,7 @@ struct KsConst {
   uint64_t half_p_mod_q;  // floor(p/2) mod q
   uint64_t p_inv;         // p^-1 mod q
   uint64_t p_inv_s;       // Shoup companion of p_inv
-  uint64_t np_inv;        // N p^-1 mod q (k_expand_leaf_ntt)
+  uint64_t np_inv;        // N p^-1 mod q (k_expand_ntt)
   uint64_t np_inv_s;      // Shoup companion of np_inv
 };
 
@@ -198,7 +198,7 @@ __device__ __forceinline__ uint32_t galois_slot(uint32_t s, uint32_t g) {
   return __brev(((g * o) & (2 * kN - 1)) >> 1) >> 21;
 }
 
-// One component of a last-level node in NTT form (k_expand_leaf_ntt).  P: the
+// One component of a node with NTT-form children (k_expand_ntt).  P: the
 // thread's slots of NTT_p(sigma(c1)); Q[0], Q[1]: NTT_q(c0), NTT_q(c1).
 __device__ __forceinline__ void leaf_ntt_comp(int comp, const uint64_t (&P)[8], const uint64_t (&Q)[2][kN],
                                               const uint64_t *key, const uint64_t *xk, uint32_t g, uint64_t *lds,
@@ -252,21 +252,25 @@ __device__ __forceinline__ void leaf_ntt_comp(int comp, const uint64_t (&P)[8], 
   }
 }
 
-// The last expansion level with NTT-form children (slot order, canonical), one
+// An expansion level with NTT-form children (slot order, canonical), one
 // workgroup per node: the form the selection vectors and the conversion's top
 // rows take, so their forward transforms disappear.  In NTT form sigma_g is a
 // slot permutation (galois_slot) and x^-k the factor psi^(-k o(s)) of slot s
-// (xk: N factors, then N Shoup companions).  A node transforms c0, c1 (mod q)
-// and sigma(c1) (mod p) once each; the q-limb key products need no transform;
-// each component's p-limb product returns to coefficient form for the
-// rounding division and its correction is transformed forward: 3 + 2 + 2
-// transforms, where k_expand_node takes 6 and the children's selection
-// vectors 4 more.  The node's own residues in coefficient form go unchanged
-// into the rounding, so the children equal k_expand_node's, transformed.
-__global__ __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_expand_leaf_ntt(uint64_t *trees, uint64_t tree_stride,
-                                                                 const uint64_t *const *key_ptrs, uint32_t level,
-                                                                 uint32_t g, uint32_t ginv, const uint64_t *xk,
-                                                                 NttTab Tq, NttTab Tp, Mod mq, Mod mp, KsConst ks) {
+// (xk: the level's N factors, then N Shoup companions).  The q-limb key
+// products need no transform; each component's p-limb product returns to
+// coefficient form for the rounding division and its correction is
+// transformed forward (2 + 2).  The p-limb product needs NTT_p(sigma(c1)):
+//   NTT_IN = false (coefficient-form node): c0, c1 forward mod q and sigma(c1)
+//     forward mod p, 3 + 4 = 7 transforms (k_expand_node takes 6, and the
+//     children's next readers 4 more);
+//   NTT_IN = true (the level above wrote NTT form): sigma(c1) from c1's
+//     slots, back to coefficient form mod q and forward mod p, 2 + 4 = 6.
+// The node's residues in coefficient form go unchanged into the rounding, so
+// the children equal k_expand_node's, transformed.
+template <bool NTT_IN>
+__global__ __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_expand_ntt(
+    uint64_t *trees, uint64_t tree_stride, const uint64_t *const *key_ptrs, uint32_t level, uint32_t g,
+    uint32_t ginv, const uint64_t *xk, NttTab Tq, NttTab Tp, Mod mq, Mod mp, KsConst ks) {
   __shared__ uint64_t lds[kNttLds];
   __shared__ uint64_t Q[2][kN];  // NTT_q(c0), NTT_q(c1), canonical; read at permuted slots
   const uint32_t nl = blockIdx.x, k = 1u << level;
@@ -275,18 +279,33 @@ __global__ __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(3))
   const uint64_t *c = tree + node * 2 * kN;
   const uint64_t *key = key_ptrs[blockIdx.y] + (size_t)level * 4 * kN;
   const int t = threadIdx.x;
-  uint64_t x[8];
+  uint64_t x[8], P[8];
+  if (NTT_IN) {
+#pragma unroll
+    for (int comp = 0; comp < 2; ++comp) {
+      load_blocked(x, c + (size_t)comp * kN);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) Q[comp][8 * t + j] = x[j];
+    }
+    __syncthreads();  // Q[1] is read at permuted slots
+    // sigma(c1) in NTT form, then its coefficients (< q < p serve mod p unchanged)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) P[j] = Q[1][galois_slot(8 * t + j, g)];
+    ntt_inv(P, lds, Tq);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) P[j] = ntt_inv_final(P[j], Tq);
+  } else {
 #pragma unroll 1
-  for (int comp = 0; comp < 2; ++comp) {
-    load_natural(x, c + (size_t)comp * kN);
-    ntt_fwd(x, lds, Tq);
+    for (int comp = 0; comp < 2; ++comp) {
+      load_natural(x, c + (size_t)comp * kN);
+      ntt_fwd(x, lds, Tq);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) Q[comp][8 * t + j] = ntt_fwd_final(x[j], Tq.q);
+      for (int j = 0; j < 8; ++j) Q[comp][8 * t + j] = ntt_fwd_final(x[j], Tq.q);
+    }
+    // sigma(c1) mod p; its residues < q < p serve unchanged
+#pragma unroll
+    for (int j = 0; j < 8; ++j) P[j] = galois_gather(c + kN, t + 256 * j, ginv, Tq.q);
   }
-  // sigma(c1) mod p; its residues < q < p serve unchanged
-  uint64_t P[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) P[j] = galois_gather(c + kN, t + 256 * j, ginv, Tq.q);
   ntt_fwd(P, lds, Tp);  // its barriers also order the Q writes before the reads below
   // the children are other tree slots: every read of c precedes these writes
   uint64_t *left = tree + (2 * node) * 2 * kN, *right = tree + (2 * node + 1) * 2 * kN;

@@ -107,7 +107,7 @@ __device__ __forceinline__ void a_planes(const uint4 *a, uint32_t (&out)[16]) {
 // pieces 1 KiB apart, and measured 4.3x its bytes in write traffic).
 //   k_sel_digits  NTT of each leaf polynomial, in place, in signed-digit
 //                 form (slot order).  grid = (fst * 2, nq).  Skipped when
-//                 the leaves are in NTT form already (k_expand_leaf_ntt):
+//                 the leaves are in NTT form already (k_expand_ntt):
 //                 k_sel_chunks<true> then takes canonical residues.
 //   k_sel_chunks  8 slots x one 32-column block x one step per workgroup:
 //                 reads each (column, j)'s 8 slots (64 contiguous bytes) into

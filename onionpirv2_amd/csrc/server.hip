@@ -141,8 +141,9 @@ struct onion_server {
   int device = 0;
   uint32_t n_cu = 0;  // compute units of the device
   uint64_t w = 0;  // 2^h
-  // device NTT tables: q then p, each rp rps irp irps (4N words); then the
-  // last expansion level's x^-k slot factors and their Shoup companions (2N)
+  // device NTT tables: q then p, each rp rps irp irps (4N words); then per
+  // expansion level a, x^-k (k = 2^a) as slot factors and their Shoup
+  // companions (2N words each)
   uint64_t *tabs = nullptr;
   const uint64_t *xk = nullptr;
   NttTab Tq{}, Tp{};
@@ -350,16 +351,28 @@ int check_db(const onion_server *s) {
 // ---- pipeline stages (nq queries; query b's ciphertext at tree b, slot 1) ------------
 
 // Nodes 2^a .. 2^a + n_active - 1 of every query's tree; X scratch in digits.
-// Levels with at least tune.fuse_min nodes across the batch run fused; the
-// last level (`leaf`), fused, writes NTT-form children.  Returns whether the
-// children are in NTT form.
-bool expand_level(const onion_server *s, Lane *L, uint32_t nq, uint32_t a, uint32_t n_active, bool leaf = false) {
-  if ((uint64_t)n_active * nq >= s->tune.fuse_min) {
-    if (leaf) {
-      k_expand_leaf_ntt<<<dim3(n_active, nq), kNttThreads, 0, L->st>>>(
-          L->trees, s->tree_words, L->gk_ptrs, a, s->gal[a], s->ginv[a], s->xk, s->Tq, s->Tp, s->mq, s->mp, s->ks);
-      return true;
-    }
+// Levels with at least tune.fuse_min nodes across the batch run fused and,
+// with ntt_out, write NTT-form children (k_expand_ntt); ntt_in: the nodes
+// are in NTT form.  Returns whether the children are in NTT form.
+bool expand_level(const onion_server *s, Lane *L, uint32_t nq, uint32_t a, uint32_t n_active, bool ntt_in,
+                  bool ntt_out) {
+  const uint64_t k = uint64_t(1) << a;
+  const bool fused = (uint64_t)n_active * nq >= s->tune.fuse_min;
+  if (ntt_in && !(fused && ntt_out)) {  // back to coefficient form (the fused levels are the lower ones)
+    k_cts_to_coeff<<<dim3(2 * n_active, nq), kNttThreads, 0, L->st>>>(L->trees + k * 2 * kN, s->tree_words, s->Tq);
+    ntt_in = false;
+  }
+  if (fused && ntt_out) {
+    const uint64_t *xk = s->xk + 2 * (size_t)a * kN;
+    if (ntt_in)
+      k_expand_ntt<true><<<dim3(n_active, nq), kNttThreads, 0, L->st>>>(
+          L->trees, s->tree_words, L->gk_ptrs, a, s->gal[a], s->ginv[a], xk, s->Tq, s->Tp, s->mq, s->mp, s->ks);
+    else
+      k_expand_ntt<false><<<dim3(n_active, nq), kNttThreads, 0, L->st>>>(
+          L->trees, s->tree_words, L->gk_ptrs, a, s->gal[a], s->ginv[a], xk, s->Tq, s->Tp, s->mq, s->mp, s->ks);
+    return true;
+  }
+  if (fused) {
     k_expand_node<<<dim3(n_active, nq), kNttThreads, 0, L->st>>>(L->trees, s->tree_words, L->gk_ptrs, a,
                                                                  s->ginv[a], s->Tq, s->Tp, s->mq, s->mp, s->ks);
     return false;
@@ -371,17 +384,21 @@ bool expand_level(const onion_server *s, Lane *L, uint32_t nq, uint32_t a, uint3
   return false;
 }
 
-// The expansion tree of each query; returns whether the leaves are in NTT form.
+// The expansion tree of each query; returns whether the leaves are in NTT
+// form.  From the first fused level on, the tree stays in NTT form: that
+// level pays one transform per node more than k_expand_node, every level
+// after it none, and the last level saves one against writing NTT-form
+// leaves from coefficient form.
 bool run_expand(const onion_server *s, Lane *L, uint32_t nq) {
   const uint32_t h = s->I.expan_height, u = s->I.useful_cnt;
-  bool leaves_ntt = false;
+  bool ntt = false;
   for (uint32_t a = 0; a < h; ++a) {
     // active nodes at level a: those whose left-most leaf is < u
     const uint64_t span = s->w >> a;  // leaves under one node
     const uint32_t n_active = (uint32_t)((u + span - 1) / span);
-    leaves_ntt = expand_level(s, L, nq, a, n_active, a + 1 == h);
+    ntt = expand_level(s, L, nq, a, n_active, ntt, true);
   }
-  return leaves_ntt;
+  return ntt;
 }
 
 // k_convert_fused<FOLD, NTT_IN> for nq queries.
@@ -982,21 +999,22 @@ int onion_gpu_create(const onion_params *params, onion_server **out) {
 
   // NTT tables and constants
   host::Tables tq = host::make_tables(I.q, kN), tp = host::make_tables(I.p, kN);
-  if ((rc = dalloc(&s->tabs, 10 * N))) { onion_gpu_destroy(s); return rc; }
-  std::vector<uint64_t> h(10 * N);
+  const size_t tab_words = (8 + 2 * (size_t)I.expan_height) * N;
+  if ((rc = dalloc(&s->tabs, tab_words))) { onion_gpu_destroy(s); return rc; }
+  std::vector<uint64_t> h(tab_words);
   const std::vector<uint64_t> *src[8] = {&tq.rp, &tq.rps, &tq.irp, &tq.irps, &tp.rp, &tp.rps, &tp.irp, &tp.irps};
   for (int i = 0; i < 8; ++i) memcpy(h.data() + i * N, src[i]->data(), N * 8);
-  if (I.expan_height > 0) {
-    // x^-k in NTT form, k = 2^(h-1): slot s (holding a(psi^o), o = 2 bitrev(s) + 1) times psi^(-k o)
-    const uint64_t base = host::invmod(host::powmod(tq.psi, uint64_t(1) << (I.expan_height - 1), I.q), I.q);
+  for (uint32_t a = 0; a < I.expan_height; ++a) {
+    // x^-k in NTT form, k = 2^a: slot s (holding a(psi^o), o = 2 bitrev(s) + 1) times psi^(-k o)
+    const uint64_t base = host::invmod(host::powmod(tq.psi, uint64_t(1) << a, I.q), I.q);
+    uint64_t *f = h.data() + (8 + 2 * (size_t)a) * N;
     for (uint32_t e = 0; e < N; ++e) {
-      const uint64_t f = host::powmod(base, 2 * host::bitrev(e, 11) + 1, I.q);
-      h[8 * N + e] = f;
-      h[9 * N + e] = host::shoup(f, I.q);
+      f[e] = host::powmod(base, 2 * host::bitrev(e, 11) + 1, I.q);
+      f[N + e] = host::shoup(f[e], I.q);
     }
   }
   s->xk = s->tabs + 8 * N;
-  if (hipMemcpy(s->tabs, h.data(), 10 * N * 8, hipMemcpyHostToDevice) != hipSuccess) {
+  if (hipMemcpy(s->tabs, h.data(), tab_words * 8, hipMemcpyHostToDevice) != hipSuccess) {
     onion_gpu_destroy(s);
     return fail(ONION_EHIP, "table upload failed");
   }
@@ -1487,7 +1505,7 @@ int onion_gpu_tap_galois(onion_server *s, uint64_t id, uint32_t level, const uin
   // place ct at node 2^level; its left child is ct + Subs(ct), so Subs(ct) = left - ct
   const uint64_t node = uint64_t(1) << level;
   HIPCHK(hipMemcpyAsync(L->trees + node * 2 * kN, ct, 2 * kN * 8, hipMemcpyHostToDevice, L->st));
-  expand_level(s, L, 1, level, 1);
+  expand_level(s, L, 1, level, 1, false, false);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, L->trees + 2 * node * 2 * kN, 2 * kN * 8, hipMemcpyDeviceToHost, L->st));
   HIPCHK(hipStreamSynchronize(L->st));

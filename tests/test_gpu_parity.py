@@ -355,8 +355,9 @@ def test_other_gadget_lengths(l, l_key):
 
 @pytest.mark.parametrize("shape", [(32, 4), (64, 8), (256, 16), (1024, 2)])
 def test_expansion_leaves_in_ntt_form(shape):
-    """A fused last expansion level (fuse_min = 1) writes NTT-form leaves:
-    the Galois substitution as a slot permutation and x^-k as a slot factor.
+    """Fused expansion levels (fuse_min = 1: all of them) run in NTT form from
+    the first on and write NTT-form leaves: the Galois substitution as a slot
+    permutation and x^-k as a slot factor.
     Tree heights 6, 7, 9 and 11 (k = 32 .. 1024 at the last level).  The
     expansion tap (leaves returned to coefficient form), a single answer, and
     batches through the matrix-core and the VALU first dimension (NTT-form
