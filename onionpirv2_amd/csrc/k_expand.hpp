@@ -200,7 +200,7 @@ __device__ __forceinline__ uint32_t galois_slot(uint32_t s, uint32_t g) {
 
 // One component of a node with NTT-form children (k_expand_ntt).  P: the
 // thread's slots of NTT_p(sigma(c1)); Q[0], Q[1]: NTT_q(c0), NTT_q(c1).
-__device__ __forceinline__ void leaf_ntt_comp(int comp, const uint64_t (&P)[8], const uint64_t (&Q)[2][kN],
+__device__ __forceinline__ void ntt_node_comp(int comp, const uint64_t (&P)[8], const uint64_t (&Q)[2][kN],
                                               const uint64_t *key, const uint64_t *xk, uint32_t g, uint64_t *lds,
                                               uint64_t *left, uint64_t *right, const NttTab &Tq, const NttTab &Tp,
                                               const Mod &mq, const Mod &mp, const KsConst &ks) {
@@ -309,9 +309,9 @@ __global__ __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(3))
   ntt_fwd(P, lds, Tp);  // its barriers also order the Q writes before the reads below
   // the children are other tree slots: every read of c precedes these writes
   uint64_t *left = tree + (2 * node) * 2 * kN, *right = tree + (2 * node + 1) * 2 * kN;
-  leaf_ntt_comp(0, P, Q, key, xk, g, lds, left, right, Tq, Tp, mq, mp, ks);
+  ntt_node_comp(0, P, Q, key, xk, g, lds, left, right, Tq, Tp, mq, mp, ks);
   asm volatile("" ::: "memory");
-  leaf_ntt_comp(1, P, Q, key, xk, g, lds, left, right, Tq, Tp, mq, mp, ks);
+  ntt_node_comp(1, P, Q, key, xk, g, lds, left, right, Tq, Tp, mq, mp, ks);
 }
 
 // Galois keys as stored: every value of limb q (p) times N^-1 mod q (mod p).
