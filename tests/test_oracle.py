@@ -176,6 +176,32 @@ def test_shift_is_multiplication_by_x_minus_k():
         assert list(map(int, out)) == negacyclic_mul(a, xk, o.q)
 
 
+def test_ntt_form_galois_permutation_and_shift_factor():
+    """The slot identities the NTT-form expansion levels use, on the oracle's
+    2048-point transform (slot s holds a(psi^o(s)), o(s) = 2 bitrev(s) + 1):
+    NTT(sigma_g a)[s] = NTT(a)[bitrev((g o(s) mod 2N) >> 1)] and
+    NTT(x^-k a)[s] = psi^(-k o(s)) NTT(a)[s], at every level's g = N/k + 1."""
+    n = 2048
+    o = Oracle(N=n, fst=64, other=2)
+    q = o.q
+    psi = int(o.L.orc_ntt_psi(o.c, 0))
+    rng = np.random.default_rng(12)
+    a = rng.integers(0, q, n, dtype=np.uint64)
+    A = o.ntt(a)
+    br = np.array([int(format(i, "011b")[::-1], 2) for i in range(n)])
+    odd = 2 * br + 1
+    for level in range(11):
+        k = 1 << level
+        g = n // k + 1
+        src = br[((g * odd) % (2 * n)) >> 1]
+        assert np.array_equal(o.ntt(galois(o, a, g, q)), A[src]), f"sigma, level {level}"
+        shifted = u64(n)
+        o.L.orc_shift_neg(o.c, ptr(a), k, ptr(shifted))
+        base = pow(pow(psi, k, q), q - 2, q)
+        exp = [int(A[s]) * pow(base, int(odd[s]), q) % q for s in range(n)]
+        assert list(map(int, o.ntt(shifted))) == exp, f"x^-k, level {level}"
+
+
 # ---------------------------------------------------------------------------------
 # key switching with the special prime
 # ---------------------------------------------------------------------------------
