@@ -124,6 +124,11 @@ struct KeyStore {
 // device arrays of per-query key pointers, and the graphs captured for it.
 struct Lane {
   hipStream_t st = nullptr;
+  // the conversion runs on st2 beside the selection stage on st (default
+  // priority, like st); fork / join order it after the expansion and before
+  // the first-dimension product
+  hipStream_t st2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   hipEvent_t ev[8]{};
   uint32_t cap = 0;
   uint64_t *trees = nullptr, *rgsw = nullptr, *fd = nullptr, *digits = nullptr;
@@ -247,16 +252,23 @@ int ensure_scratch(const onion_server *s, Lane *L, uint32_t cap) {
 
 void destroy_lane(Lane *L) {
   if (L->st) (void)hipStreamSynchronize(L->st);
+  if (L->st2) (void)hipStreamSynchronize(L->st2);
   free_scratch(L);
   for (auto &e : L->ev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : {L->fork, L->join})
+    if (e) (void)hipEventDestroy(e);
   if (L->st) (void)hipStreamDestroy(L->st);
-  L->st = nullptr;
+  if (L->st2) (void)hipStreamDestroy(L->st2);
+  L->st = L->st2 = nullptr;
 }
 
 int make_lane(std::unique_ptr<Lane> &out) {
   auto L = std::make_unique<Lane>();
-  bool ok = hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking) == hipSuccess;
+  bool ok = hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking) == hipSuccess &&
+            hipStreamCreateWithFlags(&L->st2, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&L->fork, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&L->join, hipEventDisableTiming) == hipSuccess;
   for (auto &e : L->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
   if (!ok) {
     destroy_lane(L.get());
@@ -401,11 +413,11 @@ bool run_expand(const onion_server *s, Lane *L, uint32_t nq) {
   return ntt;
 }
 
-// k_convert_fused<FOLD, NTT_IN> for nq queries.
+// k_convert_fused<FOLD, NTT_IN> for nq queries, on stream st.
 template <bool FOLD, bool NTT_IN>
 void launch_convert_fused(const onion_server *s, Lane *L, uint32_t nq, uint32_t n_ct, const uint64_t *conv,
-                          uint64_t conv_stride) {
-  k_convert_fused<FOLD, NTT_IN><<<dim3(n_ct, nq), kNttThreads, 0, L->st>>>(
+                          uint64_t conv_stride, hipStream_t st) {
+  k_convert_fused<FOLD, NTT_IN><<<dim3(n_ct, nq), kNttThreads, 0, st>>>(
       conv, conv_stride, L->rk_ptrs, s->P.gsw_l_key, s->I.base_log2_key, L->rgsw, s->rgsw_words, s->P.gsw_l,
       s->Tq, s->mq);
 }
@@ -415,38 +427,39 @@ void launch_convert_fused(const onion_server *s, Lane *L, uint32_t nq, uint32_t 
 // copies them and returns them to coefficient form in registers, the split
 // kernels after k_cts_to_coeff has done so in place.
 void run_convert(const onion_server *s, Lane *L, uint32_t nq, uint64_t *conv, uint64_t conv_stride,
-                 bool leaves_ntt = false) {
+                 bool leaves_ntt = false, hipStream_t st = nullptr) {
+  if (!st) st = L->st;
   const uint32_t nd = s->I.ndim - 1, l = s->P.gsw_l, lk = s->P.gsw_l_key;
   if (nd == 0) return;
   const uint32_t n_ct = nd * l;
   const bool fused = (uint64_t)n_ct * nq >= s->tune.fuse_min;
   if (fused && leaves_ntt && s->conv_fold) {
-    launch_convert_fused<true, true>(s, L, nq, n_ct, conv, conv_stride);
+    launch_convert_fused<true, true>(s, L, nq, n_ct, conv, conv_stride, st);
     return;
   }
   if (leaves_ntt) {
-    k_rgsw_top_rows<true><<<dim3(n_ct * 2, nq), kNttThreads, 0, L->st>>>(conv, conv_stride, L->rgsw,
+    k_rgsw_top_rows<true><<<dim3(n_ct * 2, nq), kNttThreads, 0, st>>>(conv, conv_stride, L->rgsw,
                                                                          s->rgsw_words, l, s->Tq);
-    k_cts_to_coeff<<<dim3(n_ct * 2, nq), kNttThreads, 0, L->st>>>(conv, conv_stride, s->Tq);
+    k_cts_to_coeff<<<dim3(n_ct * 2, nq), kNttThreads, 0, st>>>(conv, conv_stride, s->Tq);
   } else {
-    k_rgsw_top_rows<false><<<dim3(n_ct * 2, nq), kNttThreads, 0, L->st>>>(conv, conv_stride, L->rgsw,
+    k_rgsw_top_rows<false><<<dim3(n_ct * 2, nq), kNttThreads, 0, st>>>(conv, conv_stride, L->rgsw,
                                                                           s->rgsw_words, l, s->Tq);
   }
   if (fused) {
     if (s->conv_fold)
-      launch_convert_fused<true, false>(s, L, nq, n_ct, conv, conv_stride);
+      launch_convert_fused<true, false>(s, L, nq, n_ct, conv, conv_stride, st);
     else
-      launch_convert_fused<false, false>(s, L, nq, n_ct, conv, conv_stride);
+      launch_convert_fused<false, false>(s, L, nq, n_ct, conv, conv_stride, st);
     return;
   }
   const dim3 dgrid(n_ct * 2 * lk, nq);
   if (s->conv_fold)
-    k_decomp_ntt<true><<<dgrid, kNttThreads, 0, L->st>>>(conv, nullptr, conv_stride, L->digits, s->digit_words, lk,
+    k_decomp_ntt<true><<<dgrid, kNttThreads, 0, st>>>(conv, nullptr, conv_stride, L->digits, s->digit_words, lk,
                                                          s->I.base_log2_key, s->Tq);
   else
-    k_decomp_ntt<false><<<dgrid, kNttThreads, 0, L->st>>>(conv, nullptr, conv_stride, L->digits, s->digit_words,
+    k_decomp_ntt<false><<<dgrid, kNttThreads, 0, st>>>(conv, nullptr, conv_stride, L->digits, s->digit_words,
                                                           lk, s->I.base_log2_key, s->Tq);
-  k_ext_mac_store<<<dim3(n_ct * 2, nq), kNttThreads, 0, L->st>>>(L->digits, s->digit_words, L->rk_ptrs, 2 * lk,
+  k_ext_mac_store<<<dim3(n_ct * 2, nq), kNttThreads, 0, st>>>(L->digits, s->digit_words, L->rk_ptrs, 2 * lk,
                                                                  L->rgsw, s->rgsw_words, l, s->mq);
 }
 
@@ -516,7 +529,7 @@ void launch_gather(const onion_server *s, Lane *L, uint32_t nq) {
 // run_mux_first instead of transforming them into L->fd.  sel_ntt: the
 // selection vectors are in NTT form already.
 void run_first_dim_mfma(const onion_server *s, Lane *L, uint32_t nq, uint64_t *sel, uint64_t sel_stride,
-                        bool time_core, bool keep_ntt, bool sel_ntt) {
+                        bool time_core, bool keep_ntt, bool sel_ntt, hipEvent_t join) {
   const uint32_t fst = s->P.fst_dim, other = s->P.other_dim, ncp = cpad_for(nq), ncols = 2 * nq;
   uint8_t *Bp = reinterpret_cast<uint8_t *>(L->Qs);
   const dim3 chunk_grid(kN / 8, ncp / 32, fst / 32);
@@ -527,6 +540,7 @@ void run_first_dim_mfma(const onion_server *s, Lane *L, uint32_t nq, uint64_t *s
     k_sel_digits<<<dim3(fst * 2, nq), kNttThreads, 0, L->st>>>(sel, sel_stride, s->Tq);
     k_sel_chunks<false><<<chunk_grid, 256, 0, L->st>>>(sel, sel_stride, Bp, fst, ncols);
   }
+  if (join) (void)hipStreamWaitEvent(L->st, join, 0);
   if (time_core) (void)hipEventRecord(L->ev[6], L->st);
   const uint32_t rb = 128 / ncp;  // row blocks of 32 per workgroup
   const uint32_t n_rt = (other + 32 * rb - 1) / (32 * rb);
@@ -543,11 +557,12 @@ void run_first_dim_mfma(const onion_server *s, Lane *L, uint32_t nq, uint64_t *s
 
 // First dimension: fst selection cts per query (coefficient form, or NTT
 // form with sel_ntt; stride sel_stride) -> L->fd (other cts per query,
-// coefficient form).
+// coefficient form).  join: an event the product waits for after the
+// selection stage (the conversion on the lane's side stream).
 void run_first_dim(const onion_server *s, Lane *L, uint32_t nq, uint64_t *sel, uint64_t sel_stride,
-                   bool time_core, bool keep_ntt = false, bool sel_ntt = false) {
+                   bool time_core, bool keep_ntt = false, bool sel_ntt = false, hipEvent_t join = nullptr) {
   if (use_mfma(s, L, nq)) {
-    run_first_dim_mfma(s, L, nq, sel, sel_stride, time_core, keep_ntt, sel_ntt);
+    run_first_dim_mfma(s, L, nq, sel, sel_stride, time_core, keep_ntt, sel_ntt, join);
     return;
   }
   const uint32_t fst = s->P.fst_dim, other = s->P.other_dim, cpad = cpad_for(nq);
@@ -556,6 +571,7 @@ void run_first_dim(const onion_server *s, Lane *L, uint32_t nq, uint64_t *sel, u
     k_sel_ntt<true><<<dim3(fst * 2, nq), kNttThreads, 0, L->st>>>(sel, sel_stride, L->Qs, fst, cpad, s->Tq);
   else
     k_sel_ntt<false><<<dim3(fst * 2, nq), kNttThreads, 0, L->st>>>(sel, sel_stride, L->Qs, fst, cpad, s->Tq);
+  if (join) (void)hipStreamWaitEvent(L->st, join, 0);
   if (time_core) (void)hipEventRecord(L->ev[6], L->st);
   if (nq == 1) {
     if (fst % 16 == 0) {
@@ -675,10 +691,17 @@ void answer_device(const onion_server *s, Lane *L, uint32_t nq, bool timed) {
   if (timed) (void)hipEventRecord(L->ev[0], L->st);
   const bool leaves_ntt = run_expand(s, L, nq);
   if (timed) (void)hipEventRecord(L->ev[1], L->st);
-  run_convert(s, L, nq, leaves + (size_t)fst * 2 * kN, s->tree_words, leaves_ntt);
-  if (timed) (void)hipEventRecord(L->ev[2], L->st);
+  // the conversion (VALU) runs on the side stream beside the selection stage
+  // (memory); the first-dimension product waits for both.  They read
+  // disjoint leaves and write different buffers (RGSWs and, in split form,
+  // the digit scratch, which the product overwrites only after the join).
+  (void)hipEventRecord(L->fork, L->st);
+  (void)hipStreamWaitEvent(L->st2, L->fork, 0);
+  run_convert(s, L, nq, leaves + (size_t)fst * 2 * kN, s->tree_words, leaves_ntt, L->st2);
+  if (timed) (void)hipEventRecord(L->ev[2], L->st2);
+  (void)hipEventRecord(L->join, L->st2);
   const bool first_fused = mux_first_fused(s, L, nq), fd_ntt = first_dim_keeps_ntt(s, L, nq);
-  run_first_dim(s, L, nq, leaves, s->tree_words, timed, first_fused || fd_ntt, leaves_ntt);
+  run_first_dim(s, L, nq, leaves, s->tree_words, timed, first_fused || fd_ntt, leaves_ntt, L->join);
   if (timed) (void)hipEventRecord(L->ev[3], L->st);
   uint64_t n = s->P.other_dim;
   uint32_t d = 0;
