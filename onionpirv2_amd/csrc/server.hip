@@ -691,17 +691,23 @@ void answer_device(const onion_server *s, Lane *L, uint32_t nq, bool timed) {
   if (timed) (void)hipEventRecord(L->ev[0], L->st);
   const bool leaves_ntt = run_expand(s, L, nq);
   if (timed) (void)hipEventRecord(L->ev[1], L->st);
-  // the conversion (VALU) runs on the side stream beside the selection stage
-  // (memory); the first-dimension product waits for both.  They read
+  // a batch's conversion (VALU) runs on the side stream beside the selection
+  // stage (memory); the first-dimension product waits for both.  They read
   // disjoint leaves and write different buffers (RGSWs and, in split form,
   // the digit scratch, which the product overwrites only after the join).
-  (void)hipEventRecord(L->fork, L->st);
-  (void)hipStreamWaitEvent(L->st2, L->fork, 0);
-  run_convert(s, L, nq, leaves + (size_t)fst * 2 * kN, s->tree_words, leaves_ntt, L->st2);
-  if (timed) (void)hipEventRecord(L->ev[2], L->st2);
-  (void)hipEventRecord(L->join, L->st2);
+  // A single query's stages are too short to gain from the overlap.
+  const bool side = nq > 1;
+  hipStream_t cst = side ? L->st2 : L->st;
+  if (side) {
+    (void)hipEventRecord(L->fork, L->st);
+    (void)hipStreamWaitEvent(L->st2, L->fork, 0);
+  }
+  run_convert(s, L, nq, leaves + (size_t)fst * 2 * kN, s->tree_words, leaves_ntt, cst);
+  if (timed) (void)hipEventRecord(L->ev[2], cst);
+  if (side) (void)hipEventRecord(L->join, L->st2);
   const bool first_fused = mux_first_fused(s, L, nq), fd_ntt = first_dim_keeps_ntt(s, L, nq);
-  run_first_dim(s, L, nq, leaves, s->tree_words, timed, first_fused || fd_ntt, leaves_ntt, L->join);
+  run_first_dim(s, L, nq, leaves, s->tree_words, timed, first_fused || fd_ntt, leaves_ntt,
+                side ? L->join : nullptr);
   if (timed) (void)hipEventRecord(L->ev[3], L->st);
   uint64_t n = s->P.other_dim;
   uint32_t d = 0;
