@@ -107,8 +107,9 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
-              "the database would shard by rows with one 32 KiB gather per GPU, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU: there is no multi-GPU path (asked for --gpus {args.gpus}); "
+              "the database would shard by rows, each GPU answering its rows and one 32 KiB gather "
+              "per GPU combining them", file=sys.stderr)
         return 2
 
     from onionpirv2_amd import GpuServer, dbgen
