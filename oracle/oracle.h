@@ -10,7 +10,7 @@
  * reference repository root) or of the SEAL 4.1 routine the reference calls
  * at that line.  SEAL itself is not available, so its routines are restated
  * from their published algorithms; which of them are pinned by which test is
- * listed in tests/test_oracle_*.py.
+ * listed in tests/test_oracle.py.
  *
  * Conventions
  *   - N-coefficient polynomials are arrays of N uint64.

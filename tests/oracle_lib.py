@@ -227,6 +227,11 @@ class Oracle:
         self.L.orc_db_plaintext(self.c, seed, mode, pt, ptr(co))
         return co
 
+    def plain_to_ntt(self, coeffs):
+        out = u64(self.N)
+        self.L.orc_plain_to_ntt(self.c, ptr(np.ascontiguousarray(coeffs, dtype=np.uint64)), ptr(out))
+        return out
+
     def db_build(self, seed, mode=0):
         db = u64(self.num_pt * self.N)
         self.L.orc_db_build(self.c, seed, mode, ptr(db))

@@ -159,7 +159,7 @@ def test_small_shape_generic_kernel_and_upload():
         srv.db_upload_plain(0, pts)
         db = np.zeros(o.num_pt * N, np.uint64)
         for i in range(o.num_pt):
-            db.reshape(N, o.num_pt)[:, i] = o.ntt(np.where(pts[i] >= (o.t + 1) // 2, pts[i] + o.q - o.t, pts[i]))
+            db.reshape(N, o.num_pt)[:, i] = o.plain_to_ntt(pts[i])
         for i in (0, 5, o.num_pt - 1):
             assert np.array_equal(srv.db_read_ntt(i), db.reshape(N, o.num_pt)[:, i])
         sk = o.keygen(21)
@@ -193,7 +193,7 @@ def test_first_dim_wraps_mod_2_128():
         pts = np.full((o.num_pt, N), o.t - 1, np.uint64)
         srv.db_upload_plain(0, pts)
         db = np.zeros((N, o.num_pt), np.uint64)
-        lifted = o.ntt(np.full(N, o.t - 1 + o.q - o.t, np.uint64))
+        lifted = o.plain_to_ntt(np.full(N, o.t - 1, np.uint64))
         db[:, :] = lifted[:, None]
         wraps = sum(1 for v in lifted if 512 * int(v) * (o.q - 1) >= 2**128)
         assert wraps > 100  # the uint128 sum overflows in many slots
@@ -223,7 +223,7 @@ def test_partial_upload_updates_the_plane_copy():
     db = o.db_build(3, 0).reshape(N, o.num_pt).copy()
     for i in range(n):
         p = pts[i]
-        db[:, lo + i] = o.ntt(np.where(p >= (o.t + 1) // 2, p + o.q - o.t, p))
+        db[:, lo + i] = o.plain_to_ntt(p)
     sel = rng.integers(0, o.q, size=(3, o.fst, 2 * N), dtype=np.uint64)
     with GpuServer(64, 4) as srv:
         assert srv.get_tuning()["db_planes"] == 1

@@ -449,6 +449,66 @@ def test_db_entry_mode_layout():
     assert int(u.max()) < o.t
 
 
+def _balanced_lift(v, t, q):
+    """The residue mod q of the balanced representative of v mod t, i.e. the
+    integer in [-(t-1)/2, (t-1)/2] congruent to v mod t, read mod q."""
+    half = (t - 1) // 2
+    return ((v + half) % t - half) % q
+
+
+def _check_plain_lift(o, lift_ntt, values):
+    """lift_ntt(coeffs) -> NTT-form words.  The words must be the O(N^2)
+    negacyclic evaluation (SEAL order, minimal root) of the balanced lift."""
+    n, q, t = o.N, o.q, o.t
+    logn = n.bit_length() - 1
+    psi = min_root_python(q, n)
+    w = [[pow(psi, (2 * bitrev(j, logn) + 1) * i, q) for i in range(n)] for j in range(n)]
+    for k in range(0, len(values), n):
+        chunk = [values[(k + i) % len(values)] for i in range(n)]
+        lifted = [_balanced_lift(v, t, q) for v in chunk]
+        for v, x in zip(chunk, lifted):
+            signed = x if x < q // 2 else x - q
+            if not (-(t - 1) // 2 <= signed <= (t - 1) // 2 and (signed - v) % t == 0):
+                return False
+        want = [sum(a * wj for a, wj in zip(lifted, w[j])) % q for j in range(n)]
+        if list(map(int, lift_ntt(np.array(chunk, dtype=np.uint64)))) != want:
+            return False
+    return True
+
+
+@pytest.mark.parametrize("plain_bits,n", [(16, 16), (17, 16), (16, 32), (17, 32)])
+def test_plain_lift_is_balanced_representative(plain_bits, n):
+    """SEAL transform_to_ntt_inplace(Plaintext) as PirServer::preprocess_ntt uses
+    it (src/server.cpp:703-711): every v in [0, t) must land on the balanced
+    representative of v mod t before the NTT.  Checked for all of [0, t) at
+    t = 32771 and 65537, with the edges (t-1)/2 and (t+1)/2 in their own polynomial."""
+    o = Oracle(N=n, plain_bits=plain_bits, fst=2, other=1, l=1, l_key=1)
+    t, q = o.t, o.q
+    assert t == {16: 32771, 17: 65537}[plain_bits]
+
+    def oracle_lift(co):
+        out = u64(n)
+        o.L.orc_plain_to_ntt(o.c, ptr(np.ascontiguousarray(co)), ptr(out))
+        return out
+
+    half = (t - 1) // 2
+    edges = [half, half + 1, 0, t - 1, half - 1, half + 2]
+    assert _check_plain_lift(o, oracle_lift, edges)
+    assert _check_plain_lift(o, oracle_lift, list(range(t)))
+
+    # the check rejects the plausible slips: threshold t/2 (rounded down) and
+    # increment q - t + 1, each restated around the oracle's own transform
+    def slipped(thr, inc):
+        def f(co):
+            x = np.array([int(v) + inc if int(v) >= thr else int(v) for v in co], dtype=np.uint64)
+            return o.ntt(x)
+        return f
+
+    assert _check_plain_lift(o, slipped((t + 1) // 2, q - t), edges)
+    assert not _check_plain_lift(o, slipped(t // 2, q - t), edges)
+    assert not _check_plain_lift(o, slipped((t + 1) // 2, q - t + 1), edges)
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 def test_end_to_end_record_recovered(c1, mode):
     o, sk = c1["o"], c1["sk"]
