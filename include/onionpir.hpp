@@ -133,7 +133,10 @@ class PirServer {
 
 class PirClient {
  public:
-  explicit PirClient(const PirParams &pir_params);  // id from rand(), seed from std::random_device
+  // id from rand(); keys, errors and uniform polynomials from ChaCha20 keyed by
+  // 256 bits of operating-system entropy
+  explicit PirClient(const PirParams &pir_params);
+  // Reproducible client from a 64-bit seed: NOT SECURE, for tests and benchmarks
   PirClient(const PirParams &pir_params, size_t client_id, uint64_t seed);
   ~PirClient();
   PirClient(const PirClient &) = delete;

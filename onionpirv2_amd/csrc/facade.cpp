@@ -275,8 +275,9 @@ onion_stage_times PirServer::last_stage_times() const {
 
 // ---- PirClient -----------------------------------------------------------------------
 
-PirClient::PirClient(const PirParams &pir_params)
-    : PirClient(pir_params, (size_t)std::rand(), ((uint64_t)std::random_device{}() << 32) | std::random_device{}()) {}
+PirClient::PirClient(const PirParams &pir_params) : params_(pir_params), id_((size_t)std::rand()) {
+  throw_client(onion_client_create_secure(&params_.c_params(), id_, &cl_));
+}
 
 PirClient::PirClient(const PirParams &pir_params, size_t client_id, uint64_t seed)
     : params_(pir_params), id_(client_id) {

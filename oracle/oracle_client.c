@@ -13,11 +13,12 @@
  * Sampling rules (stream ids are part of the rule):
  *   secret key  s_i  = {0, 1, -1}[floor(r * 3 / 2^64)],   r = rand64(seed, 1, i)
  *   error       e_i  = popcnt(r & M21) - popcnt((r >> 21) & M21),
- *                      r = rand64(seed, 3<<56 | nonce, i)
+ *                      r = rand64(seed, kind_e<<56 | nonce, i)
  *   uniform     a_i  = (r_2i * 2^64 + r_2i+1) mod m,
- *                      r_j = rand64(seed, 2<<56 | limb<<48 | nonce, j)
- * Nonces: query = caller's nonce; Galois key of level a = 0x10000 + a;
- * RGSW key row r = 0x20000 + r.
+ *                      r_j = rand64(seed, kind_a<<56 | limb<<48 | nonce, j)
+ * Queries and test encryptions: kind_a = 2, kind_e = 3, the caller's nonce.
+ * Key material: kind_a = 4, kind_e = 5; Galois key of level a has nonce
+ * 0x10000 + a, RGSW key row r nonce 0x20000 + r.
  */
 #include "oracle.h"
 
@@ -29,23 +30,25 @@ typedef orc_u128 u128;
 #define K_SK 1ULL
 #define K_UNI 2ULL
 #define K_ERR 3ULL
+#define K_KEY_UNI 4ULL
+#define K_KEY_ERR 5ULL
 
 static inline uint64_t add_mod(uint64_t a, uint64_t b, uint64_t m) {
   uint64_t s = a + b;
   return s >= m ? s - m : s;
 }
 
-static void sample_uniform(uint64_t seed, uint64_t nonce, uint64_t limb, uint64_t m, uint32_t N,
-                           uint64_t *out) {
-  const uint64_t st = (K_UNI << 56) | (limb << 48) | nonce;
+static void sample_uniform(uint64_t seed, uint64_t kind, uint64_t nonce, uint64_t limb, uint64_t m,
+                           uint32_t N, uint64_t *out) {
+  const uint64_t st = (kind << 56) | (limb << 48) | nonce;
   for (uint32_t i = 0; i < N; ++i) {
     uint64_t hi = orc_rand64(seed, st, 2 * (uint64_t)i), lo = orc_rand64(seed, st, 2 * (uint64_t)i + 1);
     out[i] = (uint64_t)((((u128)hi << 64) | lo) % m);
   }
 }
 
-static void sample_error(uint64_t seed, uint64_t nonce, uint32_t N, int64_t *out) {
-  const uint64_t st = (K_ERR << 56) | nonce, M21 = (1ULL << 21) - 1;
+static void sample_error(uint64_t seed, uint64_t kind, uint64_t nonce, uint32_t N, int64_t *out) {
+  const uint64_t st = (kind << 56) | nonce, M21 = (1ULL << 21) - 1;
   for (uint32_t i = 0; i < N; ++i) {
     uint64_t r = orc_rand64(seed, st, i);
     out[i] = (int64_t)__builtin_popcountll(r & M21) - (int64_t)__builtin_popcountll((r >> 21) & M21);
@@ -85,26 +88,32 @@ static void sk_mod(const orc_ctx *c, const uint64_t *sk, uint64_t m, uint64_t *o
   for (uint32_t i = 0; i < c->N; ++i) out[i] = sk[i] > 1 ? m - 1 : sk[i];
 }
 
-/* symmetric encryption of zero mod `which`: (c0, c1) = (-a*s + e, a) */
-static void enc_zero_mod(const orc_ctx *c, int which, uint64_t seed, uint64_t nonce,
+/* symmetric encryption of zero mod `which`: (c0, c1) = (-a*s + e, a), a from
+ * the uniform stream of kind `uni` */
+static void enc_zero_mod(const orc_ctx *c, int which, uint64_t seed, uint64_t uni, uint64_t nonce,
                          const uint64_t *sk, const int64_t *e, uint64_t *c0, uint64_t *c1) {
   const uint32_t N = c->N;
   const uint64_t m = which == 0 ? c->q : c->p;
   uint64_t *s = malloc(sizeof(uint64_t) * N), *as = malloc(sizeof(uint64_t) * N);
   sk_mod(c, sk, m, s);
-  sample_uniform(seed, nonce, (uint64_t)which, m, N, c1);
+  sample_uniform(seed, uni, nonce, (uint64_t)which, m, N, c1);
   poly_mul(c, which, c1, s, as);
   for (uint32_t i = 0; i < N; ++i) c0[i] = add_mod(as[i] ? m - as[i] : 0, signed_to_mod(e[i], m), m);
   free(s);
   free(as);
 }
 
+static void encrypt_zero_kind(const orc_ctx *c, uint64_t seed, int key, uint64_t nonce, const uint64_t *sk,
+                              uint64_t *ct) {
+  int64_t *e = malloc(sizeof(int64_t) * c->N);
+  sample_error(seed, key ? K_KEY_ERR : K_ERR, nonce, c->N, e);
+  enc_zero_mod(c, 0, seed, key ? K_KEY_UNI : K_UNI, nonce, sk, e, ct, ct + c->N);
+  free(e);
+}
+
 void orc_client_encrypt_zero(const orc_ctx *c, uint64_t seed, uint64_t nonce, const uint64_t *sk,
                              uint64_t *ct) {
-  int64_t *e = malloc(sizeof(int64_t) * c->N);
-  sample_error(seed, nonce, c->N, e);
-  enc_zero_mod(c, 0, seed, nonce, sk, e, ct, ct + c->N);
-  free(e);
+  encrypt_zero_kind(c, seed, 0, nonce, sk, ct);
 }
 
 /* BFV symmetric encryption: c0 += round(q * m / t) */
@@ -134,9 +143,9 @@ void orc_client_galois_keys(const orc_ctx *c, uint64_t seed, const uint64_t *sk,
     const uint64_t nonce = 0x10000 + a;
     uint64_t *key = gk + (size_t)a * 4 * N;
     orc_galois_coeff(c, sk, g, c->q, gs);
-    sample_error(seed, nonce, N, e);
+    sample_error(seed, K_KEY_ERR, nonce, N, e);
     for (int limb = 0; limb < 2; ++limb) {
-      enc_zero_mod(c, limb, seed, nonce, sk, e, c0, c1);
+      enc_zero_mod(c, limb, seed, K_KEY_UNI, nonce, sk, e, c0, c1);
       const uint64_t m = limb == 0 ? c->q : c->p;
       if (limb == 0)
         for (uint32_t i = 0; i < N; ++i) c0[i] = add_mod(c0[i], orc_mulmod(gs[i], p_mod_q, m), m);
@@ -161,7 +170,7 @@ void orc_client_rgsw_key(const orc_ctx *c, uint64_t seed, const uint64_t *sk, ui
   for (uint32_t half = 0; half < 2; ++half)
     for (uint32_t k = 0; k < L; ++k) {
       uint64_t *row = rk + (size_t)(half * L + k) * 2 * N;
-      orc_client_encrypt_zero(c, seed, 0x20000 + half * L + k, sk, row);
+      encrypt_zero_kind(c, seed, 1, 0x20000 + half * L + k, sk, row);
       uint64_t *dst = row + (size_t)half * N;
       for (uint32_t i = 0; i < N; ++i) dst[i] = add_mod(dst[i], orc_mulmod(sk[i], g[k], c->q), c->q);
     }
