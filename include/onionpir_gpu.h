@@ -202,6 +202,19 @@ int onion_gpu_autotune(onion_server *srv, const uint32_t *batch_sizes, uint32_t 
 int onion_gpu_set_lanes(onion_server *srv, uint32_t n_lanes);
 int onion_gpu_answer_batch_lane(onion_server *srv, uint32_t lane, uint32_t n, const uint64_t *client_ids,
                                 const uint64_t *queries, uint64_t *resps, onion_stage_times *times);
+/* onion_gpu_answer_batch without allocating (no reference counterpart; for
+ * callers that cannot allocate device memory: tight serving loops and
+ * harnesses that forbid hipMalloc).  The per-query scratch comes from a
+ * 256 MiB static device array that every GPU the library runs on holds from
+ * its first use until the process ends, instead of from lane buffers; the
+ * pipeline runs eagerly (never as a captured graph, whose instantiation
+ * allocates).  n queries must fit the array: ONION_EINVAL otherwise (at
+ * 512x1024 one query, about 195 MiB).  Calls on one GPU share the array and
+ * run one at a time within the process; the array keeps the last call's
+ * contents, which no call reads.  Results are identical to
+ * onion_gpu_answer_batch. */
+int onion_gpu_answer_batch_noalloc(onion_server *srv, uint32_t n, const uint64_t *client_ids,
+                                   const uint64_t *queries, uint64_t *resps);
 /* Device bytes of per-query scratch a lane allocates for each query of its
  * largest batch (the unit of admission by device-memory budget). */
 int onion_gpu_scratch_bytes_per_query(const onion_server *srv, uint64_t *bytes);

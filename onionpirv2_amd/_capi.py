@@ -85,6 +85,7 @@ _SIGS = {
     "onion_gpu_set_lanes": (C.c_int, [C.c_void_p, C.c_uint32]),
     "onion_gpu_answer_batch_lane": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, U64P, U64P, U64P,
                                               C.POINTER(OnionStageTimes)]),
+    "onion_gpu_answer_batch_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, U64P, U64P, U64P]),
     "onion_gpu_scratch_bytes_per_query": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "onion_gpu_reserve": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
     "onion_gpu_tap_expand": (C.c_int, [C.c_void_p, C.c_uint64, U64P, U64P]),

@@ -138,6 +138,7 @@ struct Lane {
   std::map<uint32_t, hipGraphExec_t> graphs;  // by batch size
   uint64_t graph_epoch = 0;                   // tuning epoch the graphs were captured under
   std::mutex mu;                              // one batch at a time per lane
+  bool borrowed = false;  // buffers are carved from the static scratch, never freed
 };
 
 struct onion_server {
@@ -171,6 +172,7 @@ struct onion_server {
   // per-query scratch strides in words
   uint64_t tree_words = 0, rgsw_words = 0, fd_words = 0, digit_words = 0;
   std::vector<std::unique_ptr<Lane>> lanes;  // lane 0 also serves taps and ingest
+  std::unique_ptr<Lane> slane;               // lane of onion_gpu_answer_batch_noalloc
   uint64_t *io = nullptr;                    // staging for taps and ingest (lane 0)
   size_t io_words = 0;
   uint8_t *pack = nullptr;
@@ -223,7 +225,8 @@ void drop_graphs(Lane *L) {
 void free_scratch(Lane *L) {
   drop_graphs(L);
   void *bufs[] = {L->trees, L->rgsw, L->fd, L->digits, L->Qs, L->gk_ptrs, L->rk_ptrs};
-  for (void *b : bufs) dfree(b);
+  if (!L->borrowed)
+    for (void *b : bufs) dfree(b);
   L->trees = L->rgsw = L->fd = L->digits = nullptr;
   L->Qs = nullptr;
   L->gk_ptrs = L->rk_ptrs = nullptr;
@@ -284,6 +287,73 @@ uint64_t scratch_bytes_per_query(const onion_server *s) {
   const uint64_t words = s->tree_words + s->rgsw_words + s->fd_words + s->digit_words;
   const uint64_t qs = (uint64_t)kN * s->P.fst_dim * cpad_for(kMaxBatch) * 2 * 4;
   return words * 8 + 2 * sizeof(void *) + (qs + kMaxBatch - 1) / kMaxBatch;
+}
+
+// ---- the static scratch of onion_gpu_answer_batch_noalloc ---------------------------
+//
+// A file-scope device array: every GPU the library runs on holds its own copy
+// from the library's first use on that GPU until the process ends.  Host code
+// never passes `scratch` itself (in host code the name is a stand-in, not a
+// device address): onion_gpu_create resolves the array's address on its
+// device once with hipGetSymbolAddress and the no-allocation entry point uses
+// that address.  Nothing in it needs to start at zero: the pipeline writes
+// every word before it reads it, as it does in freshly allocated lane scratch.
+constexpr uint64_t kStaticScratchBytes = uint64_t(256) << 20;
+constexpr int kMaxDevices = 64;
+
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+
+namespace {
+
+std::mutex g_static_mu[kMaxDevices];      // one no-allocation call at a time per device
+void *g_static_ptr[kMaxDevices] = {};     // scratch's address on each device
+std::string g_static_err[kMaxDevices];    // why it could not be resolved
+bool g_static_tried[kMaxDevices] = {};
+std::mutex g_static_init_mu;
+
+// Resolve scratch's address on device `dev` (current) once per device; a
+// failure is kept for the no-allocation entry point to report and does not
+// stop the server's other entry points.
+void static_scratch_init(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return;
+  std::lock_guard<std::mutex> g(g_static_init_mu);
+  if (g_static_tried[dev]) return;
+  g_static_tried[dev] = true;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    g_static_err[dev] = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+    return;
+  }
+  g_static_ptr[dev] = p;
+}
+
+// Bytes of static scratch a batch of n queries takes (each buffer 256-byte
+// aligned): the lane buffers of ensure_scratch for cap = n.
+uint64_t static_bytes(const onion_server *s, uint32_t n) {
+  auto al = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
+  return al(n * s->tree_words * 8) + al(n * s->rgsw_words * 8) + al(n * s->fd_words * 8) +
+         al(n * s->digit_words * 8) + al((uint64_t)kN * s->P.fst_dim * cpad_for(n) * 2 * 4) +
+         2 * al(n * sizeof(void *));
+}
+
+// Point the static lane's buffers at the static scratch for n queries.
+void carve_static(const onion_server *s, Lane *L, uint32_t n, uint8_t *base) {
+  auto take = [&](uint64_t bytes) {
+    uint8_t *p = base;
+    base += (bytes + 255) & ~uint64_t(255);
+    return p;
+  };
+  L->trees = (uint64_t *)take(n * s->tree_words * 8);
+  L->rgsw = (uint64_t *)take(n * s->rgsw_words * 8);
+  L->fd = (uint64_t *)take(n * s->fd_words * 8);
+  L->digits = (uint64_t *)take(n * s->digit_words * 8);
+  L->Qs = (uint32_t *)take((uint64_t)kN * s->P.fst_dim * cpad_for(n) * 2 * 4);
+  L->gk_ptrs = (const uint64_t **)take(n * sizeof(void *));
+  L->rk_ptrs = (const uint64_t **)take(n * sizeof(void *));
+  L->cap = n;
 }
 
 // The caller holds the key store's gate (shared or exclusive).
@@ -782,13 +852,15 @@ int collect_times(onion_server *s, Lane *L, bool staged, onion_stage_times *out)
 }
 
 // Answer n (<= L->cap) queries on lane L: upload, run, download.
+// With eager_only, the pipeline is never captured (hipGraphInstantiate
+// allocates).
 int answer_chunk(onion_server *s, Lane *L, uint32_t n, const uint64_t *ids, const uint64_t *queries,
-                 uint64_t *resps, onion_stage_times *times) {
+                 uint64_t *resps, onion_stage_times *times, bool eager_only = false) {
   int rc;
   if ((rc = bind_keys(s, L, n, ids, s->I.ndim > 1))) return rc;
   HIPCHK(hipMemcpy2DAsync(L->trees + 2 * kN, s->tree_words * 8, queries, 2 * kN * 8, 2 * kN * 8, n,
                           hipMemcpyHostToDevice, L->st));
-  const bool graph = s->tune.use_graph != 0;
+  const bool graph = s->tune.use_graph != 0 && !eager_only;
   if (graph) {
     hipGraphExec_t ex;
     if ((rc = graph_for(s, L, n, &ex))) return rc;
@@ -1020,11 +1092,15 @@ int onion_gpu_create(const onion_params *params, onion_server **out) {
     s->n_cu = (uint32_t)cu;
   }
   s->lanes.emplace_back();
-  if ((rc = make_lane(s->lanes[0]))) {
+  if ((rc = make_lane(s->lanes[0])) || (rc = make_lane(s->slane))) {
     std::string msg = g_err;
     onion_gpu_destroy(s);
     return fail(rc, msg);
   }
+  static_scratch_init(s->device);
+  s->slane->borrowed = true;
+  s->slane->h_gk.assign(kMaxBatch, nullptr);
+  s->slane->h_rk.assign(kMaxBatch, nullptr);
 
   // NTT tables and constants
   host::Tables tq = host::make_tables(I.q, kN), tp = host::make_tables(I.p, kN);
@@ -1122,6 +1198,8 @@ int onion_gpu_destroy(onion_server *s) {
     for (auto &L : s->lanes)
       if (L) destroy_lane(L.get());
     s->lanes.clear();
+    if (s->slane) destroy_lane(s->slane.get());
+    s->slane.reset();
     s->keys.reset();  // the store goes with its last server
     void *bufs[] = {s->tabs, s->db, s->dbp, s->io, s->pack};
     for (void *b : bufs) dfree(b);
@@ -1347,6 +1425,33 @@ int onion_gpu_answer_batch_lane(onion_server *s, uint32_t lane, uint32_t n, cons
   Lane *L = s->lanes[lane].get();
   std::lock_guard<std::mutex> lk(L->mu);
   return answer_on_lane(s, L, n, ids, queries, resps, times);
+}
+
+int onion_gpu_answer_batch_noalloc(onion_server *s, uint32_t n, const uint64_t *ids, const uint64_t *queries,
+                                   uint64_t *resps) {
+  if (!s || !ids || !queries || !resps) return fail(ONION_EINVAL, "null argument");
+  if (n == 0 || n > kMaxBatch) return fail(ONION_EINVAL, "n must be in [1, 64]");
+  const uint64_t need = static_bytes(s, n);
+  if (need > kStaticScratchBytes) {
+    uint32_t fit = 0;
+    while (fit < kMaxBatch && static_bytes(s, fit + 1) <= kStaticScratchBytes) ++fit;
+    return fail(ONION_EINVAL, std::to_string(n) + " queries need " + std::to_string(need) +
+                                  " bytes of scratch; the static scratch holds " +
+                                  std::to_string(kStaticScratchBytes) + " (" + std::to_string(fit) +
+                                  " queries of this database)");
+  }
+  Shared g(s->gate);
+  Shared kg(s->keys->gate);
+  int rc;
+  if ((rc = check_db(s))) return rc;
+  if (s->device < 0 || s->device >= kMaxDevices) return fail(ONION_EHIP, "device index beyond the static scratch table");
+  void *base = g_static_ptr[s->device];
+  if (!base) return fail(ONION_EHIP, g_static_err[s->device]);
+  std::lock_guard<std::mutex> dev(g_static_mu[s->device]);
+  Lane *L = s->slane.get();
+  std::lock_guard<std::mutex> lk(L->mu);
+  carve_static(s, L, n, (uint8_t *)base);
+  return answer_chunk(s, L, n, ids, queries, resps, nullptr, true);
 }
 
 int onion_gpu_pack_response(onion_server *s, const uint64_t *resp, uint8_t *out, size_t cap,

@@ -109,6 +109,15 @@ class GpuServer:
         check(self._L.onion_gpu_answer_batch(self._h, ids.size, u64ptr(ids), u64ptr(q), u64ptr(out)))
         return out.reshape(ids.size, 2 * self.N)
 
+    def answer_batch_noalloc(self, client_ids, queries) -> np.ndarray:
+        """answer_batch with its scratch in the library's static device array
+        (no device allocation; the batch must fit the array)."""
+        ids = _u64(client_ids).reshape(-1)
+        q = _u64(queries).reshape(-1)
+        out = np.zeros(ids.size * 2 * self.N, np.uint64)
+        check(self._L.onion_gpu_answer_batch_noalloc(self._h, ids.size, u64ptr(ids), u64ptr(q), u64ptr(out)))
+        return out.reshape(ids.size, 2 * self.N)
+
     def set_lanes(self, n_lanes: int) -> None:
         check(self._L.onion_gpu_set_lanes(self._h, n_lanes))
 
