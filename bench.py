@@ -18,10 +18,11 @@ kernels run on), two batches in flight on two lanes, the serving loop under Pois
 arrivals (p50/p99 from submit to response), three databases of different
 shape resident together (128x256, 256x1024, 512x4096) answering bursts of
 7 / 33 / 100 queries, the same plus 512x8192 (100.5 GiB resident) with 1000
-clients' keys in one shared device key store under Poisson arrivals, a
-256x512 single-query shape, and the CPU oracle timed on this host on a
-bounded sample (one core, then one oracle server per core), whose responses
-must equal the GPU's bit for bit.
+clients' keys in one shared device key store under Poisson arrivals (50 to
+2000 queries/s), a 256x512 single-query shape, the batch at t = 65537, and
+the CPU oracle timed on this host on a bounded sample (3 untimed and 10 timed
+queries on one core, built -march=native here, then one oracle server per
+usable CPU up to 16), whose responses must equal the GPU's bit for bit.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extras]
 """
@@ -89,10 +90,12 @@ def main() -> int:
     ap.add_argument("--single-reps", type=int, default=20)
     ap.add_argument("--db-seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-queries", type=int, default=2)
+    ap.add_argument("--cpu-queries", type=int, default=10, help="measured oracle queries (1 core)")
+    ap.add_argument("--cpu-warmup", type=int, default=3, help="untimed oracle queries before them")
     ap.add_argument("--plain-bits", type=int, default=16)
-    ap.add_argument("--cpu-cores", type=int, default=16,
-                    help="threads of the multi-core CPU baseline (one oracle server each); 1 = skip")
+    ap.add_argument("--cpu-cores", type=int, default=0,
+                    help="threads of the multi-core CPU baseline (one oracle server each); 0 = the CPUs this "
+                         "process may run on, at most 16 (a GPU box's share); 1 = skip")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the built-in launch shapes instead of tuning on the live database")
     ap.add_argument("--no-extras", action="store_true",
@@ -101,7 +104,11 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--config5-clients", type=int, default=1000,
                     help="clients of the four-database serving workload (0 = skip it)")
-    ap.add_argument("--config5-rates", default="50,200,500,1000", help="its Poisson rates (queries/s)")
+    ap.add_argument("--config5-rates", default="50,200,500,1000,2000", help="its Poisson rates (queries/s)")
+    ap.add_argument("--config5-workers", type=int, default=2,
+                    help="batches in flight over the four databases (the serving loop's worker pool)")
+    ap.add_argument("--no-t65537", dest="t65537", action="store_false",
+                    help="skip the batch-64 line at t = 65537 (PlainMod 17)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (profiler byte-counter scale)")
     args = ap.parse_args()
@@ -283,6 +290,8 @@ def main() -> int:
         del srv, clients  # free the 8 GiB database before the other workloads
         out["config_256x512"] = single_shape(args, 256, 512)
         ok_all = ok_all and out["config_256x512"]["bit_exact"]
+        if args.t65537:
+            out["batched_t65537"] = batched_t65537(args, tuning)
         out["multi_db"] = multi_db(args)
         ok_all = ok_all and out["multi_db"]["bit_exact"]
         if args.config5_clients:
@@ -461,7 +470,7 @@ def config5(args):
 
     rates = [float(r) for r in args.config5_rates.split(",") if r]
     runs, ok_all = [], True
-    with GpuServe(max_batch=64, lanes_per_db=1, scratch_budget=120 << 30) as sv:
+    with GpuServe(max_batch=64, lanes_per_db=1, scratch_budget=120 << 30, workers=args.config5_workers) as sv:
         caps = [sv.add_db(s)[1] for s in servers]
         for rate in rates:
             n = max(40, int(rate * args.serve_seconds))
@@ -491,7 +500,8 @@ def config5(args):
             for d in range(4):
                 sd = sv.stats(d)
                 per_db.append({"shape": f"{shapes[d][0]}x{shapes[d][1]}", "queries": sd["completed"],
-                               "p50_ms": sd["p50_ms"], "p99_ms": sd["p99_ms"], "mean_batch": sd["mean_batch"]})
+                               "queries_per_s": sd["queries_per_s"], "p50_ms": sd["p50_ms"], "p99_ms": sd["p99_ms"],
+                               "mean_batch": sd["mean_batch"]})
             ok_all = ok_all and good == n
             log(f"[bench] config5 at {rate:.0f} q/s: {st['queries_per_s']:.1f} q/s done, p50 {st['p50_ms']:.1f} ms "
                 f"p99 {st['p99_ms']:.1f} ms, {good}/{n} records")
@@ -503,7 +513,7 @@ def config5(args):
         s.close()
     return {"resident_GiB": resident / 2**30, "shapes": [f"{a}x{b}" for a, b in shapes], "clients": nc,
             "key_store_GiB": key_bytes / 2**30, "batch_caps": caps, "scratch_budget_GiB": 120,
-            "runs": runs, "bit_exact": bool(ok_all)}
+            "workers": args.config5_workers, "runs": runs, "bit_exact": bool(ok_all)}
 
 
 def multi_db(args):
@@ -553,7 +563,8 @@ def multi_db(args):
             for d in range(3):
                 st = sv.stats(d)
                 per_db.append({"shape": f"{shapes[d][0]}x{shapes[d][1]}", "queries": st["completed"],
-                               "p50_ms": st["p50_ms"], "max_ms": st["max_ms"], "batches": st["batches"]})
+                               "queries_per_s": st["queries_per_s"], "p50_ms": st["p50_ms"], "max_ms": st["max_ms"],
+                               "batches": st["batches"], "mean_batch": st["mean_batch"]})
             log(f"[bench] multi-db burst {burst}: {wall * 1e3:.1f} ms, {good}/{burst} records")
             bursts.append({"burst": burst, "wall_ms": wall * 1e3, "queries_per_s": burst / wall,
                            "records_ok": good, "per_db": per_db})
@@ -563,6 +574,77 @@ def multi_db(args):
             "bit_exact": bool(ok)}
 
 
+def batched_t65537(args, tuning):
+    """The headline batch at t = 65537 (PlainMod 17, the configurations'
+    "t = 2^17"): same shape, seeds and tuning, device time per batch.  At this
+    t the 512x1024 shape is at the edge of the noise budget, so a query can
+    fail to decrypt (the oracle's response is the same); the count is
+    reported."""
+    from onionpirv2_amd import GpuServer, dbgen
+    from onionpirv2_amd.client import GpuClient
+    fst, other, B = args.fst, args.other, args.batch
+    s = GpuServer(fst, other, plain_bits=17)
+    try:
+        s.db_generate(args.db_seed, 0)
+        s.set_tuning(**tuning)
+        num_pt, t = s.info["num_pt"], s.info["t"]
+        idx = np.random.default_rng(12345).choice(num_pt, size=B, replace=False)
+        clients, qs = [], []
+        for i in range(B):
+            c = GpuClient(fst, other, client_id=i, seed=1000 + i, plain_bits=17)
+            s.set_galois_key(i, c.galois_keys())
+            s.set_rgsw_key(i, c.rgsw_key())
+            clients.append(c)
+            qs.append(c.query(int(idx[i]), nonce=i))
+        qs = np.stack(qs)
+        ids = np.arange(B, dtype=np.uint64)
+        for _ in range(args.warmup):
+            s.answer_batch(ids, qs)
+        dev = []
+        for _ in range(args.steps):
+            resps = s.answer_batch(ids, qs)
+            dev.append(s.stage_times()["total"])
+        ok = sum(int(np.array_equal(clients[i].decrypt(resps[i]), dbgen.plaintext(args.db_seed, 0, int(idx[i]), t=t)))
+                 for i in range(B))
+    finally:
+        s.close()
+    log(f"[bench] t = 65537: {B * 1e3 / np.mean(dev):.1f} queries/s, {ok}/{B} records decrypt")
+    return {"plain_modulus": t, "batch": B, "queries_per_s": B * 1e3 / float(np.mean(dev)),
+            "step_ms": float(np.mean(dev)), "records_ok": ok}
+
+
+def host_cpu():
+    """Model name and CPU counts of this host."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count(), "cpus_usable": usable}
+
+
+def native_oracle():
+    """Build the oracle for this host's own ISA (-march=native, the
+    reference's flags) and point the oracle loader at it; the tests keep the
+    portable build.  Returns the build's flags."""
+    import subprocess
+    try:
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native"], check=True,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=300)
+        os.environ["ONION_ORACLE_LIB"] = os.path.join(ROOT, "oracle", "_build", "liboracle_native.so")
+        return "gcc -O3 -march=native"
+    except Exception as e:  # noqa: BLE001
+        log(f"[bench] native oracle build failed ({e!r}); portable build used")
+        return "gcc -O3 -march=x86-64-v3"
+
+
 def cpu_baseline_fresh(args, queries, idx, single_resp):
     """cpu_baseline against the single-query GPU responses already taken."""
     class _Resp:
@@ -570,22 +652,26 @@ def cpu_baseline_fresh(args, queries, idx, single_resp):
             return single_resp[i]
     from onionpirv2_amd.client import GpuClient
     clients = [GpuClient(args.fst, args.other, client_id=i, seed=1000 + i, plain_bits=args.plain_bits)
-               for i in range(args.cpu_queries)]
+               for i in range(args.cpu_warmup + args.cpu_queries)]
     return cpu_baseline(args, clients, queries, idx, _Resp())
 
 
 def cpu_baseline(args, clients, queries, idx, srv):
     """The CPU oracle (oracle/, a C restatement of the reference server, one
-    core) on the same database and queries; its responses must equal the GPU's."""
+    core, built with -march=native on this host) on the same database and
+    queries: cpu_warmup untimed queries, then cpu_queries timed ones (the
+    reference's protocol, src/tests.cpp:16); every response must equal the
+    GPU's."""
     import concurrent.futures as cf
 
+    flags = native_oracle()
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle, ptr
 
     o = Oracle(fst=args.fst, other=args.other, plain_bits=args.plain_bits)
     t0 = time.time()
     db = np.zeros(o.num_pt * N, np.uint64)
-    workers = min(16, os.cpu_count() or 1)
+    workers = max(1, min(16, host_cpu()["cpus_usable"]))
     chunk = (o.num_pt + workers - 1) // workers
     with cf.ThreadPoolExecutor(workers) as ex:
         list(ex.map(lambda w: o.L.orc_db_build_range(o.c, args.db_seed, 0, w * chunk,
@@ -593,20 +679,23 @@ def cpu_baseline(args, clients, queries, idx, srv):
                     range(workers)))
     log(f"[bench] oracle DB built in {time.time() - t0:.1f} s ({workers} threads)")
     times, match = [], 0
-    for i in range(args.cpu_queries):
+    n = args.cpu_warmup + args.cpu_queries
+    for i in range(n):
         c = clients[i]
         gk, rkn = c.galois_keys(), o.rgsw_key_ntt(c.rgsw_key())
         resp, tm = o.answer(db, gk, rkn, queries[i])
-        times.append(tm["total"])
+        if i >= args.cpu_warmup:
+            times.append(tm["total"])
         match += int(np.array_equal(resp, srv.answer(i, queries[i])))
-        log(f"[bench] oracle query {i}: {tm['total']:.2f} s "
+        log(f"[bench] oracle query {i}{' (warm-up)' if i < args.cpu_warmup else ''}: {tm['total']:.2f} s "
             + " ".join(f"{k}={v:.2f}" for k, v in tm.items()))
     out = {"value": 1.0 / float(np.mean(times)), "unit": "queries/s", "cores": 1, "kind": "port",
-           "sample": f"{args.cpu_queries} full single queries of the same workload (same DB, clients, "
-                     f"queries) through the C oracle, 1 thread",
-           "seconds_per_query": float(np.mean(times)),
-           "responses_equal_gpu": f"{match}/{args.cpu_queries}"}
-    if args.cpu_cores > 1:
+           "sample": f"{args.cpu_warmup} untimed + {args.cpu_queries} timed full single queries of the same "
+                     f"workload (same DB, clients, queries) through the C oracle, 1 thread",
+           "seconds_per_query": float(np.mean(times)), "p50_s": pct(times, 50), "p99_s": pct(times, 99),
+           "build": flags, **host_cpu(),
+           "responses_equal_gpu": f"{match}/{n}"}
+    if args.cpu_cores != 1:
         out["multicore"] = cpu_multicore(args, o, db, clients, queries, srv)
     return out
 
@@ -618,7 +707,7 @@ def cpu_multicore(args, o, db, clients, queries, srv):
 
     from oracle_lib import Oracle
 
-    k = max(1, min(args.cpu_cores, os.cpu_count() or 1))
+    k = args.cpu_cores if args.cpu_cores > 1 else max(1, min(16, host_cpu()["cpus_usable"]))
     n = len(clients)
     orcs = [Oracle(fst=args.fst, other=args.other, plain_bits=args.plain_bits) for _ in range(k)]
     keys = [(c.galois_keys(), o.rgsw_key_ntt(c.rgsw_key())) for c in clients]
@@ -631,8 +720,9 @@ def cpu_multicore(args, o, db, clients, queries, srv):
     log(f"[bench] oracle on {k} cores: {k} queries in {wall:.2f} s")
     return {"value": k / wall, "unit": "queries/s", "cores": k, "kind": "port",
             "sample": f"{k} full single queries of the same workload, one per thread, each on its own "
-                      f"oracle server over the shared database",
-            "responses_equal_gpu": f"{match}/{k}"}
+                      f"oracle server over the shared database (threads: the CPUs this process may use, at "
+                      f"most 16, a GPU box's CPU share)",
+            **host_cpu(), "responses_equal_gpu": f"{match}/{k}"}
 
 
 if __name__ == "__main__":

@@ -62,6 +62,7 @@ typedef struct onion_param_info {
   uint32_t galois_key_words;  /* h * 4 * N                                         */
   uint32_t rgsw_key_words;    /* 2 * l_key * 2 * N                                 */
   uint32_t response_bytes;    /* packed response size                              */
+  uint32_t poly_degree;       /* N (a ciphertext is 2N words)                      */
 } onion_param_info;
 
 /* Per-stage device time of the last answer, milliseconds, section names of the
@@ -216,8 +217,11 @@ int onion_gpu_answer_batch_lane(onion_server *srv, uint32_t lane, uint32_t n, co
 int onion_gpu_answer_batch_noalloc(onion_server *srv, uint32_t n, const uint64_t *client_ids,
                                    const uint64_t *queries, uint64_t *resps);
 /* Device bytes of per-query scratch a lane allocates for each query of its
- * largest batch (the unit of admission by device-memory budget). */
+ * largest batch (a 64-query lane's scratch divided by 64). */
 int onion_gpu_scratch_bytes_per_query(const onion_server *srv, uint64_t *bytes);
+/* Device bytes of the scratch one lane allocates for batches of up to cap
+ * queries (1..64; the unit of admission by device-memory budget). */
+int onion_gpu_scratch_bytes(const onion_server *srv, uint32_t cap, uint64_t *bytes);
 /* Allocate lane `lane`'s scratch for batches of up to n_queries (capped at 64)
  * now rather than on first use; fails with ONION_ENOMEM when it does not fit. */
 int onion_gpu_reserve(onion_server *srv, uint32_t lane, uint32_t n_queries);

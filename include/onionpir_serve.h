@@ -9,10 +9,13 @@
  * needs:
  *
  *   - several databases (onion_server handles of any shape) served together;
- *   - per database, `lanes_per_db` worker threads, each driving one lane of the
- *     server (its own HIP stream and scratch), so batches overlap on the GPU;
- *   - continuous batching: a free lane takes every waiting query of its
- *     database, up to the database's batch cap, as one batch;
+ *   - one pool of worker threads for all databases, each driving a free lane
+ *     of the server it picks (its own HIP stream and scratch), so batches
+ *     overlap on the GPU; a free worker serves the database whose oldest
+ *     waiting query is the oldest of all, so no database waits behind
+ *     another's backlog;
+ *   - continuous batching: a worker takes every waiting query of the
+ *     database it picks, up to the database's batch cap, as one batch;
  *   - admission by device-memory budget: a database's batch cap is the largest
  *     that fits its lanes' scratch into what remains of the budget, and the
  *     scratch is allocated when the database is added;
@@ -41,9 +44,11 @@ typedef struct onion_serve_config {
   uint32_t lanes_per_db;   /* concurrent pipelines per database (1..16)              */
   uint64_t scratch_budget; /* device bytes of lane scratch over all databases;
                               0 = no limit beyond max_batch                        */
-  uint32_t linger_us;      /* a lane with fewer than its cap waiting waits up to
-                              this long after the oldest arrival for more (0 =
-                              launch at once)                                      */
+  uint32_t linger_us;      /* a database with fewer than its cap waiting is left
+                              up to this long after its oldest arrival for more
+                              (0 = launch at once)                                */
+  uint32_t workers;        /* batches in flight over all databases (1..64); 0 =
+                              one per lane of every database added             */
 } onion_serve_config;
 
 typedef struct onion_serve_stats {
@@ -65,9 +70,10 @@ int onion_serve_create(const onion_serve_config *cfg, onion_serve **out);
 int onion_serve_destroy(onion_serve *sv);
 
 /* Serve `srv` (not owned; it must outlive sv and hold a complete database).
- * Sets its lane count to lanes_per_db and reserves the scratch of every lane;
- * ONION_ENOMEM when the remaining budget does not fit one query per lane.
- * *db_id numbers the database for submit; *batch_cap gets the admitted cap. */
+ * Sets its lane count to lanes_per_db and reserves the scratch of every lane
+ * for the largest batch cap (<= max_batch) whose lanes fit the remaining
+ * budget; ONION_ENOMEM when not even one query per lane fits.  *db_id
+ * numbers the database for submit; *batch_cap gets the admitted cap. */
 int onion_serve_add_db(onion_serve *sv, onion_server *srv, uint32_t *db_id, uint32_t *batch_cap);
 
 /* Queue one query (2*N uint64, copied) of client_id against database db_id. */
@@ -79,6 +85,10 @@ int onion_serve_submit(onion_serve *sv, uint32_t db_id, uint64_t client_id, cons
  * failed query returns its error code and retires the ticket too;
  * ONION_ETIMEDOUT leaves it pending. */
 int onion_serve_wait(onion_serve *sv, uint64_t ticket, uint64_t *resp, uint32_t timeout_ms);
+
+/* Retire a ticket without its response (a pending query still runs; its
+ * result is dropped).  ONION_EINVAL for an unknown ticket. */
+int onion_serve_discard(onion_serve *sv, uint64_t ticket);
 
 /* Statistics of database db_id, or over all databases for db_id = UINT32_MAX. */
 int onion_serve_stats_get(onion_serve *sv, uint32_t db_id, onion_serve_stats *out);

@@ -29,7 +29,7 @@ class OnionParamInfo(C.Structure):
                 ("base_log2", C.c_uint32), ("base_log2_key", C.c_uint32), ("ndim", C.c_uint32),
                 ("expan_height", C.c_uint32), ("useful_cnt", C.c_uint32), ("num_pt", C.c_uint64),
                 ("galois_key_words", C.c_uint32), ("rgsw_key_words", C.c_uint32),
-                ("response_bytes", C.c_uint32)]
+                ("response_bytes", C.c_uint32), ("poly_degree", C.c_uint32)]
 
 
 class OnionStageTimes(C.Structure):
@@ -87,6 +87,7 @@ _SIGS = {
                                               C.POINTER(OnionStageTimes)]),
     "onion_gpu_answer_batch_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, U64P, U64P, U64P]),
     "onion_gpu_scratch_bytes_per_query": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "onion_gpu_scratch_bytes": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64)]),
     "onion_gpu_reserve": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
     "onion_gpu_tap_expand": (C.c_int, [C.c_void_p, C.c_uint64, U64P, U64P]),
     "onion_gpu_tap_convert": (C.c_int, [C.c_void_p, C.c_uint64, U64P, U64P]),

@@ -74,6 +74,7 @@ inline int derive_params(const onion_params &P, onion_param_info &I, std::string
   I.rgsw_key_words = 2 * P.gsw_l_key * 2 * (uint32_t)N;
   const uint32_t wbits = (uint32_t)std::ceil(std::log2((double)I.small_q));
   I.response_bytes = (uint32_t)((2 * N * wbits + 7) / 8);
+  I.poly_degree = (uint32_t)N;
   return ONION_OK;
 }
 

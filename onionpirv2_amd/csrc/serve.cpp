@@ -1,6 +1,7 @@
 // serve.cpp -- serving loop of include/onionpir_serve.h: per-database request
-// queues, one worker thread per (database, lane) doing continuous batching on
-// its lane, admission by device-memory budget, latency statistics.
+// queues, one pool of worker threads for all databases doing continuous
+// batching on the lanes of the database each picks (oldest waiting query
+// first), admission by device-memory budget, latency statistics.
 //
 // The device work is entirely onion_gpu_answer_batch_lane (server.hip); this
 // file is host scheduling only.
@@ -47,10 +48,10 @@ struct Stats {
 
 struct Db {
   onion_server *srv = nullptr;
-  uint32_t cap = 1, lanes = 1;
+  uint32_t cap = 1;
   size_t ct_words = 0;  // 2N
   std::deque<std::shared_ptr<Request>> queue;
-  std::vector<std::thread> workers;
+  std::vector<uint8_t> lane_busy;
   Stats st;
 };
 
@@ -62,6 +63,7 @@ struct onion_serve {
   std::condition_variable cv_work, cv_done;
   std::vector<std::unique_ptr<Db>> dbs;
   std::unordered_map<uint64_t, std::shared_ptr<Request>> tickets;
+  std::vector<std::thread> workers;
   uint64_t next_ticket = 1;
   uint64_t budget_used = 0;
   bool stop = false;
@@ -69,31 +71,59 @@ struct onion_serve {
 
 namespace {
 
-// Worker of lane `lane` of database d: take every waiting query (up to the
-// cap) as one batch, answer it on the lane, publish the responses.
-void worker(onion_serve *sv, Db *D, uint32_t lane) {
-  const size_t ct = D->ct_words;
+// The database whose oldest waiting query is the oldest of all, among those
+// with a free lane (nullptr when none); *lane gets that free lane.  The
+// caller holds sv->m.
+Db *pick(onion_serve *sv, uint32_t *lane) {
+  Db *best = nullptr;
+  for (auto &D : sv->dbs) {
+    if (D->queue.empty()) continue;
+    auto it = std::find(D->lane_busy.begin(), D->lane_busy.end(), 0);
+    if (it == D->lane_busy.end()) continue;
+    if (!best || D->queue.front()->t_submit < best->queue.front()->t_submit) {
+      best = D.get();
+      *lane = (uint32_t)(it - D->lane_busy.begin());
+    }
+  }
+  return best;
+}
+
+// A worker of the pool: take the picked database's waiting queries (up to
+// its cap) as one batch, answer it on the free lane, publish the responses.
+void worker(onion_serve *sv) {
   std::vector<uint64_t> ids, qbuf, rbuf;
   std::vector<std::shared_ptr<Request>> batch;
   const auto linger = std::chrono::microseconds(sv->cfg.linger_us);
   for (;;) {
     batch.clear();
+    Db *D = nullptr;
+    uint32_t lane = 0;
     {
       std::unique_lock<std::mutex> g(sv->m);
-      sv->cv_work.wait(g, [&] { return sv->stop || !D->queue.empty(); });
-      if (sv->stop) return;
-      if (sv->cfg.linger_us && D->queue.size() < D->cap) {
-        const auto deadline = D->queue.front()->t_submit + linger;
-        sv->cv_work.wait_until(g, deadline, [&] { return sv->stop || D->queue.size() >= D->cap; });
+      for (;;) {
         if (sv->stop) return;
-        if (D->queue.empty()) continue;  // another lane took them
+        D = pick(sv, &lane);
+        if (!D) {
+          sv->cv_work.wait(g);
+          continue;
+        }
+        if (sv->cfg.linger_us && D->queue.size() < D->cap) {
+          const auto deadline = D->queue.front()->t_submit + linger;
+          if (Clock::now() < deadline) {
+            sv->cv_work.wait_until(g, deadline);
+            continue;  // pick again: more queries, another database, or the deadline
+          }
+        }
+        break;
       }
       const size_t n = std::min<size_t>(D->cap, D->queue.size());
       for (size_t i = 0; i < n; ++i) {
         batch.push_back(std::move(D->queue.front()));
         D->queue.pop_front();
       }
+      D->lane_busy[lane] = 1;
     }
+    const size_t ct = D->ct_words;
     const uint32_t n = (uint32_t)batch.size();
     ids.resize(n);
     qbuf.resize(n * ct);
@@ -116,6 +146,7 @@ void worker(onion_serve *sv, Db *D, uint32_t lane) {
     const auto now = Clock::now();
     {
       std::lock_guard<std::mutex> g(sv->m);
+      D->lane_busy[lane] = 0;
       D->st.batches += 1;
       D->st.batched += n;
       for (uint32_t i = 0; i < n; ++i) {
@@ -134,6 +165,7 @@ void worker(onion_serve *sv, Db *D, uint32_t lane) {
       }
     }
     sv->cv_done.notify_all();
+    sv->cv_work.notify_all();  // a lane is free again
   }
 }
 
@@ -168,8 +200,10 @@ int onion_serve_create(const onion_serve_config *cfg, onion_serve **out) {
   if (!cfg || !out) return fail(ONION_EINVAL, "null argument");
   if (cfg->max_batch < 1 || cfg->max_batch > 64) return fail(ONION_EINVAL, "max_batch must be in [1, 64]");
   if (cfg->lanes_per_db < 1 || cfg->lanes_per_db > 16) return fail(ONION_EINVAL, "lanes_per_db must be in [1, 16]");
+  if (cfg->workers > 64) return fail(ONION_EINVAL, "workers must be in [0, 64]");
   onion_serve *sv = new onion_serve();
   sv->cfg = *cfg;
+  for (uint32_t k = 0; k < cfg->workers; ++k) sv->workers.emplace_back(worker, sv);
   *out = sv;
   return ONION_OK;
 }
@@ -181,8 +215,17 @@ int onion_serve_destroy(onion_serve *sv) {
     sv->stop = true;
   }
   sv->cv_work.notify_all();
-  for (auto &D : sv->dbs)
-    for (auto &t : D->workers) t.join();
+  for (auto &t : sv->workers) t.join();
+  {
+    // queries still queued fail
+    std::lock_guard<std::mutex> g(sv->m);
+    for (auto &D : sv->dbs)
+      for (auto &r : D->queue) {
+        r->status = ONION_ESTATE;
+        r->err = "serving loop stopped";
+      }
+  }
+  sv->cv_done.notify_all();
   delete sv;
   return ONION_OK;
 }
@@ -191,9 +234,10 @@ int onion_serve_add_db(onion_serve *sv, onion_server *srv, uint32_t *db_id, uint
   if (!sv || !srv) return fail(ONION_EINVAL, "null argument");
   onion_param_info info;
   if (onion_gpu_param_info(srv, &info) != ONION_OK) return fail(ONION_EINVAL, onion_gpu_last_error());
-  uint64_t bpq = 0;
-  if (onion_gpu_scratch_bytes_per_query(srv, &bpq) != ONION_OK) return fail(ONION_EINVAL, onion_gpu_last_error());
   const uint32_t lanes = sv->cfg.lanes_per_db;
+  std::vector<uint64_t> bytes(sv->cfg.max_batch + 1, 0);  // lane scratch by cap
+  for (uint32_t c = 1; c <= sv->cfg.max_batch; ++c)
+    if (onion_gpu_scratch_bytes(srv, c, &bytes[c]) != ONION_OK) return fail(ONION_EINVAL, onion_gpu_last_error());
   uint32_t cap = sv->cfg.max_batch;
   uint64_t reserved = 0;
   {
@@ -204,13 +248,13 @@ int onion_serve_add_db(onion_serve *sv, onion_server *srv, uint32_t *db_id, uint
       if (D->srv == srv) return fail(ONION_EINVAL, "server already added");
     if (sv->cfg.scratch_budget) {
       const uint64_t left = sv->cfg.scratch_budget > sv->budget_used ? sv->cfg.scratch_budget - sv->budget_used : 0;
-      cap = (uint32_t)std::min<uint64_t>(cap, left / lanes / bpq);
+      while (cap > 0 && lanes * bytes[cap] > left) --cap;
       if (cap == 0)
         return fail(ONION_ENOMEM, "scratch budget: " + std::to_string(left) + " bytes left, " +
-                                      std::to_string(lanes) + " lanes x " + std::to_string(bpq) +
-                                      " bytes per query needed");
+                                      std::to_string(lanes) + " lanes x " + std::to_string(bytes[1]) +
+                                      " bytes for one query needed");
     }
-    reserved = (uint64_t)lanes * cap * bpq;
+    reserved = (uint64_t)lanes * bytes[cap];
     sv->budget_used += reserved;
   }
   int rc = onion_gpu_set_lanes(srv, lanes);
@@ -226,12 +270,12 @@ int onion_serve_add_db(onion_serve *sv, onion_server *srv, uint32_t *db_id, uint
   auto D = std::make_unique<Db>();
   D->srv = srv;
   D->cap = cap;
-  D->lanes = lanes;
-  D->ct_words = (size_t)2 * (info.galois_key_words / (4 * info.expan_height));
-  Db *Dp = D.get();
+  D->lane_busy.assign(lanes, 0);
+  D->ct_words = (size_t)2 * info.poly_degree;
   sv->dbs.push_back(std::move(D));
   const uint32_t id = (uint32_t)sv->dbs.size() - 1;
-  for (uint32_t k = 0; k < lanes; ++k) Dp->workers.emplace_back(worker, sv, Dp, k);
+  if (sv->cfg.workers == 0)  // one worker per lane of every database
+    for (uint32_t k = 0; k < lanes; ++k) sv->workers.emplace_back(worker, sv);
   if (db_id) *db_id = id;
   if (batch_cap) *batch_cap = cap;
   return ONION_OK;
@@ -277,6 +321,13 @@ int onion_serve_wait(onion_serve *sv, uint64_t ticket, uint64_t *resp, uint32_t 
   sv->tickets.erase(ticket);
   if (r->status != ONION_OK) return fail(r->status, r->err);
   memcpy(resp, r->resp.data(), r->resp.size() * 8);
+  return ONION_OK;
+}
+
+int onion_serve_discard(onion_serve *sv, uint64_t ticket) {
+  if (!sv) return fail(ONION_EINVAL, "null argument");
+  std::lock_guard<std::mutex> g(sv->m);
+  if (sv->tickets.erase(ticket) == 0) return fail(ONION_EINVAL, "unknown ticket " + std::to_string(ticket));
   return ONION_OK;
 }
 

@@ -281,12 +281,16 @@ int make_lane(std::unique_ptr<Lane> &out) {
   return ONION_OK;
 }
 
+// Device bytes ensure_scratch allocates for a lane of `cap` queries.
+uint64_t scratch_bytes(const onion_server *s, uint32_t cap) {
+  const uint64_t words = s->tree_words + s->rgsw_words + s->fd_words + s->digit_words;
+  return cap * (words * 8 + 2 * sizeof(void *)) + (uint64_t)kN * s->P.fst_dim * cpad_for(cap) * 2 * 4;
+}
+
 // Device bytes of lane scratch per query of a kMaxBatch-query lane (the
 // selection tile Qs is shared by the batch and counted pro rata).
 uint64_t scratch_bytes_per_query(const onion_server *s) {
-  const uint64_t words = s->tree_words + s->rgsw_words + s->fd_words + s->digit_words;
-  const uint64_t qs = (uint64_t)kN * s->P.fst_dim * cpad_for(kMaxBatch) * 2 * 4;
-  return words * 8 + 2 * sizeof(void *) + (qs + kMaxBatch - 1) / kMaxBatch;
+  return (scratch_bytes(s, kMaxBatch) + kMaxBatch - 1) / kMaxBatch;
 }
 
 // ---- the static scratch of onion_gpu_answer_batch_noalloc ---------------------------
@@ -1237,6 +1241,13 @@ int onion_gpu_scratch_bytes_per_query(const onion_server *s, uint64_t *bytes) {
   return ONION_OK;
 }
 
+int onion_gpu_scratch_bytes(const onion_server *s, uint32_t cap, uint64_t *bytes) {
+  if (!s || !bytes) return fail(ONION_EINVAL, "null argument");
+  if (cap < 1 || cap > kMaxBatch) return fail(ONION_EINVAL, "cap must be in [1, 64]");
+  *bytes = scratch_bytes(s, cap);
+  return ONION_OK;
+}
+
 int onion_gpu_reserve(onion_server *s, uint32_t lane, uint32_t n) {
   if (!s) return fail(ONION_EINVAL, "null server");
   Shared g(s->gate);
@@ -1330,8 +1341,10 @@ int onion_gpu_set_galois_key(onion_server *s, uint64_t id, const uint64_t *keys,
   ck.gk_levels = (uint32_t)(n / level_words);
   HIPCHK(hipMemcpyAsync(ck.gk, keys, n * 8, hipMemcpyHostToDevice, st));
   // stored times N^-1 per limb: the expansion's inverse transforms then need no scaling
-  k_key_scale<<<(uint32_t)((n + 255) / 256), 256, 0, st>>>(ck.gk, n, s->Tq.n_inv, s->Tp.n_inv, s->mq, s->mp);
-  HIPCHK(hipGetLastError());
+  if (n) {  // a database without expansion levels takes an empty key
+    k_key_scale<<<(uint32_t)((n + 255) / 256), 256, 0, st>>>(ck.gk, n, s->Tq.n_inv, s->Tp.n_inv, s->mq, s->mp);
+    HIPCHK(hipGetLastError());
+  }
   HIPCHK(hipStreamSynchronize(st));
   return ONION_OK;
 }

@@ -1,8 +1,9 @@
 """Serving loop over device-resident databases (binding of include/onionpir_serve.h).
 
-``GpuServe`` queues queries per database; per database, ``lanes_per_db``
-worker threads in libonionpir.so each take every waiting query (up to the
-database's admitted batch cap) as one batch on their own lane of the server.
+``GpuServe`` queues queries per database; a pool of worker threads in
+libonionpir.so serves all databases: a free worker picks the database whose
+oldest waiting query is the oldest of all and takes its waiting queries (up to
+the database's admitted batch cap) as one batch on a free lane of that server.
 Responses are bit-identical to ``GpuServer.answer`` on the same inputs.
 """
 from __future__ import annotations
@@ -20,7 +21,7 @@ FOREVER = 0xFFFFFFFF
 
 class ServeConfig(C.Structure):
     _fields_ = [("max_batch", C.c_uint32), ("lanes_per_db", C.c_uint32), ("scratch_budget", C.c_uint64),
-                ("linger_us", C.c_uint32)]
+                ("linger_us", C.c_uint32), ("workers", C.c_uint32)]
 
 
 class ServeStats(C.Structure):
@@ -37,6 +38,7 @@ _SIGS = {
     "onion_serve_add_db": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "onion_serve_submit": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, U64P, C.POINTER(C.c_uint64)]),
     "onion_serve_wait": (C.c_int, [C.c_void_p, C.c_uint64, U64P, C.c_uint32]),
+    "onion_serve_discard": (C.c_int, [C.c_void_p, C.c_uint64]),
     "onion_serve_stats_get": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(ServeStats)]),
     "onion_serve_reset_stats": (C.c_int, [C.c_void_p]),
 }
@@ -74,10 +76,13 @@ def check(rc: int) -> None:
 
 
 class GpuServe:
-    def __init__(self, max_batch: int = 64, lanes_per_db: int = 2, scratch_budget: int = 0, linger_us: int = 0):
+    def __init__(self, max_batch: int = 64, lanes_per_db: int = 2, scratch_budget: int = 0, linger_us: int = 0,
+                 workers: int = 0):
+        """workers: batches in flight over all databases (0 = one per lane of
+        every database added)."""
         L = _lib()
         self._L = L
-        cfg = ServeConfig(max_batch, lanes_per_db, scratch_budget, linger_us)
+        cfg = ServeConfig(max_batch, lanes_per_db, scratch_budget, linger_us, workers)
         h = C.c_void_p()
         check(L.onion_serve_create(C.byref(cfg), C.byref(h)))
         self._h = h
@@ -121,6 +126,10 @@ class GpuServe:
         out = np.zeros(self._ct_words[db_id], np.uint64)
         check(self._L.onion_serve_wait(self._h, ticket, u64ptr(out), timeout_ms))
         return out
+
+    def discard(self, ticket: int) -> None:
+        """Retire a ticket without its response."""
+        check(self._L.onion_serve_discard(self._h, ticket))
 
     def stats(self, db_id: int | None = None) -> dict:
         s = ServeStats()

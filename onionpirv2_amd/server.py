@@ -136,6 +136,12 @@ class GpuServer:
         check(self._L.onion_gpu_scratch_bytes_per_query(self._h, C.byref(b)))
         return b.value
 
+    def scratch_bytes(self, cap: int) -> int:
+        """Device bytes of one lane's scratch for batches of up to cap queries."""
+        b = C.c_uint64()
+        check(self._L.onion_gpu_scratch_bytes(self._h, cap, C.byref(b)))
+        return b.value
+
     def pack_response(self, resp) -> bytes:
         r = _u64(resp).reshape(-1)
         cap = self.info["response_bytes"]

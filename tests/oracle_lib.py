@@ -13,7 +13,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-LIB_PATH = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+# the portable build; ONION_ORACLE_LIB may name another build of the same
+# sources (the benchmark's -march=native one)
+LIB_PATH = os.environ.get("ONION_ORACLE_LIB") or os.path.join(ORACLE_DIR, "_build", "liboracle.so")
 
 U64P = C.POINTER(C.c_uint64)
 
@@ -55,9 +57,10 @@ def build() -> None:
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        path = os.environ.get("ONION_ORACLE_LIB") or LIB_PATH
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(path)
         P = C.POINTER(OrcCtx)
         L.orc_create.restype = P
         L.orc_create.argtypes = [C.c_uint32] * 6

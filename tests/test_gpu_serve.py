@@ -201,7 +201,11 @@ def test_serve_budget_admission(dbs):
     s0, s1 = dbs["servers"]
     b0, b1 = s0.scratch_bytes_per_query(), s1.scratch_bytes_per_query()
     assert b1 > b0 > 0
-    budget = 2 * 3 * b0 + b0  # two lanes of three queries on db 0, then too little for db 1
+    assert s0.scratch_bytes(64) == 64 * b0
+    assert s0.scratch_bytes(1) < s0.scratch_bytes(3) < s0.scratch_bytes(4)
+    # two lanes of three queries on db 0, then less than one query's lanes of db 1
+    budget = 2 * s0.scratch_bytes(3) + s0.scratch_bytes(1)
+    assert 2 * s0.scratch_bytes(4) > budget and 2 * s1.scratch_bytes(1) > s0.scratch_bytes(1)
     with GpuServe(max_batch=64, lanes_per_db=2, scratch_budget=budget) as sv:
         db, cap = sv.add_db(s0)
         assert cap == 3
@@ -214,6 +218,60 @@ def test_serve_budget_admission(dbs):
         assert sv.stats(db)["mean_batch"] <= 3
     for i in range(7):
         assert np.array_equal(got[i], s0.answer(int(c[i]), q[i]))
+
+
+def test_serve_one_by_one_database_and_discard():
+    """A 1 x 1 database (no expansion levels, no further dimensions) in the
+    serving loop; a discarded ticket is gone, the others still answer."""
+    from onionpirv2_amd import GpuServer, OnionValueError
+    from onionpirv2_amd.client import GpuClient
+    from onionpirv2_amd.serve import GpuServe
+    with GpuServer(1, 1, plain_bits=PLAIN_BITS) as srv:
+        assert srv.info["expan_height"] == 0 and srv.info["poly_degree"] == N
+        srv.db_generate(5, 0)
+        c = GpuClient(1, 1, client_id=4, seed=44, plain_bits=PLAIN_BITS)
+        srv.set_galois_key(4, c.galois_keys())
+        srv.set_rgsw_key(4, c.rgsw_key())
+        qs = [c.query(0, nonce=k) for k in range(3)]
+        with GpuServe(max_batch=4, lanes_per_db=1, workers=1) as sv:
+            db, cap = sv.add_db(srv)
+            tickets = [sv.submit(db, 4, q) for q in qs]
+            sv.discard(tickets[1])
+            with pytest.raises(OnionValueError):
+                sv.discard(tickets[1])
+            for k in (0, 2):
+                r = sv.wait(tickets[k])
+                assert np.array_equal(r, srv.answer(4, qs[k]))
+                assert np.array_equal(c.decrypt(r), record_of(5, 0, srv))
+            with pytest.raises(OnionValueError):
+                sv.wait(tickets[1], timeout_ms=0)
+
+
+def record_of(seed, idx, srv):
+    from onionpirv2_amd import dbgen
+    return dbgen.plaintext(seed, 0, idx, t=srv.info["t"])
+
+
+def test_serve_one_worker_serves_oldest_first(dbs):
+    """One worker for two databases: the query that arrived first is answered
+    in the first batch although the other database's queue fills behind it;
+    every response equals the query answered alone."""
+    from onionpirv2_amd.serve import GpuServe
+    s0, s1 = dbs["servers"]
+    c0, i0, q0 = make_queries(dbs, 0, 1, seed=960)
+    c1, i1, q1 = make_queries(dbs, 1, 16, seed=961)
+    with GpuServe(max_batch=2, lanes_per_db=1, workers=1) as sv:
+        d0, _ = sv.add_db(s0)
+        d1, _ = sv.add_db(s1)
+        first = sv.submit(d0, int(c0[0]), q0[0])
+        rest = [sv.submit(d1, int(c1[k]), q1[k]) for k in range(16)]
+        r0 = sv.wait(first)
+        assert sv.stats(d1)["completed"] <= 2  # at most the batch that ran beside nothing else
+        got = [sv.wait(t, db_id=d1) for t in rest]
+        assert sv.stats(d1)["batches"] >= 8
+    assert np.array_equal(r0, s0.answer(int(c0[0]), q0[0]))
+    for k in range(16):
+        assert np.array_equal(got[k], s1.answer(int(c1[k]), q1[k]))
 
 
 def test_serve_poll_and_timeout(dbs):
