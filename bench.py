@@ -149,7 +149,10 @@ def main() -> int:
     ids = np.arange(B, dtype=np.uint64)
     log(f"[bench] {B} clients keyed, queries built in {time.time() - t0:.2f} s")
 
-    tuning = srv.autotune([1, B], reps=3) if args.autotune else srv.get_tuning()
+    # 100 = config 4's largest burst: more than one pass, so the batch-split
+    # choice (64 + 36 or 50 + 50) is timed too
+    tune_sizes = [1, B] + ([100] if B == 64 else [])
+    tuning = srv.autotune(tune_sizes, reps=3) if args.autotune else srv.get_tuning()
     log(f"[bench] tuning {tuning}")
 
     # ---- single-query path ---------------------------------------------------------
@@ -271,6 +274,7 @@ def main() -> int:
                                   "algorithmic_bytes": alg_bytes},
         "cpu_baseline": None,
         "tuning": tuning,
+        "autotune_sizes": tune_sizes if args.autotune else None,
     }
 
     out["single_query_graph"] = graph_single

@@ -121,6 +121,18 @@ typedef struct onion_tuning {
                              on every database write, for fd_mfma_variant 3;
                              0: free it (the default).  Only for first
                              dimensions that are a multiple of 32.          */
+  uint32_t ntt_radix;     /* pass schedule of the transforms in the fused
+                             multiplexer levels (and onion_gpu_tap_ntt):
+                             8 = radix-8 register passes (3 + 3 + 3 + 2
+                             stages, three LDS exchanges per transform);
+                             4 = radix-4 passes (2 + 2 + 2 + 2 + 3 stages,
+                             four exchanges).  Bit-identical results.       */
+  uint32_t batch_split;   /* a call with more queries than one pass takes
+                             (64) runs them in chunks: 0 = chunks of 64 and
+                             the rest (100 = 64 + 36); 1 = equal chunks
+                             (100 = 50 + 50).  The serving loop splits a
+                             backlog above a database's batch cap the same
+                             way.                                           */
 } onion_tuning;
 
 typedef struct onion_server onion_server;
@@ -196,11 +208,13 @@ int onion_gpu_last_stage_times(const onion_server *srv, onion_stage_times *out);
 int onion_gpu_get_tuning(const onion_server *srv, onion_tuning *out);
 int onion_gpu_set_tuning(onion_server *srv, const onion_tuning *tuning);
 /* Pick the tuning on the live database (no reference counterpart): times the
- * whole pipeline for each of the n_sizes batch sizes (1..64) with a synthetic
- * client, minimum of `reps` runs, and walks each choice in turn keeping what
- * lowers the sum over the sizes of each size's device time relative to the
- * starting tuning's (so every size weighs alike).  Applies the result and
- * writes it to *chosen (may be NULL).  Waits for batches in flight. */
+ * whole pipeline for each of the n_sizes batch sizes (1..256; above 64 in
+ * chunks as tune.batch_split cuts them) with a synthetic client, host clock
+ * from the first launch to completion, median of `reps` runs, and walks each
+ * choice in turn keeping what lowers the sum over the sizes of each size's
+ * time relative to the starting tuning's (so every size weighs alike).
+ * Applies the result and writes it to *chosen (may be NULL).  Waits for
+ * batches in flight. */
 int onion_gpu_autotune(onion_server *srv, const uint32_t *batch_sizes, uint32_t n_sizes, uint32_t reps,
                        onion_tuning *chosen);
 

@@ -264,7 +264,7 @@ __global__ __launch_bounds__(256) void k_rgsw_fold_last(uint64_t *G, uint64_t g_
 // run on one XCD one after another (grid half * nq / Q, block index mod 8 =
 // XCD), so the pair's two 2 MiB rows come from HBM once and the 16-byte loads
 // at the 1 KiB slot stride hit that L2.
-template <bool FIRST, int Q = 1>
+template <bool FIRST, int Q = 1, int R = 8>
 __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, uint64_t cts_stride, uint64_t half,
                                                                const uint64_t *out2, uint32_t ncols, uint32_t nq,
                                                                const uint64_t *G, uint64_t g_stride, uint32_t l,
@@ -371,7 +371,7 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
       {
         uint64_t v[8];
         y_minus_x(poly, v);
-        ntt_inv(v, lds, T);
+        ntt_inv_sched<R>(v, lds, T);
 #pragma unroll
         for (int j = 0; j < 8; ++j) coef[t + 256 * j] = ntt_inv_final(v[j], T);
       }
@@ -381,7 +381,7 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
         uint64_t d[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) d[j] = (coef[t + 256 * j] >> sh) & mask;
-        ntt_fwd<true>(d, lds, T);  // digits < 2^base_log2
+        ntt_fwd_sched<R, true>(d, lds, T);  // digits < 2^base_log2
 #pragma unroll
         for (int j = 0; j < 8; ++j) d[j] = ntt_fwd_below_8q(d[j], q);
         mac_row(lo, hi, d, Gb + (size_t)(poly * l + kk) * 2 * kN);

@@ -225,6 +225,180 @@ __device__ __forceinline__ void ntt_inv(uint64_t (&x)[8], uint64_t *lds, const N
   inv8(x, tw, q, true);
 }
 
+// ---- radix-4 schedule: 2 + 2 + 2 + 2 + 3 stages -------------------------------
+// The same butterflies, stage order, twiddles and lazy-reduction points as
+// ntt_fwd / ntt_inv (the results are bit-identical), in five register passes
+// and four LDS exchanges instead of four and three.  In a radix-4 pass with
+// element stride 2^s0 (top span H = 2^(s0+1)) a thread owns two groups,
+// g = 0, 1 with group index t + 256 g, each the four coefficients
+// b, b + H/2, b + H, b + 3H/2 (x[4g + k] = b + k 2^s0): stage H pairs
+// (k, k + 2) under twiddle m = N/(2H) + b/(2H), stage H/2 pairs (0, 1) under
+// 2m and (2, 3) under 2m + 1.
+
+// base coefficient of group gi in the radix-4 pass whose element stride is 2^s0
+__device__ __forceinline__ int r4_base(int gi, int s0) {
+  return (gi & ((1 << s0) - 1)) | ((gi >> s0) << (s0 + 2));
+}
+
+struct Tw4 {
+  uint64_t w[6], s[6];  // group g: w[3g] span H, w[3g + 1 .. 3g + 2] span H/2
+};
+
+__device__ __forceinline__ void load_tw4(Tw4 &tw, const uint64_t *tab, const uint64_t *tabs, int t, int s0) {
+  __builtin_amdgcn_sched_barrier(0);  // as in load_tw8
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    const int m = (1 << (9 - s0)) + (r4_base(t + 256 * g, s0) >> (s0 + 2));
+    tw.w[3 * g] = tab[m];
+    tw.s[3 * g] = tabs[m];
+    tw.w[3 * g + 1] = tab[2 * m];
+    tw.s[3 * g + 1] = tabs[2 * m];
+    tw.w[3 * g + 2] = tab[2 * m + 1];
+    tw.s[3 * g + 2] = tabs[2 * m + 1];
+  }
+}
+
+// forward radix-4 on both groups; bit s of CMASK reduces the inputs of stage
+// s (0 = span H, 1 = H/2); SMALL: inputs below 2^32 (span-H stage)
+template <int CMASK, bool SMALL = false>
+__device__ __forceinline__ void fwd4(uint64_t (&x)[8], const Tw4 &tw, uint64_t q) {
+  const uint64_t two_q = 2 * q;
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (SMALL)
+        ct_bfly_small(x[4 * g + k], x[4 * g + k + 2], tw.w[3 * g], tw.s[3 * g], q, two_q);
+      else
+        ct_bfly<(CMASK & 1) != 0>(x[4 * g + k], x[4 * g + k + 2], tw.w[3 * g], tw.s[3 * g], q, two_q);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+      ct_bfly<(CMASK & 2) != 0>(x[4 * g + 2 * k], x[4 * g + 2 * k + 1], tw.w[3 * g + 1 + k], tw.s[3 * g + 1 + k],
+                                q, two_q);
+  }
+}
+
+// inverse radix-4 on both groups (span H/2, then H)
+__device__ __forceinline__ void inv4(uint64_t (&x)[8], const Tw4 &tw, uint64_t q) {
+  const uint64_t two_q = 2 * q;
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+      gs_bfly(x[4 * g + 2 * k], x[4 * g + 2 * k + 1], tw.w[3 * g + 1 + k], tw.s[3 * g + 1 + k], q, two_q);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) gs_bfly(x[4 * g + k], x[4 * g + k + 2], tw.w[3 * g], tw.s[3 * g], q, two_q);
+  }
+}
+
+__device__ __forceinline__ void lds_put4(uint64_t *lds, const uint64_t (&x)[8], int t, int s0) {
+#pragma unroll
+  for (int g = 0; g < 2; ++g)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) lds[lds_pad(r4_base(t + 256 * g, s0) + (k << s0))] = x[4 * g + k];
+}
+__device__ __forceinline__ void lds_get4(const uint64_t *lds, uint64_t (&x)[8], int t, int s0) {
+#pragma unroll
+  for (int g = 0; g < 2; ++g)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) x[4 * g + k] = lds[lds_pad(r4_base(t + 256 * g, s0) + (k << s0))];
+}
+
+// ntt_fwd in the radix-4 schedule (same entry and exit layouts and bounds).
+// The first pass's groups are coefficients t + 256 (g + 2k), a register
+// permutation of the natural layout.
+template <bool SMALL = false>
+__device__ __forceinline__ void ntt_fwd_r4(uint64_t (&x)[8], uint64_t *lds, const NttTab &T) {
+  const int t = threadIdx.x;
+  const uint64_t q = T.q;
+  uint64_t y[8];
+#pragma unroll
+  for (int g = 0; g < 2; ++g)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) y[4 * g + k] = x[g + 2 * k];
+  Tw4 tw;
+  load_tw4(tw, T.rp, T.rps, t, 9);  // spans 1024, 512 (stages 0, 1)
+  fwd4<0, SMALL>(y, tw, q);
+  load_tw4(tw, T.rp, T.rps, t, 7);  // spans 256, 128 (stages 2, 3: reduce before 3)
+  __syncthreads();
+  lds_put4(lds, y, t, 9);
+  __syncthreads();
+  lds_get4(lds, y, t, 7);
+  fwd4<2>(y, tw, q);
+  load_tw4(tw, T.rp, T.rps, t, 5);  // spans 64, 32 (stages 4, 5)
+  __syncthreads();
+  lds_put4(lds, y, t, 7);
+  __syncthreads();
+  lds_get4(lds, y, t, 5);
+  fwd4<0>(y, tw, q);
+  load_tw4(tw, T.rp, T.rps, t, 3);  // spans 16, 8 (stages 6, 7: reduce before 7)
+  __syncthreads();
+  lds_put4(lds, y, t, 5);
+  __syncthreads();
+  lds_get4(lds, y, t, 3);
+  fwd4<2>(y, tw, q);
+  Tw8 tw8;  // spans 4, 2, 1 on slots 8t .. 8t+7 (stages 8, 9, 10)
+  load_tw8(tw8, T.rp, T.rps, 256 + t);
+  __syncthreads();
+  lds_put4(lds, y, t, 3);
+  __syncthreads();
+  lds_get(lds, x, 8 * t, 1);
+  fwd8<0>(x, tw8, q, true);
+}
+
+// ntt_inv in the radix-4 schedule (same entry and exit layouts and bounds).
+__device__ __forceinline__ void ntt_inv_r4(uint64_t (&x)[8], uint64_t *lds, const NttTab &T) {
+  const int t = threadIdx.x;
+  const uint64_t q = T.q;
+  {
+    Tw8 tw8;  // spans 1, 2, 4 on slots 8t .. 8t+7
+    load_tw8(tw8, T.irp, T.irps, 256 + t);
+    inv8(x, tw8, q, true);
+  }
+  Tw4 tw;
+  load_tw4(tw, T.irp, T.irps, t, 3);  // spans 8, 16
+  __syncthreads();
+  lds_put(lds, x, 8 * t, 1);
+  __syncthreads();
+  lds_get4(lds, x, t, 3);
+  inv4(x, tw, q);
+#pragma unroll
+  for (int s0 = 5; s0 <= 9; s0 += 2) {  // spans 32, 64 | 128, 256 | 512, 1024
+    load_tw4(tw, T.irp, T.irps, t, s0);
+    __syncthreads();
+    lds_put4(lds, x, t, s0 - 2);
+    __syncthreads();
+    lds_get4(lds, x, t, s0);
+    inv4(x, tw, q);
+  }
+  uint64_t y[8];
+#pragma unroll
+  for (int g = 0; g < 2; ++g)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) y[g + 2 * k] = x[4 * g + k];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) x[j] = y[j];
+}
+
+// Transforms in the schedule R (8: radix-8 passes, 4: radix-4 passes).
+template <int R, bool SMALL = false>
+__device__ __forceinline__ void ntt_fwd_sched(uint64_t (&x)[8], uint64_t *lds, const NttTab &T) {
+  static_assert(R == 8 || R == 4, "radix 8 or 4");
+  if constexpr (R == 4)
+    ntt_fwd_r4<SMALL>(x, lds, T);
+  else
+    ntt_fwd<SMALL>(x, lds, T);
+}
+template <int R>
+__device__ __forceinline__ void ntt_inv_sched(uint64_t (&x)[8], uint64_t *lds, const NttTab &T) {
+  static_assert(R == 8 || R == 4, "radix 8 or 4");
+  if constexpr (R == 4)
+    ntt_inv_r4(x, lds, T);
+  else
+    ntt_inv(x, lds, T);
+}
+
 // Canonical value of a forward-NTT output in [0, 16q).
 __device__ __forceinline__ uint64_t ntt_fwd_final(uint64_t v, uint64_t q) {
   if (v >= 8 * q) v -= 8 * q;

@@ -116,7 +116,14 @@ void worker(onion_serve *sv) {
         }
         break;
       }
-      const size_t n = std::min<size_t>(D->cap, D->queue.size());
+      // a backlog above the cap: the cap, or with batch_split = 1 an equal
+      // share of it (100 waiting at cap 64: 50 now, 50 next)
+      size_t n = std::min<size_t>(D->cap, D->queue.size());
+      onion_tuning tn;
+      if (D->queue.size() > D->cap && onion_gpu_get_tuning(D->srv, &tn) == ONION_OK && tn.batch_split == 1) {
+        const size_t k = (D->queue.size() + D->cap - 1) / D->cap;
+        n = (D->queue.size() + k - 1) / k;
+      }
       for (size_t i = 0; i < n; ++i) {
         batch.push_back(std::move(D->queue.front()));
         D->queue.pop_front();

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <cstring>
@@ -183,7 +184,7 @@ struct onion_server {
   // fuse_min 512 runs a single query's first multiplexer level (512 pairs)
   // fused on the first dimension's NTT-form outputs, 1.4% faster than split
   // (batches of 64 take the same time either way)
-  onion_tuning tune{512, 256, 4, 3, 0, 2, 2, 1, 0};
+  onion_tuning tune{512, 256, 4, 3, 0, 2, 2, 1, 0, 8, 0};
   uint64_t tune_epoch = 1;
   Gate gate;
   mutable std::mutex last_mu;
@@ -762,7 +763,7 @@ void run_first_dim(const onion_server *s, Lane *L, uint32_t nq, uint64_t *sel, u
     launch_fd_bat<128>(s, L, nq);
   }
   if (time_core) (void)hipEventRecord(L->ev[7], L->st);
-  if (!keep_ntt) k_ntt_inv<<<other * 2 * nq, kNttThreads, 0, L->st>>>(L->fd, L->fd, s->Tq);
+  if (!keep_ntt) k_ntt_inv<8><<<other * 2 * nq, kNttThreads, 0, L->st>>>(L->fd, L->fd, s->Tq);
 }
 
 // n NTT-form cts per query in fd back to coefficient form.
@@ -795,8 +796,12 @@ void run_mux_level(const onion_server *s, Lane *L, uint32_t nq, uint64_t n, uint
   if (half * nq >= s->tune.fuse_min) {
     if (in_ntt) {
       fold_last(s, L, nq, g);
-      k_mux_fused_ntt<false><<<dim3((uint32_t)half, nq), kNttThreads, 0, L->st>>>(
-          L->fd, s->fd_words, half, nullptr, 0, nq, g, s->rgsw_words, l, s->I.base_log2, s->Tq, s->mq);
+      if (s->tune.ntt_radix == 4)
+        k_mux_fused_ntt<false, 1, 4><<<dim3((uint32_t)half, nq), kNttThreads, 0, L->st>>>(
+            L->fd, s->fd_words, half, nullptr, 0, nq, g, s->rgsw_words, l, s->I.base_log2, s->Tq, s->mq);
+      else
+        k_mux_fused_ntt<false><<<dim3((uint32_t)half, nq), kNttThreads, 0, L->st>>>(
+            L->fd, s->fd_words, half, nullptr, 0, nq, g, s->rgsw_words, l, s->I.base_log2, s->Tq, s->mq);
     }
     else
       k_mux_fused<<<dim3((uint32_t)half, nq), kNttThreads, 0, L->st>>>(L->fd, s->fd_words, half, g, s->rgsw_words,
@@ -836,8 +841,17 @@ bool first_dim_keeps_ntt(const onion_server *s, const Lane *L, uint32_t nq) {
 void run_mux_first(const onion_server *s, Lane *L, uint32_t nq) {
   const uint32_t half = s->P.other_dim / 2;
   fold_last(s, L, nq, L->rgsw);
-  if (nq % 2 == 0)
+  const bool r4 = s->tune.ntt_radix == 4;
+  if (nq % 2 == 0 && r4)
+    k_mux_fused_ntt<true, 2, 4><<<half * nq / 2, kNttThreads, 0, L->st>>>(
+        L->fd, s->fd_words, half, L->digits, 2 * nq, nq, L->rgsw, s->rgsw_words, s->P.gsw_l, s->I.base_log2, s->Tq,
+        s->mq);
+  else if (nq % 2 == 0)
     k_mux_fused_ntt<true, 2><<<half * nq / 2, kNttThreads, 0, L->st>>>(
+        L->fd, s->fd_words, half, L->digits, 2 * nq, nq, L->rgsw, s->rgsw_words, s->P.gsw_l, s->I.base_log2, s->Tq,
+        s->mq);
+  else if (r4)
+    k_mux_fused_ntt<true, 1, 4><<<half * nq, kNttThreads, 0, L->st>>>(
         L->fd, s->fd_words, half, L->digits, 2 * nq, nq, L->rgsw, s->rgsw_words, s->P.gsw_l, s->I.base_log2, s->Tq,
         s->mq);
   else
@@ -975,8 +989,19 @@ int answer_chunk(onion_server *s, Lane *L, uint32_t n, const uint64_t *ids, cons
   return collect_times(s, L, !graph, times);
 }
 
-// n queries on lane L in chunks of kMaxBatch; the caller holds the gate shared
-// and L->mu.
+// Size of the next chunk of n waiting queries, at most cap per pass: the cap
+// itself (then the rest), or with batch_split = 1 equal chunks.
+uint32_t next_chunk(uint32_t split, uint32_t n, uint32_t cap) {
+  if (n <= cap) return n;
+  if (split == 1) {
+    const uint32_t k = (n + cap - 1) / cap;
+    return (n + k - 1) / k;
+  }
+  return cap;
+}
+
+// n queries on lane L in chunks of at most kMaxBatch (tune.batch_split); the
+// caller holds the gate shared and L->mu.
 int answer_on_lane(onion_server *s, Lane *L, uint32_t n, const uint64_t *ids, const uint64_t *queries,
                    uint64_t *resps, onion_stage_times *times) {
   int rc;
@@ -984,8 +1009,8 @@ int answer_on_lane(onion_server *s, Lane *L, uint32_t n, const uint64_t *ids, co
   if (n == 0) return ONION_OK;
   if ((rc = ensure_scratch(s, L, std::min(n, kMaxBatch)))) return rc;
   onion_stage_times sum{};
-  for (uint32_t off = 0; off < n; off += kMaxBatch) {
-    const uint32_t m = std::min(kMaxBatch, n - off);
+  for (uint32_t off = 0, m = 0; off < n; off += m) {
+    m = next_chunk(s->tune.batch_split, n - off, kMaxBatch);
     onion_stage_times t{};
     if ((rc = answer_chunk(s, L, m, ids + off, queries + (size_t)off * 2 * kN, resps + (size_t)off * 2 * kN, &t)))
       return rc;
@@ -1006,8 +1031,9 @@ int answer_on_lane(onion_server *s, Lane *L, uint32_t n, const uint64_t *ids, co
 // Device time of one pass of the pipeline for n queries already bound on lane
 // L (minimum over reps, after one untimed pass that also captures the graph).
 int time_pipeline(onion_server *s, Lane *L, uint32_t n, uint32_t reps, double *ms) {
-  double best = 1e30;
+  std::vector<double> runs;
   for (uint32_t r = 0; r <= reps; ++r) {
+    const auto t0 = std::chrono::steady_clock::now();
     if (s->tune.use_graph) {
       hipGraphExec_t ex;
       int rc = graph_for(s, L, n, &ex);
@@ -1022,29 +1048,45 @@ int time_pipeline(onion_server *s, Lane *L, uint32_t n, uint32_t reps, double *m
       HIPCHK(hipEventRecord(L->ev[5], L->st));
     }
     HIPCHK(hipEventSynchronize(L->ev[5]));
-    float t = 0;
-    HIPCHK(hipEventElapsedTime(&t, L->ev[0], L->ev[5]));
-    if (r > 0) best = std::min(best, (double)t);
+    const std::chrono::duration<double, std::milli> t = std::chrono::steady_clock::now() - t0;
+    if (r > 0) runs.push_back(t.count());
   }
-  *ms = best;
+  std::sort(runs.begin(), runs.end());
+  *ms = runs[runs.size() / 2];
+  return ONION_OK;
+}
+
+// A call of n queries (any n) as answer_on_lane runs it: chunk by chunk.
+int time_call(onion_server *s, Lane *L, uint32_t n, uint32_t reps, double *ms) {
+  *ms = 0;
+  for (uint32_t off = 0, m = 0; off < n; off += m) {
+    m = next_chunk(s->tune.batch_split, n - off, kMaxBatch);
+    double t;
+    int rc = time_pipeline(s, L, m, reps, &t);
+    if (rc) return rc;
+    *ms += t;
+  }
   return ONION_OK;
 }
 
 // Coordinate descent over the launch-shape choices on the live database with
 // a synthetic client (uniform key words and queries: the work does not depend
-// on the values).  Score = sum over the batch sizes of the device time
+// on the values).  Score = sum over the batch sizes of the call's time
+// (host clock from the first launch to completion, median of the runs: the
+// launch cost counts, as it does for a caller waiting on the call)
 // relative to the starting choice's, so every size counts alike (in plain ms
 // per query a single query would outweigh a batch of 64 and decide choices
 // that only move the batch).  The caller holds the gate exclusively.
 int autotune_impl(onion_server *s, const uint32_t *sizes, uint32_t n_sizes, uint32_t reps, onion_tuning *out) {
   Lane *L = s->lanes[0].get();
   uint32_t maxn = 1;
-  bool has1 = false, hasb = false, has_small = false;
+  bool has1 = false, hasb = false, has_small = false, has_big = false;
   for (uint32_t i = 0; i < n_sizes; ++i) {
-    maxn = std::max(maxn, sizes[i]);
+    maxn = std::max(maxn, std::min(sizes[i], kMaxBatch));
     has1 |= sizes[i] == 1;
     hasb |= sizes[i] >= 2;
     has_small |= sizes[i] >= 2 && sizes[i] < 16;
+    has_big |= sizes[i] > kMaxBatch && sizes[i] % kMaxBatch != 0;
   }
   int rc;
   if ((rc = ensure_scratch(s, L, maxn))) return rc;
@@ -1080,7 +1122,7 @@ int autotune_impl(onion_server *s, const uint32_t *sizes, uint32_t n_sizes, uint
     double total = 0;
     for (uint32_t i = 0; i < n_sizes; ++i) {
       double ms;
-      int r = time_pipeline(s, L, sizes[i], reps, &ms);
+      int r = time_call(s, L, sizes[i], reps, &ms);
       if (r) return r;
       if (base[i] == 0) base[i] = std::max(ms, 1e-6);
       total += ms / base[i];
@@ -1110,7 +1152,9 @@ int autotune_impl(onion_server *s, const uint32_t *sizes, uint32_t n_sizes, uint
       {&onion_tuning::fd_b1_variant, {0, 1, 2, 3}, has1},
       {&onion_tuning::fd_rows_b1, {64, 128, 256, 512}, has1},
       {&onion_tuning::fuse_min, {1, 256, 512, 1024, 4096, 1u << 30}, true},
+      {&onion_tuning::ntt_radix, {8, 4}, true},
       {&onion_tuning::use_graph, {0, 1}, true},
+      {&onion_tuning::batch_split, {0, 1}, has_big},
   };
   for (const Axis &ax : axes) {
     if (!ax.active) continue;
@@ -1465,7 +1509,7 @@ int onion_gpu_set_rgsw_key(onion_server *s, uint64_t id, const uint64_t *rows, s
   }
   HIPCHK(hipMemcpyAsync(ck.rk, rows, n * 8, hipMemcpyHostToDevice, st));
   // GSWEval::gsw_ntt_negacyclic_harvey: every half-row to NTT form
-  k_ntt_fwd<<<(uint32_t)(n / kN), kNttThreads, 0, st>>>(ck.rk, ck.rk, s->Tq);
+  k_ntt_fwd<8><<<(uint32_t)(n / kN), kNttThreads, 0, st>>>(ck.rk, ck.rk, s->Tq);
   if (s->conv_fold)  // stored folded: the conversion multiplies the last row by the leaf itself
     k_rgsw_fold_last<<<dim3(4 * kN / 256, 1), 256, 0, st>>>(ck.rk, 0, s->P.gsw_l_key, s->I.base_log2_key, s->mq);
   HIPCHK(hipGetLastError());
@@ -1597,6 +1641,8 @@ int onion_gpu_set_tuning(onion_server *s, const onion_tuning *t) {
   if (t->fd_mfma_variant > 3) return fail(ONION_EINVAL, "fd_mfma_variant must be 0..3");
   if (t->db_planes > 1) return fail(ONION_EINVAL, "db_planes must be 0 or 1");
   if (t->db_residues > 1) return fail(ONION_EINVAL, "db_residues must be 0 or 1");
+  if (t->ntt_radix != 8 && t->ntt_radix != 4) return fail(ONION_EINVAL, "ntt_radix must be 8 or 4");
+  if (t->batch_split > 1) return fail(ONION_EINVAL, "batch_split must be 0 or 1");
   Exclusive x(s->gate);
   const uint32_t had = s->tune.db_planes, had_r = s->tune.db_residues;
   s->tune = *t;
@@ -1620,7 +1666,7 @@ int onion_gpu_autotune(onion_server *s, const uint32_t *sizes, uint32_t n_sizes,
   if (!s || !sizes) return fail(ONION_EINVAL, "null argument");
   if (n_sizes < 1 || n_sizes > 16) return fail(ONION_EINVAL, "n_sizes must be in [1, 16]");
   for (uint32_t i = 0; i < n_sizes; ++i)
-    if (sizes[i] < 1 || sizes[i] > kMaxBatch) return fail(ONION_EINVAL, "batch sizes must be in [1, 64]");
+    if (sizes[i] < 1 || sizes[i] > 4 * kMaxBatch) return fail(ONION_EINVAL, "batch sizes must be in [1, 256]");
   if (reps < 1 || reps > 20) return fail(ONION_EINVAL, "reps must be in [1, 20]");
   Exclusive x(s->gate);
   int rc;
@@ -1733,10 +1779,15 @@ int onion_gpu_tap_ntt(onion_server *s, int which, int inverse, uint64_t *polys, 
   std::lock_guard<std::mutex> lk(L->mu);
   const NttTab &T = which == 0 ? s->Tq : s->Tp;
   HIPCHK(hipMemcpyAsync(s->io, polys, (size_t)n * kN * 8, hipMemcpyHostToDevice, L->st));
-  if (inverse)
-    k_ntt_inv<<<n, kNttThreads, 0, L->st>>>(s->io, s->io, T);
+  const bool r4 = s->tune.ntt_radix == 4;
+  if (inverse && r4)
+    k_ntt_inv<4><<<n, kNttThreads, 0, L->st>>>(s->io, s->io, T);
+  else if (inverse)
+    k_ntt_inv<8><<<n, kNttThreads, 0, L->st>>>(s->io, s->io, T);
+  else if (r4)
+    k_ntt_fwd<4><<<n, kNttThreads, 0, L->st>>>(s->io, s->io, T);
   else
-    k_ntt_fwd<<<n, kNttThreads, 0, L->st>>>(s->io, s->io, T);
+    k_ntt_fwd<8><<<n, kNttThreads, 0, L->st>>>(s->io, s->io, T);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(polys, s->io, (size_t)n * kN * 8, hipMemcpyDeviceToHost, L->st));
   HIPCHK(hipStreamSynchronize(L->st));

@@ -43,18 +43,26 @@ def test_derived_parameters(cfg1):
 
 
 @pytest.mark.parametrize("which", [0, 1])
-def test_ntt_matches_oracle(cfg1, which):
+@pytest.mark.parametrize("radix", [8, 4])
+def test_ntt_matches_oracle(cfg1, which, radix):
+    """Forward and inverse transforms mod q and mod p in both pass schedules
+    (radix-8: 3+3+3+2 stages; radix-4: 2+2+2+2+3)."""
     o, srv = cfg1["o"], cfg1["srv"]
     rng = np.random.default_rng(1 + which)
     m = o.q if which == 0 else o.p
     polys = rng.integers(0, m, size=(8, N), dtype=np.uint64)
     polys[0] = 0
     polys[1] = m - 1
-    fwd = srv.tap_ntt(polys, which=which)
-    exp = np.stack([o.ntt(p, which) for p in polys]).reshape(-1)
-    assert np.array_equal(fwd, exp)
-    inv = srv.tap_ntt(fwd, which=which, inverse=True)
-    assert np.array_equal(inv, polys.reshape(-1))
+    saved = srv.get_tuning()
+    try:
+        srv.set_tuning(ntt_radix=radix)
+        fwd = srv.tap_ntt(polys, which=which)
+        exp = np.stack([o.ntt(p, which) for p in polys]).reshape(-1)
+        assert np.array_equal(fwd, exp)
+        inv = srv.tap_ntt(fwd, which=which, inverse=True)
+        assert np.array_equal(inv, polys.reshape(-1))
+    finally:
+        srv.set_tuning(**saved)
 
 
 def test_galois_keyswitch_matches_oracle(cfg1):
@@ -486,10 +494,13 @@ def test_answer_batch_matches_single_and_oracle(cfg1, nq):
                                     dict(fd_mfma_variant=1), dict(fd_mfma_variant=2),
                                     dict(fd_mfma_variant=3, db_residues=1), dict(fd_mfma_variant=3, db_residues=0),
                                     dict(db_planes=0),
-                                    dict(db_planes=0, fd_mfma_variant=1)])
+                                    dict(db_planes=0, fd_mfma_variant=1),
+                                    dict(ntt_radix=4), dict(ntt_radix=4, fuse_min=1)])
 def test_tuning_choices_are_bit_identical(cfg1, tuning):
     """Every launch-shape choice (fused vs split external products, first-dim
-    tile shapes) gives the oracle's bits, for a single query and a batch."""
+    tile shapes, transform pass schedules) gives the oracle's bits, for a
+    single query and batches of 5 and 4 (the first multiplexer level takes
+    one query per workgroup for an odd batch, two for an even one)."""
     o, srv = cfg1["o"], cfg1["srv"]
     saved = srv.get_tuning()
     try:
@@ -507,6 +518,38 @@ def test_tuning_choices_are_bit_identical(cfg1, tuning):
         got = srv.answer_batch([CID] * 5, np.stack([q] * 5))
         for i in range(5):
             assert np.array_equal(got[i], exp)
+        got = srv.answer_batch([CID] * 4, np.stack([q] * 4))
+        for i in range(4):
+            assert np.array_equal(got[i], exp)
+    finally:
+        srv.set_tuning(**saved)
+
+
+def test_batch_split_above_one_pass(cfg1):
+    """A call of 70 queries runs as 64 + 6 (batch_split = 0) or 35 + 35
+    (batch_split = 1): the same responses, each the single-query answer; two
+    also checked against the oracle.  The autotuner's batch-split axis runs for
+    a size above 64 and leaves results unchanged."""
+    o, srv = cfg1["o"], cfg1["srv"]
+    saved = srv.get_tuning()
+    rng = np.random.default_rng(70)
+    idx = rng.choice(o.num_pt, size=70, replace=False)
+    qs = np.stack([o.query(CLIENT_SEED, 2000 + i, cfg1["sk"], int(idx[i])) for i in range(70)])
+    ids = [CID] * 70
+    try:
+        srv.set_tuning(batch_split=0)
+        a = srv.answer_batch(ids, qs)
+        srv.set_tuning(batch_split=1)
+        b = srv.answer_batch(ids, qs)
+        assert np.array_equal(a, b)
+        for i in (0, 34, 35, 63, 64, 69):
+            assert np.array_equal(a[i], srv.answer(CID, qs[i])), f"query {i}"
+        for i in (5, 66):
+            exp, _ = o.answer(cfg1["db"], cfg1["gk"], cfg1["rkn"], qs[i])
+            assert np.array_equal(a[i], exp)
+        chosen = srv.autotune([1, 70], reps=1)
+        assert chosen["batch_split"] in (0, 1) and chosen["ntt_radix"] in (8, 4)
+        assert np.array_equal(srv.answer_batch(ids, qs), a)
     finally:
         srv.set_tuning(**saved)
 

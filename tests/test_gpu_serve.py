@@ -274,6 +274,32 @@ def test_serve_one_worker_serves_oldest_first(dbs):
         assert np.array_equal(got[k], s1.answer(int(c1[k]), q1[k]))
 
 
+def test_serve_even_split_of_a_backlog(dbs):
+    """batch_split = 1 on a database: a backlog above its cap (10 waiting at
+    cap 4) is taken in equal shares; every response still equals the query
+    answered alone."""
+    from onionpirv2_amd.serve import GpuServe
+    s0, s1 = dbs["servers"]
+    saved = s0.get_tuning()
+    c0, i0, q0 = make_queries(dbs, 0, 10, seed=970)
+    c1, i1, q1 = make_queries(dbs, 1, 1, seed=971)
+    try:
+        s0.set_tuning(batch_split=1)
+        with GpuServe(max_batch=4, lanes_per_db=1, workers=1) as sv:
+            d0, cap = sv.add_db(s0)
+            d1, _ = sv.add_db(s1)
+            assert cap == 4
+            busy = sv.submit(d1, int(c1[0]), q1[0])  # keeps the worker while the backlog forms
+            ts = [sv.submit(d0, int(c0[k]), q0[k]) for k in range(10)]
+            got = [sv.wait(t, db_id=d0) for t in ts]
+            sv.wait(busy, db_id=d1)
+            assert sv.stats(d0)["batches"] >= 3
+    finally:
+        s0.set_tuning(**saved)
+    for k in range(10):
+        assert np.array_equal(got[k], s0.answer(int(c0[k]), q0[k]))
+
+
 def test_serve_poll_and_timeout(dbs):
     from onionpirv2_amd.serve import GpuServe, ServeTimeout
     srv = dbs["servers"][0]
