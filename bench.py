@@ -87,7 +87,8 @@ def main() -> int:
     ap.add_argument("--fst", type=int, default=512)
     ap.add_argument("--other", type=int, default=1024)
     ap.add_argument("--batch", type=int, default=64)
-    ap.add_argument("--single-reps", type=int, default=20)
+    ap.add_argument("--single-reps", type=int, default=200,
+                    help="single queries per leg (tuned, graph, eager): enough for a p99")
     ap.add_argument("--db-seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-queries", type=int, default=10, help="measured oracle queries (1 core)")
@@ -169,6 +170,7 @@ def main() -> int:
         stages.append(st)
     log(f"[bench] single query p50 {pct(single_lat, 50):.3f} ms")
     graph_single = single_query_graph(srv, queries, args.single_reps, single_resp)
+    eager_single = single_query_graph(srv, queries, args.single_reps, single_resp, use_graph=0)
 
     # ---- batched path (the timed steps) -----------------------------------------------
     for _ in range(args.warmup):
@@ -191,21 +193,29 @@ def main() -> int:
     log(f"[bench] records recovered {ok}/{B}")
 
     # ---- rooflines of the first-dimension kernels ----------------------------------------
-    # per-kernel events exist only for eagerly launched pipelines: with graphs
-    # chosen, time the kernels on extra eager passes (same kernels, same shapes)
-    if tuning.get("use_graph"):
+    # per-kernel and per-stage events exist only for eagerly launched
+    # pipelines: where the tuning replays graphs (use_graph 1: every size, 2:
+    # single queries), time the kernels and stages on extra eager passes (same
+    # kernels, same shapes); the device totals stay the replayed calls' own
+    dev_ms = [st["total"] for st in bstages]
+    dev_single_ms = [st["total"] for st in stages]
+    single_graphed = tuning.get("use_graph") in (1, 2)
+    batch_graphed = tuning.get("use_graph") == 1
+    if single_graphed or batch_graphed:
         srv.set_tuning(use_graph=0)
-        cores = []
-        for r in range(max(3, args.single_reps // 2)):
-            srv.answer(r % B, queries[r % B])
-            cores.append(srv.stage_times()["first_dim_core"])
-        fd_cores = []
-        for _ in range(max(1, args.steps // 2)):
-            srv.answer_batch(ids, queries)
-            fd_cores.append(srv.stage_times()["first_dim_core"])
+        if single_graphed:
+            stages = []
+            for r in range(max(3, args.single_reps // 2)):
+                srv.answer(r % B, queries[r % B])
+                stages.append(srv.stage_times())
+            cores = [st["first_dim_core"] for st in stages]
+        if batch_graphed:
+            bstages = []
+            for _ in range(max(1, args.steps // 2)):
+                srv.answer_batch(ids, queries)
+                bstages.append(srv.stage_times())
         srv.set_tuning(**tuning)
-    else:
-        fd_cores = [st["first_dim_core"] for st in bstages]
+    fd_cores = [st["first_dim_core"] for st in bstages]
     # single query: k_first_dim_b1 streams the database once (HBM bound)
     core_ms = float(np.mean(cores))
     alg_bytes = db_bytes + N * fst * 16 + other * 2 * N * 8
@@ -224,8 +234,6 @@ def main() -> int:
     # value: device time of each step (HIP events from the first kernel to the
     # last, queries already in HBM); the wall clock around answer_batch also
     # holds the PCIe copies of queries and responses and is reported apart
-    dev_ms = [st["total"] for st in bstages]
-    dev_single_ms = [st["total"] for st in stages]
     value = B * 1e3 / float(np.mean(dev_ms))
     single_qps_dev = 1e3 / float(np.mean(dev_single_ms))
 
@@ -278,7 +286,8 @@ def main() -> int:
     }
 
     out["single_query_graph"] = graph_single
-    ok_all = bit_exact and graph_single["bit_exact"]
+    out["single_query_eager"] = eager_single
+    ok_all = bit_exact and graph_single["bit_exact"] and eager_single["bit_exact"]
 
     # ---- two lanes: two batches in flight on two streams --------------------------------
     if not args.no_extras:
@@ -312,10 +321,11 @@ def main() -> int:
     return 0 if ok_all else 1
 
 
-def single_query_graph(srv, queries, reps, single_resp):
-    """The single-query pipeline captured once as a hipGraph and replayed."""
+def single_query_graph(srv, queries, reps, single_resp, use_graph=1):
+    """The single-query pipeline captured once as a hipGraph and replayed
+    (use_graph=1), or launched eagerly kernel by kernel (use_graph=0)."""
     saved = srv.get_tuning()
-    srv.set_tuning(use_graph=1)
+    srv.set_tuning(use_graph=use_graph)
     try:
         srv.answer(0, queries[0])  # capture
         lat, same = [], True
@@ -328,7 +338,7 @@ def single_query_graph(srv, queries, reps, single_resp):
                 same = same and np.array_equal(got, single_resp[i])
     finally:
         srv.set_tuning(**saved)
-    log(f"[bench] single query (graph) p50 {pct(lat, 50):.3f} ms")
+    log(f"[bench] single query ({'graph' if use_graph else 'eager'}) p50 {pct(lat, 50):.3f} ms")
     return {"queries_per_s": 1e3 / float(np.mean(lat)), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
             "bit_exact": bool(same)}
 
@@ -418,14 +428,22 @@ def single_shape(args, fst, other, reps=20):
         lat.append((time.perf_counter() - t) * 1e3)
         st.append(s.stage_times())
         ok += int(np.array_equal(c.decrypt(r), dbgen.plaintext(args.db_seed + 7, 0, idx[k], t=s.info["t"])))
+    dev = [x["total"] for x in st]
     db_bytes = num_pt * N * 8
+    if s.get_tuning()["use_graph"]:
+        # replayed calls report only their total: stage times from eager passes
+        s.set_tuning(use_graph=0)
+        st = []
+        for k in range(reps // 2):
+            s.answer(1, qs[k])
+            st.append(s.stage_times())
     core = float(np.mean([x["first_dim_core"] for x in st]))
     alg = db_bytes + N * fst * 16 + other * 2 * N * 8
     s.close()
     log(f"[bench] {fst}x{other}: single query p50 {pct(lat, 50):.3f} ms, first dim {alg / core / 1e6:.0f} GB/s")
     return {"shape": f"{fst}x{other}", "db_GiB": db_bytes / 2**30, "queries": reps,
             "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
-            "device_ms": float(np.mean([x["total"] for x in st])),
+            "device_ms": float(np.mean(dev)),
             "stage_ms": {k: float(np.mean([x[k] for x in st])) for k in st[0]},
             "first_dim_GBps": alg / (core * 1e-3) / 1e9, "first_dim_frac_of_hbm_peak": alg / (core * 1e-3) / 1e9 / HBM_PEAK_GBPS,
             "records_ok": ok, "bit_exact": ok == reps}

@@ -92,7 +92,12 @@ typedef struct onion_tuning {
                              (else 4) 16-byte loads in flight per lane, bit 1
                              = nontemporal loads                             */
   uint32_t use_graph;     /* 1: each lane captures its per-batch pipeline in a
-                             hipGraph once per batch size and replays it      */
+                             hipGraph once per batch size and replays it; 2
+                             (the default): single queries only (one launch
+                             instead of ~45 keeps the synchronous call's tail
+                             short; batches of varying size launch eagerly);
+                             0: never.  A replayed call's stage times hold
+                             only the total.                                 */
   uint32_t fd_mfma_min;   /* batches of at least this many queries (and >= 2)
                              take the first dimension on the int8 matrix
                              cores (exact signed-digit products); 0 = never  */

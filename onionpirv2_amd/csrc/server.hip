@@ -184,7 +184,7 @@ struct onion_server {
   // fuse_min 512 runs a single query's first multiplexer level (512 pairs)
   // fused on the first dimension's NTT-form outputs, 1.4% faster than split
   // (batches of 64 take the same time either way)
-  onion_tuning tune{512, 256, 4, 3, 0, 2, 2, 1, 0, 8, 0};
+  onion_tuning tune{512, 256, 4, 3, 2, 2, 2, 1, 0, 8, 0};
   uint64_t tune_epoch = 1;
   Gate gate;
   mutable std::mutex last_mu;
@@ -963,6 +963,12 @@ int collect_times(onion_server *s, Lane *L, bool staged, onion_stage_times *out)
   return ONION_OK;
 }
 
+// Whether a call of n queries replays a captured graph (tune.use_graph: 0
+// never, 1 always, 2 single queries only).
+bool graph_for_size(const onion_server *s, uint32_t n) {
+  return s->tune.use_graph == 1 || (s->tune.use_graph == 2 && n == 1);
+}
+
 // Answer n (<= L->cap) queries on lane L: upload, run, download.
 // With eager_only, the pipeline is never captured (hipGraphInstantiate
 // allocates).
@@ -972,7 +978,7 @@ int answer_chunk(onion_server *s, Lane *L, uint32_t n, const uint64_t *ids, cons
   if ((rc = bind_keys(s, L, n, ids, s->I.ndim > 1))) return rc;
   HIPCHK(hipMemcpy2DAsync(L->trees + 2 * kN, s->tree_words * 8, queries, 2 * kN * 8, 2 * kN * 8, n,
                           hipMemcpyHostToDevice, L->st));
-  const bool graph = s->tune.use_graph != 0 && !eager_only;
+  const bool graph = graph_for_size(s, n) && !eager_only;
   if (graph) {
     hipGraphExec_t ex;
     if ((rc = graph_for(s, L, n, &ex))) return rc;
@@ -1034,7 +1040,7 @@ int time_pipeline(onion_server *s, Lane *L, uint32_t n, uint32_t reps, double *m
   std::vector<double> runs;
   for (uint32_t r = 0; r <= reps; ++r) {
     const auto t0 = std::chrono::steady_clock::now();
-    if (s->tune.use_graph) {
+    if (graph_for_size(s, n)) {
       hipGraphExec_t ex;
       int rc = graph_for(s, L, n, &ex);
       if (rc) return rc;
@@ -1153,7 +1159,7 @@ int autotune_impl(onion_server *s, const uint32_t *sizes, uint32_t n_sizes, uint
       {&onion_tuning::fd_rows_b1, {64, 128, 256, 512}, has1},
       {&onion_tuning::fuse_min, {1, 256, 512, 1024, 4096, 1u << 30}, true},
       {&onion_tuning::ntt_radix, {8, 4}, true},
-      {&onion_tuning::use_graph, {0, 1}, true},
+      {&onion_tuning::use_graph, {0, 1, 2}, true},
       {&onion_tuning::batch_split, {0, 1}, has_big},
   };
   for (const Axis &ax : axes) {
@@ -1637,7 +1643,7 @@ int onion_gpu_set_tuning(onion_server *s, const onion_tuning *t) {
     return fail(ONION_EINVAL, "fd_rows_b1 must be a multiple of 32 in [32, 1024]");
   if (t->fd_tile_rows != 2 && t->fd_tile_rows != 4) return fail(ONION_EINVAL, "fd_tile_rows must be 2 or 4");
   if (t->fd_b1_variant > 3) return fail(ONION_EINVAL, "fd_b1_variant must be 0..3");
-  if (t->use_graph > 1) return fail(ONION_EINVAL, "use_graph must be 0 or 1");
+  if (t->use_graph > 2) return fail(ONION_EINVAL, "use_graph must be 0, 1 or 2");
   if (t->fd_mfma_variant > 3) return fail(ONION_EINVAL, "fd_mfma_variant must be 0..3");
   if (t->db_planes > 1) return fail(ONION_EINVAL, "db_planes must be 0 or 1");
   if (t->db_residues > 1) return fail(ONION_EINVAL, "db_residues must be 0 or 1");

@@ -114,18 +114,23 @@ def test_graph_replays_match_eager(dbs):
     replay equals the eager result for its own queries."""
     srv = dbs["servers"][0]
     sets = {n: [make_queries(dbs, 0, n, seed=100 * n + r) for r in range(4)] for n in (1, 5)}
-    eager = {n: [srv.answer_batch(c, q) for c, _, q in sets[n]] for n in sets}
     saved = srv.get_tuning()
+    assert saved["use_graph"] == 2  # the default: single queries replay a graph
     try:
-        srv.set_tuning(use_graph=1)
-        for r in range(4):
-            for n in (1, 5):
-                c, idx, q = sets[n][r]
-                got = srv.answer_batch(c, q)
-                assert np.array_equal(got, eager[n][r]), f"replay {r} batch {n}"
-                # eager work between replays: an NTT round trip on the same lane
-                x = np.arange(2 * N, dtype=np.uint64)
-                assert np.array_equal(srv.tap_ntt(srv.tap_ntt(x), inverse=True), x)
+        srv.set_tuning(use_graph=0)
+        eager = {n: [srv.answer_batch(c, q) for c, _, q in sets[n]] for n in sets}
+        for mode in (1, 2):  # every size replays / single queries replay, batches run eagerly
+            srv.set_tuning(use_graph=mode)
+            for r in range(4):
+                for n in (1, 5):
+                    c, idx, q = sets[n][r]
+                    got = srv.answer_batch(c, q)
+                    assert np.array_equal(got, eager[n][r]), f"use_graph {mode} replay {r} batch {n}"
+                    # eager work between replays: an NTT round trip on the same lane
+                    x = np.arange(2 * N, dtype=np.uint64)
+                    assert np.array_equal(srv.tap_ntt(srv.tap_ntt(x), inverse=True), x)
+            c, idx, q = sets[1][0]
+            assert np.array_equal(srv.answer(int(c[0]), q[0]), eager[1][0][0])
         t = srv.stage_times()
         assert t["total"] > 0
         for i in range(5):
