@@ -108,6 +108,7 @@ def main() -> int:
     ap.add_argument("--config5-rates", default="50,200,500,1000,2000", help="its Poisson rates (queries/s)")
     ap.add_argument("--config5-workers", type=int, default=2,
                     help="batches in flight over the four databases (the serving loop's worker pool)")
+    ap.add_argument("--config5-lanes", type=int, default=1, help="lanes per database in the serving loop")
     ap.add_argument("--no-t65537", dest="t65537", action="store_false",
                     help="skip the batch-64 line at t = 65537 (PlainMod 17)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -492,7 +493,8 @@ def config5(args):
 
     rates = [float(r) for r in args.config5_rates.split(",") if r]
     runs, ok_all = [], True
-    with GpuServe(max_batch=64, lanes_per_db=1, scratch_budget=120 << 30, workers=args.config5_workers) as sv:
+    with GpuServe(max_batch=64, lanes_per_db=args.config5_lanes, scratch_budget=120 << 30,
+                  workers=args.config5_workers) as sv:
         caps = [sv.add_db(s)[1] for s in servers]
         for rate in rates:
             n = max(40, int(rate * args.serve_seconds))
@@ -535,7 +537,8 @@ def config5(args):
         s.close()
     return {"resident_GiB": resident / 2**30, "shapes": [f"{a}x{b}" for a, b in shapes], "clients": nc,
             "key_store_GiB": key_bytes / 2**30, "batch_caps": caps, "scratch_budget_GiB": 120,
-            "workers": args.config5_workers, "runs": runs, "bit_exact": bool(ok_all)}
+            "workers": args.config5_workers, "lanes_per_db": args.config5_lanes, "runs": runs,
+            "bit_exact": bool(ok_all)}
 
 
 def multi_db(args):
