@@ -109,6 +109,8 @@ def main() -> int:
     ap.add_argument("--config5-workers", type=int, default=2,
                     help="batches in flight over the four databases (the serving loop's worker pool)")
     ap.add_argument("--config5-lanes", type=int, default=1, help="lanes per database in the serving loop")
+    ap.add_argument("--multi-db-linger-us", type=int, default=0,
+                    help="how long the three-database bursts' first arrivals wait for the rest of their burst")
     ap.add_argument("--no-t65537", dest="t65537", action="store_false",
                     help="skip the batch-64 line at t = 65537 (PlainMod 17)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -543,8 +545,9 @@ def config5(args):
 
 def multi_db(args):
     """Three databases resident together (128x256, 256x1024, 512x4096: 0.5 +
-    4 + 32 GiB); bursts of 7 / 33 / 100 queries spread over them go through
-    the serving loop; every record must decrypt."""
+    4 + 32 GiB), each tuned by the autotuner for the per-database shares of
+    the bursts (3, 11 and 34 queries); bursts of 7 / 33 / 100 queries spread
+    over them go through the serving loop; every record must decrypt."""
     from onionpirv2_amd import GpuServer, dbgen
     from onionpirv2_amd.client import GpuClient
     from onionpirv2_amd.serve import GpuServe
@@ -563,9 +566,12 @@ def multi_db(args):
         servers.append(s)
     resident = sum(s.info["num_pt"] * N * 8 for s in servers)
     log(f"[bench] multi-db: {resident / 2**30:.1f} GiB resident, set up in {time.time() - t0:.1f} s")
+    t0 = time.time()
+    tunings = [s.autotune([3, 11, 34], reps=2) for s in servers]
+    log(f"[bench] multi-db: tuned in {time.time() - t0:.1f} s")
     rng = np.random.default_rng(4242)
     bursts, ok = [], True
-    with GpuServe(max_batch=64, lanes_per_db=1) as sv:
+    with GpuServe(max_batch=64, lanes_per_db=1, linger_us=args.multi_db_linger_us) as sv:
         for s in servers:
             sv.add_db(s)
         for burst in (7, 33, 100):
@@ -596,6 +602,8 @@ def multi_db(args):
     for s in servers:
         s.close()
     return {"resident_GiB": resident / 2**30, "shapes": [f"{a}x{b}" for a, b in shapes], "bursts": bursts,
+            "tuning": {f"{a}x{b}": t for (a, b), t in zip(shapes, tunings)}, "autotune_sizes": [3, 11, 34],
+            "linger_us": args.multi_db_linger_us,
             "bit_exact": bool(ok)}
 
 

@@ -186,7 +186,7 @@ struct onion_server {
   // (batches of 64 take the same time either way)
   onion_tuning tune{512, 256, 4, 3, 2, 2, 2, 1, 0, 8, 0};
   uint64_t tune_epoch = 1;
-  Gate gate;
+  mutable Gate gate;  // tuning reads take it shared (onion_gpu_get_tuning)
   mutable std::mutex last_mu;
   onion_stage_times last{};
 };
@@ -1633,6 +1633,7 @@ int onion_gpu_pack_response(onion_server *s, const uint64_t *resp, uint8_t *out,
 
 int onion_gpu_get_tuning(const onion_server *s, onion_tuning *out) {
   if (!s || !out) return fail(ONION_EINVAL, "null argument");
+  Shared g(s->gate);  // set_tuning and the autotuner write it exclusively
   *out = s->tune;
   return ONION_OK;
 }
