@@ -481,7 +481,8 @@ def config5(args):
         servers[0].set_galois_key(c, cl.galois_keys())
         servers[0].set_rgsw_key(c, cl.rgsw_key())
     t_keys = time.time() - t0
-    key_bytes = nc * (servers[-1].info["galois_key_words"] + servers[-1].info["rgsw_key_words"]) * 8
+    # device bytes: the Galois key words are held with their Shoup companions
+    key_bytes = nc * (2 * servers[-1].info["galois_key_words"] + servers[-1].info["rgsw_key_words"]) * 8
     log(f"[bench] config5: {resident / 2**30:.1f} GiB resident ({t_db:.1f} s), {nc} clients' keys "
         f"({key_bytes / 2**30:.2f} GiB, one shared store) in {t_keys:.1f} s")
     rng = np.random.default_rng(555)

@@ -178,7 +178,8 @@ int onion_gpu_db_read_ntt(onion_server *srv, uint64_t pt, uint64_t *out_n);
 /* PirServer::set_client_galois_key (src/server.cpp:480-484).  n_words = levels
  * x 4N with expan_height <= levels <= log2 N: a client may upload more levels
  * than this database needs, so that one upload serves every database that
- * shares the key store (onion_gpu_share_keys). */
+ * shares the key store (onion_gpu_share_keys).  The device holds each key
+ * word with its Shoup companion: 2 x n_words x 8 bytes per client. */
 int onion_gpu_set_galois_key(onion_server *srv, uint64_t client_id, const uint64_t *keys,
                              size_t n_words);
 /* PirServer::set_client_gsw_key (src/server.cpp:486-500): coefficient-form

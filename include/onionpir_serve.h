@@ -19,7 +19,9 @@
  *   - admission by device-memory budget: a database's batch cap is the largest
  *     that fits its lanes' scratch into what remains of the budget, and the
  *     scratch is allocated when the database is added;
- *   - latency statistics (submit to response ready): p50, p99, max, queries/s.
+ *   - latency statistics (submit to response ready): p50, p99, max, queries/s
+ *     (quantiles, mean and max over each database's most recent 2^20
+ *     completions).
  *
  * Thread safety: submit, wait and stats may be called from any threads.  A
  * query whose client has no keys on its database fails alone (ONION_ENOKEY);

@@ -213,8 +213,6 @@ struct CrtB {
   uint4 c[N];
 };
 
-constexpr int kCrtLds = kRes * 1024 * 4;  // int32 sums of one block's 1024 outputs, residue-major
-
 // S mod 2^128 of one output from its 17 residue sums (see the header), then
 // mod q.
 __device__ __forceinline__ uint64_t crt_rebuild(const int32_t (&sum)[kRes], const Mod &mq) {

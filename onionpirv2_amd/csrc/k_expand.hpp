@@ -25,6 +25,10 @@
 
 namespace onion {
 
+// Device words per Galois-key level: the key (2 components x 2 limbs x N),
+// then the Shoup companions of its words.
+constexpr size_t kKeyLevelWords = 8 * (size_t)kN;
+
 struct KsConst {
   uint64_t half_p;        // floor(p/2)
   uint64_t half_p_mod_q;  // floor(p/2) mod q
@@ -47,10 +51,11 @@ __device__ __forceinline__ uint64_t galois_gather(const uint64_t *in, int j, uin
 }
 
 // trees: query b's tree at trees + b*tree_stride (2w cts, slot s at s*2N)
-// keys:  query b's Galois keys at key_ptrs[b]; this level's key at + level*4N,
-//        laid out [comp][limb q,p][N], NTT form, each limb already scaled by
-//        N^-1 (k_key_scale at upload), so the inverse transforms of the key
-//        products need no scaling of their own
+// keys:  query b's Galois keys at key_ptrs[b]; this level's block of 8N at
+//        + level*8N: the key [comp][limb q,p][N], NTT form, each limb already
+//        scaled by N^-1 (so the inverse transforms of the key products need
+//        no scaling of their own), then the Shoup companion of every word at
+//        + 4N (k_key_prep at upload), so a key product is a Shoup product
 // xbuf:  query b's X at xbuf + b*x_stride, [node - first][comp][limb][N]
 __global__ __launch_bounds__(kNttThreads) void k_expand_ks(const uint64_t *trees, uint64_t tree_stride,
                                                            const uint64_t *const *key_ptrs, uint32_t level,
@@ -60,18 +65,18 @@ __global__ __launch_bounds__(kNttThreads) void k_expand_ks(const uint64_t *trees
   const uint32_t nl = blockIdx.x >> 2, comp = (blockIdx.x >> 1) & 1, limb = blockIdx.x & 1;
   const uint64_t node = (1ull << level) + nl;
   const NttTab &T = limb ? Tp : Tq;
-  const Mod &m = limb ? mp : mq;
   const uint64_t *c1 = trees + (size_t)blockIdx.y * tree_stride + node * 2 * kN + kN;
-  const uint64_t *key = key_ptrs[blockIdx.y] + (size_t)level * 4 * kN + (size_t)(2 * comp + limb) * kN;
+  const uint64_t *key = key_ptrs[blockIdx.y] + (size_t)level * kKeyLevelWords + (size_t)(2 * comp + limb) * kN;
   uint64_t x[8];
   // sigma(c1); its residues < q < p serve both limbs unchanged
 #pragma unroll
   for (int j = 0; j < 8; ++j) x[j] = galois_gather(c1, threadIdx.x + 256 * j, ginv, Tq.q);
   ntt_fwd(x, lds, T);
-  uint64_t kv[8];
+  uint64_t kv[8], kvs[8];
   load_blocked(kv, key);
+  load_blocked(kvs, key + 4 * kN);
 #pragma unroll
-  for (int j = 0; j < 8; ++j) x[j] = mulmod_barrett(x[j], kv[j], m);
+  for (int j = 0; j < 8; ++j) x[j] = mul_shoup_lazy(x[j], kv[j], kvs[j], T.q);  // < 2q: ntt_inv's input bound
   ntt_inv(x, lds, T);
   uint64_t *dst = xbuf + (size_t)blockIdx.y * x_stride + ((size_t)(nl * 2 + comp) * 2 + limb) * kN;
 #pragma unroll
@@ -132,16 +137,18 @@ __device__ __forceinline__ void expand_node_comp(int comp, const uint64_t (&pq)[
                                                  const uint64_t *c0, uint32_t k, uint32_t ginv, const NttTab &Tq,
                                                  const NttTab &Tp, const Mod &mq, const Mod &mp, const KsConst &ks) {
   const int t = threadIdx.x;
-  uint64_t x[8], xq[8], kv[8];
+  uint64_t x[8], xq[8], kv[8], kvs[8];
   load_blocked(kv, key + (size_t)(2 * comp) * kN);
+  load_blocked(kvs, key + (size_t)(4 + 2 * comp) * kN);
 #pragma unroll
-  for (int j = 0; j < 8; ++j) x[j] = mulmod_barrett(pq[8 * t + j], kv[j], mq);
+  for (int j = 0; j < 8; ++j) x[j] = mul_shoup_lazy(pq[8 * t + j], kv[j], kvs[j], Tq.q);
   ntt_inv(x, lds, Tq);
 #pragma unroll
   for (int j = 0; j < 8; ++j) xq[j] = ntt_inv_canonical(x[j], Tq.q);
   load_blocked(kv, key + (size_t)(2 * comp + 1) * kN);
+  load_blocked(kvs, key + (size_t)(5 + 2 * comp) * kN);
 #pragma unroll
-  for (int j = 0; j < 8; ++j) x[j] = mulmod_barrett(pp[8 * t + j], kv[j], mp);
+  for (int j = 0; j < 8; ++j) x[j] = mul_shoup_lazy(pp[8 * t + j], kv[j], kvs[j], Tp.q);
   ntt_inv(x, lds, Tp);
 #pragma unroll
   for (int j = 0; j < 8; ++j)
@@ -167,7 +174,7 @@ __global__ __launch_bounds__(kNttThreads) void k_expand_node(uint64_t *trees, ui
   uint64_t *tree = trees + (size_t)blockIdx.y * tree_stride;
   const uint64_t *c0 = tree + node * 2 * kN;
   const uint64_t *c1 = c0 + kN;
-  const uint64_t *key = key_ptrs[blockIdx.y] + (size_t)level * 4 * kN;
+  const uint64_t *key = key_ptrs[blockIdx.y] + (size_t)level * kKeyLevelWords;
   const int t = threadIdx.x;
   uint64_t x[8];
   // sigma(c1); its residues < q < p serve both limbs unchanged
@@ -206,11 +213,12 @@ __device__ __forceinline__ void ntt_node_comp(int comp, const uint64_t (&P)[8], 
                                               const Mod &mq, const Mod &mp, const KsConst &ks) {
   const int t = threadIdx.x;
   const uint64_t q = Tq.q, p = Tp.q;
-  uint64_t x[8], kv[8];
+  uint64_t x[8], kv[8], ksh[8];
   // the p-limb product in coefficient form: the rounding term of the key switch
   load_blocked(kv, key + (size_t)(2 * comp + 1) * kN);
+  load_blocked(ksh, key + (size_t)(5 + 2 * comp) * kN);
 #pragma unroll
-  for (int j = 0; j < 8; ++j) x[j] = mulmod_barrett(P[j], kv[j], mp);
+  for (int j = 0; j < 8; ++j) x[j] = mul_shoup_lazy(P[j], kv[j], ksh[j], p);  // < 2p: ntt_inv's input bound
   ntt_inv(x, lds, Tp);
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
@@ -225,19 +233,23 @@ __device__ __forceinline__ void ntt_node_comp(int comp, const uint64_t (&P)[8], 
   ntt_fwd(x, lds, Tq);
   // slot pairs 8t + j, 8t + j + 1: 16-byte loads and stores
   const uint4 *kq = reinterpret_cast<const uint4 *>(key + (size_t)(2 * comp) * kN + 8 * t);
+  const uint4 *kqs = reinterpret_cast<const uint4 *>(key + (size_t)(4 + 2 * comp) * kN + 8 * t);
   uint4 *lo = reinterpret_cast<uint4 *>(left + (size_t)comp * kN + 8 * t);
   uint4 *ro = reinterpret_cast<uint4 *>(right + (size_t)comp * kN + 8 * t);
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const uint4 kp = kq[i];
+    const uint4 kp = kq[i], kps = kqs[i];
     uint64_t lv[2], rv[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int j = 2 * i + h;
       const uint32_t s = 8 * t + j, src = galois_slot(s, g);
       const uint64_t kj = h ? ((uint64_t)kp.w << 32) | kp.z : ((uint64_t)kp.y << 32) | kp.x;
-      // N^-1 NTT(X_q) = NTT_q(sigma(c1)) * key_q (as stored)
-      const uint64_t xq = mulmod_barrett(Q[1][src], kj, mq);
+      const uint64_t kjs = h ? ((uint64_t)kps.w << 32) | kps.z : ((uint64_t)kps.y << 32) | kps.x;
+      // N^-1 NTT(X_q) = NTT_q(sigma(c1)) * key_q (as stored), in [0, 2q): the
+      // subtraction below leaves it below 2q, and the Shoup product after it
+      // takes any input
+      const uint64_t xq = mul_shoup_lazy(Q[1][src], kj, kjs, q);
       uint64_t ksv = mul_shoup_lazy(sub_mod(xq, ntt_fwd_final(x[j], q), q), ks.np_inv, ks.np_inv_s, q);
       if (ksv >= q) ksv -= q;
       // c'_0 = sigma(c0) + KS_0, c'_1 = KS_1
@@ -277,7 +289,7 @@ __global__ __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(3))
   const uint64_t node = (uint64_t)k + nl;
   uint64_t *tree = trees + (size_t)blockIdx.y * tree_stride;
   const uint64_t *c = tree + node * 2 * kN;
-  const uint64_t *key = key_ptrs[blockIdx.y] + (size_t)level * 4 * kN;
+  const uint64_t *key = key_ptrs[blockIdx.y] + (size_t)level * kKeyLevelWords;
   const int t = threadIdx.x;
   uint64_t x[8], P[8];
   if (NTT_IN) {
@@ -314,14 +326,32 @@ __global__ __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(3))
   ntt_node_comp(1, P, Q, key, xk, g, lds, left, right, Tq, Tp, mq, mp, ks);
 }
 
-// Galois keys as stored: every value of limb q (p) times N^-1 mod q (mod p).
-// Layout [level][comp][limb][N]; grid = n_words / 256.
-__global__ __launch_bounds__(256) void k_key_scale(uint64_t *keys, uint64_t n_words, uint64_t nq_inv, uint64_t np_inv,
-                                                   Mod mq, Mod mp) {
-  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+// Shoup companion floor(w 2^64 / m) of w < m, from the Barrett ratio
+// R = floor(2^128 / m) of m: floor(w R / 2^64) = w R_hi + floor(w R_lo / 2^64)
+// is the companion or one less (w R / 2^64 falls short of w 2^64 / m by
+// w (2^128 mod m) / (m 2^64) < 1), and the remainder w 2^64 - c m, below 2m
+// and so equal to -c m mod 2^64, tells which.
+__device__ __forceinline__ uint64_t shoup_companion(uint64_t w, const Mod &m) {
+  uint64_t c = w * m.r_hi + mulhi64(w, m.r_lo);
+  if ((uint64_t)0 - c * m.q >= m.q) ++c;
+  return c;
+}
+
+// Galois keys as stored: every value of limb q (p) times N^-1 mod q (mod p),
+// then its Shoup companion.  Layout [level][key | companions][comp][limb][N]
+// (kKeyLevelWords per level); the uploaded keys are in the first half of each
+// level.  grid = levels * 4N / 256.
+__global__ __launch_bounds__(256) void k_key_prep(uint64_t *keys, uint64_t n_words, uint64_t nq_inv, uint64_t np_inv,
+                                                  Mod mq, Mod mp) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;  // index among the levels' 4N key words
   if (i >= n_words) return;
-  const bool limb_p = (i / kN) & 1;
-  keys[i] = limb_p ? mulmod_barrett(keys[i], np_inv, mp) : mulmod_barrett(keys[i], nq_inv, mq);
+  const uint64_t level = i / (4 * kN), o = i % (4 * kN);
+  uint64_t *w = keys + level * kKeyLevelWords + o;
+  const bool limb_p = (o / kN) & 1;
+  const Mod &m = limb_p ? mp : mq;
+  const uint64_t v = mulmod_barrett(*w, limb_p ? np_inv : nq_inv, m);
+  w[0] = v;
+  w[4 * kN] = shoup_companion(v, m);
 }
 
 }  // namespace onion

@@ -39,11 +39,25 @@ struct Request {
   Clock::time_point t_submit, t_done;
 };
 
+// Latencies kept per database for the quantiles: the most recent kLatWindow
+// completions (a ring), so a long-running loop holds at most 8 MiB each.
+constexpr size_t kLatWindow = size_t(1) << 20;
+
 struct Stats {
   uint64_t submitted = 0, completed = 0, failed = 0, batches = 0, batched = 0;
   std::vector<double> lat_ms;
+  size_t lat_next = 0;  // next ring slot once lat_ms holds kLatWindow values
   bool any = false;
   Clock::time_point first, last;
+
+  void add_latency(double ms) {
+    if (lat_ms.size() < kLatWindow) {
+      lat_ms.push_back(ms);
+    } else {
+      lat_ms[lat_next] = ms;
+      lat_next = (lat_next + 1) % kLatWindow;
+    }
+  }
 };
 
 struct Db {
@@ -163,7 +177,7 @@ void worker(onion_serve *sv) {
         if (rc[i] == ONION_OK) {
           r.resp.assign(rbuf.begin() + i * ct, rbuf.begin() + (i + 1) * ct);
           D->st.completed += 1;
-          D->st.lat_ms.push_back(std::chrono::duration<double, std::milli>(now - r.t_submit).count());
+          D->st.add_latency(std::chrono::duration<double, std::milli>(now - r.t_submit).count());
         } else {
           r.err = msg[i];
           D->st.failed += 1;

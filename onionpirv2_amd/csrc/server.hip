@@ -56,7 +56,7 @@ int fail(int code, const std::string &msg) {
   } while (0)
 
 struct ClientKeys {
-  uint64_t *gk = nullptr;  // levels x 4N, NTT form
+  uint64_t *gk = nullptr;  // levels x kKeyLevelWords: NTT-form key, then Shoup companions
   uint64_t *rk = nullptr;  // 2 l_key x 2N, NTT form
   uint32_t gk_levels = 0;
 };
@@ -1097,7 +1097,9 @@ int autotune_impl(onion_server *s, const uint32_t *sizes, uint32_t n_sizes, uint
   int rc;
   if ((rc = ensure_scratch(s, L, maxn))) return rc;
   uint64_t *gk = nullptr, *rk = nullptr;
-  if ((rc = dalloc(&gk, s->I.galois_key_words)) || (rc = dalloc(&rk, s->I.rgsw_key_words))) {
+  // Galois keys as the store holds them: key and Shoup companions per level
+  const size_t gk_dev_words = 2 * (size_t)s->I.galois_key_words;
+  if ((rc = dalloc(&gk, gk_dev_words)) || (rc = dalloc(&rk, s->I.rgsw_key_words))) {
     dfree(gk);
     return rc;
   }
@@ -1106,10 +1108,10 @@ int autotune_impl(onion_server *s, const uint32_t *sizes, uint32_t n_sizes, uint
     ~Cleanup() { dfree(a); dfree(b); }
   } cleanup{gk, rk};
   {
-    std::vector<uint64_t> h(std::max<size_t>(s->I.galois_key_words, s->I.rgsw_key_words));
+    std::vector<uint64_t> h(std::max<size_t>(gk_dev_words, s->I.rgsw_key_words));
     uint64_t key = stream_key(0x5475ULL, 1);
     for (size_t i = 0; i < h.size(); ++i) h[i] = rand64_keyed(key, i) % s->I.q;
-    HIPCHK(hipMemcpy(gk, h.data(), (size_t)s->I.galois_key_words * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(gk, h.data(), gk_dev_words * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(rk, h.data(), (size_t)s->I.rgsw_key_words * 8, hipMemcpyHostToDevice));
     std::vector<uint64_t> q(2 * kN);
     for (uint32_t b = 0; b < maxn; ++b) {
@@ -1473,23 +1475,27 @@ int onion_gpu_set_galois_key(onion_server *s, uint64_t id, const uint64_t *keys,
   Exclusive kx(ks->gate);  // no batch of any server sharing the store is in flight
   hipStream_t st = s->lanes[0]->st;
   ClientKeys &ck = ks->clients[id];
-  if (ck.gk && ck.gk_levels * level_words < n) {
+  const size_t levels = n / level_words;
+  if (ck.gk && ck.gk_levels < levels) {
     dfree(ck.gk);
     ck.gk = nullptr;
     ck.gk_levels = 0;
   }
   if (!ck.gk) {
-    int rc = dalloc(&ck.gk, n);
+    int rc = dalloc(&ck.gk, levels * kKeyLevelWords);
     if (rc) {
       if (!ck.rk) ks->clients.erase(id);
       return rc;
     }
   }
-  ck.gk_levels = (uint32_t)(n / level_words);
-  HIPCHK(hipMemcpyAsync(ck.gk, keys, n * 8, hipMemcpyHostToDevice, st));
-  // stored times N^-1 per limb: the expansion's inverse transforms then need no scaling
+  ck.gk_levels = (uint32_t)levels;
+  // each level's 4N words into the first half of its kKeyLevelWords block,
+  // stored times N^-1 per limb (the expansion's inverse transforms then need
+  // no scaling), with their Shoup companions in the second half
   if (n) {  // a database without expansion levels takes an empty key
-    k_key_scale<<<(uint32_t)((n + 255) / 256), 256, 0, st>>>(ck.gk, n, s->Tq.n_inv, s->Tp.n_inv, s->mq, s->mp);
+    HIPCHK(hipMemcpy2DAsync(ck.gk, kKeyLevelWords * 8, keys, level_words * 8, level_words * 8, levels,
+                            hipMemcpyHostToDevice, st));
+    k_key_prep<<<(uint32_t)((n + 255) / 256), 256, 0, st>>>(ck.gk, n, s->Tq.n_inv, s->Tp.n_inv, s->mq, s->mp);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipStreamSynchronize(st));
