@@ -186,6 +186,40 @@ def test_small_shape_generic_kernel_and_upload():
         assert np.array_equal(srv.answer(1, q), exp)
 
 
+@pytest.mark.parametrize("fst,other", [(1024, 2), (64, 1), (12, 4), (33, 8)])
+def test_edge_shapes_single_and_batch(fst, other):
+    """Shapes at the edges of the path, single queries and a batch of 5,
+    against the oracle: fst = 1024 (expansion height 11 = log2 N, the
+    tallest tree), other = 1 (one dimension: no conversion, no multiplexer),
+    fst = 12 and 33 (first dimensions off the matrix cores' multiple of 32:
+    the batch takes the 64-bit VALU tile).  At fst = 1024 the noise of an
+    11-level expansion summed over 1024 products exceeds the decryption
+    bound at both plaintext moduli (the oracle's responses do not decrypt
+    either), so there only the bits are compared."""
+    from onionpirv2_amd import GpuServer
+    o = Oracle(fst=fst, other=other)
+    db = o.db_build(31, 0)
+    sk = o.keygen(77)
+    gk, rk = o.galois_keys(77, sk), o.rgsw_key(77, sk)
+    rkn = o.rgsw_key_ntt(rk)
+    rng = np.random.default_rng(fst * 100 + other)
+    idx = [0, o.num_pt - 1] + [int(i) for i in rng.integers(0, o.num_pt, size=3)]
+    qs = np.stack([o.query(77, 600 + k, sk, i) for k, i in enumerate(idx)])
+    exp = [o.answer(db, gk, rkn, q)[0] for q in qs]
+    with GpuServer(fst, other) as srv:
+        srv.db_generate(31, 0)
+        srv.set_galois_key(3, gk)
+        srv.set_rgsw_key(3, rk)
+        for k in (0, 1):
+            got = srv.answer(3, qs[k])
+            assert np.array_equal(got, exp[k]), f"single query {k}"
+            if fst <= 512:
+                assert np.array_equal(o.decrypt_small(sk, got), o.db_plaintext(31, 0, idx[k]))
+        got = srv.answer_batch([3] * len(idx), qs)
+        for k in range(len(idx)):
+            assert np.array_equal(got[k], exp[k]), f"batch query {k}"
+
+
 def test_first_dim_wraps_mod_2_128():
     """All database words and selection values q-1: 512 * (q-1)^2 >= 2^128,
     so the reference's uint128 accumulator wraps; the device must agree."""
