@@ -32,6 +32,18 @@ __device__ __forceinline__ uint64_t mul_shoup_lazy(uint64_t x, uint64_t w, uint6
   return x * w - mulhi64(x, ws) * q;
 }
 
+// x * w mod q in [0, 4q) for any 64-bit x (q < 2^62): Shoup with a truncated
+// quotient.  floor(x ws / 2^64) = x1 s1 + floor((x1 s0 + x0 s1 + hi(x0 s0)) / 2^32)
+// (x = x1 2^32 + x0, ws = s1 2^32 + s0); x1 s1 + hi(x1 s0) + hi(x0 s1) is at
+// most 2 below it, so the remainder is below 4q instead of 2q.  Three fewer
+// multiply-adds than the exact quotient, and no zero-extended addends.  For
+// the transforms' butterflies, whose bounds allow for it.
+__device__ __forceinline__ uint64_t mul_shoup_4q(uint64_t x, uint64_t w, uint64_t ws, uint64_t q) {
+  const uint32_t x0 = (uint32_t)x, x1 = (uint32_t)(x >> 32), s0 = (uint32_t)ws, s1 = (uint32_t)(ws >> 32);
+  const uint64_t qh = (uint64_t)x1 * s1 + __umulhi(x1, s0) + __umulhi(x0, s1);
+  return x * w - qh * q;
+}
+
 __device__ __forceinline__ uint64_t add_mod(uint64_t a, uint64_t b, uint64_t q) {
   uint64_t s = a + b;
   return s >= q ? s - q : s;

@@ -2,14 +2,16 @@
 // one workgroup of 256 threads with 8 coefficients per thread in registers.
 //
 // Forward: Cooley-Tukey with bit-reversed twiddles psi^bitrev(k) and lazy
-// butterflies: a Shoup product is in [0, 2q) for any 64-bit input, so each
-// stage grows the bound on the values by 2q, and with q < 2^60 a value may
-// grow to 16q before it must be brought down.  The input half of a butterfly
-// is reduced by 8q only in stages 3 and 7 (of 0..10), so the values stay
-// below 16q throughout (ntt_fwd_final brings them to [0, q)).  Output slot
-// e = a(psi^(2*bitrev(e)+1)), the order SEAL's ntt_negacyclic_harvey produces.  Inverse: Gentleman-Sande
-// with psi^-bitrev(k), values in [0, 2q); the N^-1 scale is applied by the
-// caller as it reads the result out (ntt_inv_final).
+// butterflies: the twiddle product (mul_shoup_4q, a Shoup product with a
+// truncated quotient) is in [0, 4q) for any 64-bit input, so each stage grows
+// the bound on the values by 4q, and with q < 2^60 a value may grow to 16q
+// before it must be brought down.  The input half of a butterfly is reduced by
+// 8q in stages 3, 5, 7 and 9 (of 0..10), so the values stay below 16q
+// throughout (ntt_fwd_final brings them to [0, q)).  Output slot
+// e = a(psi^(2*bitrev(e)+1)), the order SEAL's ntt_negacyclic_harvey
+// produces.  Inverse: Gentleman-Sande with psi^-bitrev(k), values in [0, 4q);
+// the N^-1 scale is applied by the caller as it reads the result out
+// (ntt_inv_final).
 //
 // The 11 radix-2 stages run as four register passes (3 + 3 + 3 + 2 stages).
 // In a pass with top span H a thread owns the 8 coefficients b + j*H/4,
@@ -35,15 +37,15 @@ constexpr int kNttLds = 2048 + 2048 / 16;  // padded LDS words per transform
 __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
 
 // Forward butterfly on values below 16q; CORRECT first reduces x from
-// [0, 16q) to [0, 8q).  Outputs are below (bound of u) + 2q.
+// [0, 16q) to [0, 8q).  Outputs are below (bound of u) + 4q.
 template <bool CORRECT>
 __device__ __forceinline__ void ct_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t ws, uint64_t q,
                                         uint64_t two_q) {
   uint64_t u = x;
   if (CORRECT && u >= 4 * two_q) u -= 4 * two_q;
-  const uint64_t v = mul_shoup_lazy(y, w, ws, q);
+  const uint64_t v = mul_shoup_4q(y, w, ws, q);
   x = u + v;
-  y = u + two_q - v;
+  y = u + 2 * two_q - v;
 }
 
 // The first forward stage for inputs below 2^32 (digit polynomials): u needs
@@ -60,13 +62,14 @@ __device__ __forceinline__ void ct_bfly_small(uint64_t &x, uint64_t &y, uint64_t
   y = u + two_q - v;
 }
 
+// Inverse butterfly on values below 4q; outputs below 4q.
 __device__ __forceinline__ void gs_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t ws,
                                         uint64_t q, uint64_t two_q) {
-  const uint64_t u = x, v = y;
+  const uint64_t u = x, v = y, four_q = 2 * two_q;
   uint64_t s = u + v;
-  if (s >= two_q) s -= two_q;
+  if (s >= four_q) s -= four_q;
   x = s;
-  y = mul_shoup_lazy(u + two_q - v, w, ws, q);
+  y = mul_shoup_4q(u + four_q - v, w, ws, q);
 }
 
 __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int mask) {
@@ -153,9 +156,9 @@ __device__ __forceinline__ void lds_get(const uint64_t *lds, uint64_t (&x)[8], i
 
 // Forward NTT.  Entry: natural layout, values < 4q (SMALL: < 2^32, as digit
 // polynomials are).  Exit: blocked layout, values in [0, 16q) (q < 2^60).
-// Bounds by stage: 6q 8q 10q | 10q 12q 14q | 16q 10q 12q | 14q 16q, the
-// reductions being in stage 3 (pass B, span 128) and stage 7 (pass C,
-// span 8).  Uses
+// Bounds by stage: 8q 12q 16q | 12q 16q 12q | 16q 12q 16q | 12q 16q, the
+// reductions being in stages 3 and 5 (pass B, spans 128 and 32), stage 7
+// (pass C, span 8) and stage 9 (pass D, span 2).  Uses
 // lds[kNttLds]; begins with a barrier so the buffer may have been read just
 // before the call.
 template <bool SMALL = false>
@@ -173,7 +176,7 @@ __device__ __forceinline__ void ntt_fwd(uint64_t (&x)[8], uint64_t *lds, const N
   lds_put(lds, x, t, 256);
   __syncthreads();
   lds_get(lds, x, bB, 32);
-  fwd8<1>(x, tw, q, true);
+  fwd8<5>(x, tw, q, true);
   // pass C: spans 16, 8, 4 (stride 4): m = 64 + t/4
   const int bC = pass_base(t, 2);
   load_tw8(tw, T.rp, T.rps, 64 + (t >> 2));
@@ -188,11 +191,11 @@ __device__ __forceinline__ void ntt_fwd(uint64_t (&x)[8], uint64_t *lds, const N
   lds_put(lds, x, bC, 4);
   __syncthreads();
   lds_get(lds, x, 8 * t, 1);
-  fwd8<0>(x, tw, q, false);
+  fwd8<2>(x, tw, q, false);
 }
 
-// Inverse NTT without the N^-1 scale.  Entry: blocked layout, values < 2q.
-// Exit: natural layout, values in [0, 2q).  Begins with a barrier.
+// Inverse NTT without the N^-1 scale.  Entry: blocked layout, values < 4q.
+// Exit: natural layout, values in [0, 4q).  Begins with a barrier.
 __device__ __forceinline__ void ntt_inv(uint64_t (&x)[8], uint64_t *lds, const NttTab &T) {
   const int t = threadIdx.x;
   const uint64_t q = T.q;
@@ -326,25 +329,25 @@ __device__ __forceinline__ void ntt_fwd_r4(uint64_t (&x)[8], uint64_t *lds, cons
   __syncthreads();
   lds_get4(lds, y, t, 7);
   fwd4<2>(y, tw, q);
-  load_tw4(tw, T.rp, T.rps, t, 5);  // spans 64, 32 (stages 4, 5)
+  load_tw4(tw, T.rp, T.rps, t, 5);  // spans 64, 32 (stages 4, 5: reduce before 5)
   __syncthreads();
   lds_put4(lds, y, t, 7);
   __syncthreads();
   lds_get4(lds, y, t, 5);
-  fwd4<0>(y, tw, q);
+  fwd4<2>(y, tw, q);
   load_tw4(tw, T.rp, T.rps, t, 3);  // spans 16, 8 (stages 6, 7: reduce before 7)
   __syncthreads();
   lds_put4(lds, y, t, 5);
   __syncthreads();
   lds_get4(lds, y, t, 3);
   fwd4<2>(y, tw, q);
-  Tw8 tw8;  // spans 4, 2, 1 on slots 8t .. 8t+7 (stages 8, 9, 10)
+  Tw8 tw8;  // spans 4, 2, 1 on slots 8t .. 8t+7 (stages 8, 9, 10: reduce before 9)
   load_tw8(tw8, T.rp, T.rps, 256 + t);
   __syncthreads();
   lds_put4(lds, y, t, 3);
   __syncthreads();
   lds_get(lds, x, 8 * t, 1);
-  fwd8<0>(x, tw8, q, true);
+  fwd8<2>(x, tw8, q, true);
 }
 
 // ntt_inv in the radix-4 schedule (same entry and exit layouts and bounds).
@@ -414,11 +417,15 @@ __device__ __forceinline__ uint64_t ntt_fwd_final(uint64_t v, uint64_t q) {
 // residue.
 __device__ __forceinline__ uint64_t ntt_fwd_below_8q(uint64_t v, uint64_t q) { return v >= 8 * q ? v - 8 * q : v; }
 
-// Canonical value of an inverse-NTT output in [0, 2q), not scaled (for inputs
+// Canonical value of an inverse-NTT output in [0, 4q), not scaled (for inputs
 // that carry the N^-1 factor already).
-__device__ __forceinline__ uint64_t ntt_inv_canonical(uint64_t v, uint64_t q) { return v >= q ? v - q : v; }
+__device__ __forceinline__ uint64_t ntt_inv_canonical(uint64_t v, uint64_t q) {
+  if (v >= 2 * q) v -= 2 * q;
+  return v >= q ? v - q : v;
+}
 
-// Canonical value of an inverse-NTT output in [0, 2q), scaled by N^-1.
+// Canonical value of an inverse-NTT output in [0, 4q), scaled by N^-1 (the
+// Shoup product takes any input and lands in [0, 2q)).
 __device__ __forceinline__ uint64_t ntt_inv_final(uint64_t v, const NttTab &T) {
   const uint64_t r = mul_shoup_lazy(v, T.n_inv, T.n_inv_s, T.q);
   return r >= T.q ? r - T.q : r;
