@@ -107,11 +107,7 @@ typedef struct onion_tuning {
                                persistent workgroup per compute unit that
                                reduces each tile's outputs between the next
                                tile's products (needs the plane copy and
-                               fst % 64 == 0, else 0 runs); 3: the product in
-                               17 residues (17 int8 products per 64x64-bit
-                               product instead of 64 digit products; needs
-                               the residue copy, else the digit kernels
-                               run)                                         */
+                               fst % 64 == 0, else 0 runs)                 */
   uint32_t db_planes;     /* 1: keep a second device copy of the database in
                              signed-digit planes (as many bytes again as the
                              database), rebuilt on every database write, so
@@ -121,11 +117,6 @@ typedef struct onion_tuning {
                              multiple of 32.  The server starts with 1 when
                              the copy takes at most a third of the free
                              device memory, else with 0.                    */
-  uint32_t db_residues;   /* 1: keep a device copy of the database as 17
-                             residues per word (17 bytes per word), rebuilt
-                             on every database write, for fd_mfma_variant 3;
-                             0: free it (the default).  Only for first
-                             dimensions that are a multiple of 32.          */
   uint32_t ntt_radix;     /* pass schedule of the transforms in the fused
                              multiplexer levels (and onion_gpu_tap_ntt):
                              8 = radix-8 register passes (3 + 3 + 3 + 2

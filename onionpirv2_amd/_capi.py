@@ -40,7 +40,7 @@ class OnionStageTimes(C.Structure):
 class OnionTuning(C.Structure):
     _fields_ = [("fuse_min", C.c_uint32), ("fd_rows_b1", C.c_uint32), ("fd_tile_rows", C.c_uint32),
                 ("fd_b1_variant", C.c_uint32), ("use_graph", C.c_uint32), ("fd_mfma_min", C.c_uint32),
-                ("fd_mfma_variant", C.c_uint32), ("db_planes", C.c_uint32), ("db_residues", C.c_uint32),
+                ("fd_mfma_variant", C.c_uint32), ("db_planes", C.c_uint32),
                 ("ntt_radix", C.c_uint32), ("batch_split", C.c_uint32)]
 
 

@@ -31,7 +31,6 @@
 #include "k_db.hpp"
 #include "k_expand.hpp"
 #include "k_first_dim.hpp"
-#include "k_crt.hpp"
 #include "k_first_dim_mfma.hpp"
 #include "k_gsw.hpp"
 #include "k_ntt.hpp"
@@ -164,10 +163,9 @@ struct onion_server {
   std::vector<uint32_t> gal, ginv;  // each level's Galois element N/2^a + 1, its inverse mod 2N
   uint64_t ms_M = 0;
   uint32_t ms_f = 0;
-  // database, its signed-digit plane copy (tune.db_planes) and its residue
-  // copy (tune.db_residues)
+  // database and its signed-digit plane copy (tune.db_planes)
   uint64_t *db = nullptr;
-  uint8_t *dbp = nullptr, *dbr = nullptr;
+  uint8_t *dbp = nullptr;
   std::vector<uint8_t> pt_written;
   uint64_t n_written = 0;
   // client keys (possibly shared with other servers)
@@ -184,7 +182,7 @@ struct onion_server {
   // fuse_min 512 runs a single query's first multiplexer level (512 pairs)
   // fused on the first dimension's NTT-form outputs, 1.4% faster than split
   // (batches of 64 take the same time either way)
-  onion_tuning tune{512, 256, 4, 3, 2, 2, 2, 1, 0, 8, 0};
+  onion_tuning tune{512, 256, 4, 3, 2, 2, 2, 1, 8, 0};
   uint64_t tune_epoch = 1;
   mutable Gate gate;  // tuning reads take it shared (onion_gpu_get_tuning)
   mutable std::mutex last_mu;
@@ -221,11 +219,9 @@ uint32_t cpad_for(uint32_t nq) {
 }
 
 // uint32 words of a lane's selection buffer for `cap` queries: limbs for one
-// query (8 bytes per slot, j and column), chunks for batches (8 digit bytes or
-// 17 residue bytes per slot, j and column).
+// query, digit chunks for batches (8 bytes per slot, j and column either way).
 uint64_t qs_words(const onion_server *s, uint32_t cap) {
-  const uint64_t n = (uint64_t)kN * s->P.fst_dim * cpad_for(cap);
-  return cap >= 2 && s->P.fst_dim % 32 == 0 ? (n * kRes + 3) / 4 : n * 2;
+  return (uint64_t)kN * s->P.fst_dim * cpad_for(cap) * 2;
 }
 
 void drop_graphs(Lane *L) {
@@ -438,43 +434,6 @@ int sync_planes(onion_server *s) {
   return ONION_OK;
 }
 
-// Bytes of the residue copy: 17 per word, rows padded to a multiple of 32.
-uint64_t residues_bytes(const onion_server *s) {
-  return (uint64_t)kN * ((s->P.other_dim + 31) / 32 * 32) * s->P.fst_dim * kRes;
-}
-
-// Rebuild the residue chunks of plaintexts pt_lo .. pt_lo + n - 1 (records
-// of 16 plaintexts of one row, as planes_update).
-void residues_update(const onion_server *s, hipStream_t st, uint64_t pt_lo, uint64_t n) {
-  if (!s->dbr || !n) return;
-  const uint64_t lo = pt_lo / 16, hi = (pt_lo + n + 15) / 16;
-  k_db_residues<<<dim3((uint32_t)((hi - lo + 255) / 256), kN), 256, 0, st>>>(s->db, s->dbr, s->P.fst_dim,
-                                                                             s->P.other_dim, lo, hi);
-}
-
-// Allocate (and fill) or free the residue copy to match tune.db_residues
-// (for first dimensions that are a multiple of 32).  The caller holds the
-// gate exclusively.
-int sync_residues(onion_server *s) {
-  const bool want = s->tune.db_residues && s->P.fst_dim % 32 == 0;
-  if (want == (s->dbr != nullptr)) return ONION_OK;
-  hipStream_t st = s->lanes[0]->st;
-  if (!want) {
-    HIPCHK(hipStreamSynchronize(st));
-    dfree(s->dbr);
-    s->dbr = nullptr;
-    return ONION_OK;
-  }
-  int rc;
-  if ((rc = dalloc(&s->dbr, residues_bytes(s)))) return rc;
-  if (s->n_written) {
-    residues_update(s, st, 0, s->I.num_pt);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(st));
-  }
-  return ONION_OK;
-}
-
 int check_db(const onion_server *s) {
   if (!s->db || s->n_written != s->I.num_pt)
     return fail(ONION_ESTATE, "database incomplete: " + std::to_string(s->n_written) + " of " +
@@ -650,58 +609,11 @@ void launch_gather(const onion_server *s, Lane *L, uint32_t nq) {
 // keep_ntt: leave the outputs in out2 (NTT form) for
 // run_mux_first instead of transforming them into L->fd.  sel_ntt: the
 // selection vectors are in NTT form already.
-// The residue product runs when chosen and the server keeps the residue copy.
-bool use_crt(const onion_server *s) {
-  return s->tune.fd_mfma_variant == 3 && s->dbr && s->P.fst_dim % 32 == 0;
-}
-
-// Prefetch depth of the database's chunks: as deep as the registers of the
-// column-block count allow, and a divisor of the steps per tile.
-template <int CB, int D>
-void launch_crt_d(const onion_server *s, Lane *L, const uint8_t *Br, uint32_t n_rt, uint32_t ncols) {
-  const uint32_t total = kN * n_rt, G = std::max(8u, std::min(total, s->n_cu) / 8 * 8);
-  k_first_dim_crt<CB, D><<<G, 512, 0, L->st>>>(s->dbr, Br, L->digits, s->P.fst_dim, s->P.other_dim, ncols, n_rt,
-                                               s->mq);
-}
-template <int CB>
-void launch_crt(const onion_server *s, Lane *L, const uint8_t *Br, uint32_t n_rt, uint32_t ncols) {
-  const uint32_t steps = s->P.fst_dim / 32;
-  constexpr int DMAX = CB >= 2 ? 2 : 1;  // deeper rings spill at 128 columns
-  if (DMAX >= 4 && steps % 4 == 0)
-    launch_crt_d<CB, DMAX >= 4 ? 4 : 1>(s, L, Br, n_rt, ncols);
-  else if (DMAX >= 2 && steps % 2 == 0)
-    launch_crt_d<CB, DMAX >= 2 ? 2 : 1>(s, L, Br, n_rt, ncols);
-  else
-    launch_crt_d<CB, 1>(s, L, Br, n_rt, ncols);
-}
-
 void run_first_dim_mfma(const onion_server *s, Lane *L, uint32_t nq, uint64_t *sel, uint64_t sel_stride,
                         bool time_core, bool keep_ntt, bool sel_ntt, hipEvent_t join) {
   const uint32_t fst = s->P.fst_dim, other = s->P.other_dim, ncp = cpad_for(nq), ncols = 2 * nq;
   uint8_t *Bp = reinterpret_cast<uint8_t *>(L->Qs);
   const dim3 chunk_grid(kN / 8, ncp / 32, fst / 32);
-  if (use_crt(s)) {
-    if (sel_ntt) {
-      k_sel_residues<true><<<chunk_grid, 256, 0, L->st>>>(sel, sel_stride, Bp, fst, ncols);
-    } else {
-      k_sel_digits<<<dim3(fst * 2, nq), kNttThreads, 0, L->st>>>(sel, sel_stride, s->Tq);
-      k_sel_residues<false><<<chunk_grid, 256, 0, L->st>>>(sel, sel_stride, Bp, fst, ncols);
-    }
-    if (join) (void)hipStreamWaitEvent(L->st, join, 0);
-    if (time_core) (void)hipEventRecord(L->ev[6], L->st);
-    // tiles of 4 blocks of 32 x 32: one row block x four column blocks, ...
-    const uint32_t rows_per_tile = 32 * (128 / ncp);
-    const uint32_t n_rt = (other + rows_per_tile - 1) / rows_per_tile;
-    if (ncp == 32)
-      launch_crt<1>(s, L, Bp, n_rt, ncols);
-    else if (ncp == 64)
-      launch_crt<2>(s, L, Bp, n_rt, ncols);
-    else
-      launch_crt<4>(s, L, Bp, n_rt, ncols);
-    if (time_core) (void)hipEventRecord(L->ev[7], L->st);
-    if (!keep_ntt) launch_gather(s, L, nq);
-    return;
-  }
   if (sel_ntt) {
     k_sel_chunks<true><<<chunk_grid, 256, 0, L->st>>>(sel, sel_stride, Bp, fst, ncols);
   } else {
@@ -1154,7 +1066,7 @@ int autotune_impl(onion_server *s, const uint32_t *sizes, uint32_t n_sizes, uint
     bool active;
   };
   const std::vector<Axis> axes = {
-      {&onion_tuning::fd_mfma_variant, {0, 1, 2, 3}, hasb},
+      {&onion_tuning::fd_mfma_variant, {0, 1, 2}, hasb},
       {&onion_tuning::fd_mfma_min, {2, 4, 8, 16, 0}, has_small},
       {&onion_tuning::fd_tile_rows, {2, 4}, hasb},
       {&onion_tuning::fd_b1_variant, {0, 1, 2, 3}, has1},
@@ -1322,8 +1234,7 @@ int onion_gpu_create(const onion_params *params, onion_server **out) {
   s->pt_written.assign(I.num_pt, 0);
   // the plane copy when it takes at most a third of the free device memory
   // (the rest stays for batch scratch and other databases); without it the
-  // products convert in-kernel.  The residue copy is made on request only
-  // (tune.db_residues).
+  // products convert in-kernel.
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && (uint64_t)free_b / 3 < planes_bytes(s))
     s->tune.db_planes = 0;
@@ -1352,7 +1263,7 @@ int onion_gpu_destroy(onion_server *s) {
     if (s->slane) destroy_lane(s->slane.get());
     s->slane.reset();
     s->keys.reset();  // the store goes with its last server
-    void *bufs[] = {s->tabs, s->db, s->dbp, s->dbr, s->io, s->pack};
+    void *bufs[] = {s->tabs, s->db, s->dbp, s->io, s->pack};
     for (void *b : bufs) dfree(b);
   }
   delete s;
@@ -1420,7 +1331,6 @@ int onion_gpu_db_generate(onion_server *s, uint64_t seed, int mode) {
                                                        s->lift, s->Tq);
   }
   planes_update(s, st, 0, s->I.num_pt);
-  residues_update(s, st, 0, s->I.num_pt);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(st));
   std::fill(s->pt_written.begin(), s->pt_written.end(), 1);
@@ -1442,7 +1352,6 @@ int onion_gpu_db_upload_plain(onion_server *s, uint64_t pt_lo, uint64_t n_pt, co
     HIPCHK(hipMemcpyAsync(s->io, coeffs + off * kN, n * kN * 8, hipMemcpyHostToDevice, st));
     k_db_plain<<<(uint32_t)n, kNttThreads, 0, st>>>(s->io, s->db, pt_lo + off, s->I.num_pt, s->lift, s->Tq);
     planes_update(s, st, pt_lo + off, n);
-    residues_update(s, st, pt_lo + off, n);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     for (uint64_t i = 0; i < n; ++i) {
@@ -1651,24 +1560,18 @@ int onion_gpu_set_tuning(onion_server *s, const onion_tuning *t) {
   if (t->fd_tile_rows != 2 && t->fd_tile_rows != 4) return fail(ONION_EINVAL, "fd_tile_rows must be 2 or 4");
   if (t->fd_b1_variant > 3) return fail(ONION_EINVAL, "fd_b1_variant must be 0..3");
   if (t->use_graph > 2) return fail(ONION_EINVAL, "use_graph must be 0, 1 or 2");
-  if (t->fd_mfma_variant > 3) return fail(ONION_EINVAL, "fd_mfma_variant must be 0..3");
+  if (t->fd_mfma_variant > 2) return fail(ONION_EINVAL, "fd_mfma_variant must be 0, 1 or 2");
   if (t->db_planes > 1) return fail(ONION_EINVAL, "db_planes must be 0 or 1");
-  if (t->db_residues > 1) return fail(ONION_EINVAL, "db_residues must be 0 or 1");
   if (t->ntt_radix != 8 && t->ntt_radix != 4) return fail(ONION_EINVAL, "ntt_radix must be 8 or 4");
   if (t->batch_split > 1) return fail(ONION_EINVAL, "batch_split must be 0 or 1");
   Exclusive x(s->gate);
-  const uint32_t had = s->tune.db_planes, had_r = s->tune.db_residues;
+  const uint32_t had = s->tune.db_planes;
   s->tune = *t;
   s->tune.fuse_min = std::max(1u, t->fuse_min);
   ++s->tune_epoch;  // graphs captured before hold the old kernels and buffers
   int rc;
   if ((rc = sync_planes(s))) {
     s->tune.db_planes = had && s->dbp ? 1 : 0;
-    s->tune.db_residues = had_r && s->dbr ? 1 : 0;
-    return rc;
-  }
-  if ((rc = sync_residues(s))) {
-    s->tune.db_residues = had_r && s->dbr ? 1 : 0;
     return rc;
   }
   return ONION_OK;

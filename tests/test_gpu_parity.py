@@ -246,8 +246,8 @@ def test_first_dim_wraps_mod_2_128():
         rng = np.random.default_rng(8)
         sel2 = rng.integers(0, o.q, size=(o.fst, 2 * N), dtype=np.uint64)
         batch = np.stack([sel, sel2, sel])
-        for mfma_min, planes, var in ((2, 1, 0), (2, 1, 2), (2, 1, 3), (2, 0, 0), (0, 0, 0)):
-            srv.set_tuning(fd_mfma_min=mfma_min, db_planes=planes, fd_mfma_variant=var, db_residues=int(var == 3))
+        for mfma_min, planes, var in ((2, 1, 0), (2, 1, 2), (2, 0, 0), (0, 0, 0)):
+            srv.set_tuning(fd_mfma_min=mfma_min, db_planes=planes, fd_mfma_variant=var)
             gb = srv.tap_first_dim_batch(batch)
             assert np.array_equal(gb[0], exp) and np.array_equal(gb[2], exp), f"fd_mfma_min {mfma_min} var {var}"
             assert np.array_equal(gb[1], o.first_dim(db.reshape(-1), sel2))
@@ -269,7 +269,6 @@ def test_partial_upload_updates_the_plane_copy():
     sel = rng.integers(0, o.q, size=(3, o.fst, 2 * N), dtype=np.uint64)
     with GpuServer(64, 4) as srv:
         assert srv.get_tuning()["db_planes"] == 1
-        srv.set_tuning(db_residues=1)  # the residue copy is rebuilt on writes too
         srv.db_generate(3, 0)
         srv.db_upload_plain(lo, pts)
         srv.set_tuning(fd_mfma_min=2)
@@ -277,8 +276,6 @@ def test_partial_upload_updates_the_plane_copy():
         for b in range(3):
             assert np.array_equal(got[b], o.first_dim(db.reshape(-1), sel[b])), f"query {b}"
         srv.set_tuning(fd_mfma_variant=2)
-        assert np.array_equal(srv.tap_first_dim_batch(sel), got)
-        srv.set_tuning(fd_mfma_variant=3)
         assert np.array_equal(srv.tap_first_dim_batch(sel), got)
         srv.set_tuning(db_planes=0)
         assert np.array_equal(srv.tap_first_dim_batch(sel), got)
@@ -306,32 +303,6 @@ def test_persistent_first_dim_matches_oracle(fst, other, nq):
         srv.set_tuning(fd_mfma_min=2, fd_mfma_variant=2)
         got = srv.tap_first_dim_batch(sel)
         for b in sorted({0, nq - 1}):
-            assert np.array_equal(got[b], o.first_dim(db, sel[b])), f"query {b}"
-        srv.set_tuning(fd_mfma_variant=0)
-        assert np.array_equal(srv.tap_first_dim_batch(sel), got)
-
-
-@pytest.mark.parametrize("fst,other,nq", [(1024, 8, 3), (128, 64, 40), (32, 16, 20), (96, 16, 33), (64, 2, 2)])
-def test_residue_first_dim_matches_oracle(fst, other, nq):
-    """The product in 17 residues (fd_mfma_variant = 3) against the oracle and
-    the digit kernels.  fst = 1024: 32 steps per tile (the largest sums, up to
-    2^130); fst = 32 and 96: one and three steps.  nq = 3 / 20 / 40 take one,
-    two and four column blocks of 32 (tiles of 128, 64 and 32 rows); other =
-    8, 16 and 2 leave rows of a tile masked.  Query 0's first four slots are
-    all q - 1, so those sums pass 2^128 at fst >= 512."""
-    from onionpirv2_amd import GpuServer
-    o = Oracle(fst=fst, other=other)
-    db = o.db_build(8, 0)
-    rng = np.random.default_rng(fst + 7 * nq)
-    sel = rng.integers(0, o.q, size=(nq, fst, 2 * N), dtype=np.uint64)
-    sel[0, :, :4] = o.q - 1
-    sel[-1, :2] = 0
-    with GpuServer(fst, other) as srv:
-        srv.set_tuning(db_residues=1)
-        srv.db_generate(8, 0)
-        srv.set_tuning(fd_mfma_min=2, fd_mfma_variant=3)
-        got = srv.tap_first_dim_batch(sel)
-        for b in sorted({0, nq // 2, nq - 1}):
             assert np.array_equal(got[b], o.first_dim(db, sel[b])), f"query {b}"
         srv.set_tuning(fd_mfma_variant=0)
         assert np.array_equal(srv.tap_first_dim_batch(sel), got)
@@ -481,8 +452,8 @@ def test_first_dim_batch_matches_oracle(cfg1, nq):
     try:
         srv.set_tuning(fd_mfma_min=0)
         assert np.array_equal(srv.tap_first_dim_batch(sel), got)
-        for var, planes in ((0, 1), (1, 1), (2, 1), (3, 1), (0, 0), (1, 0)):
-            srv.set_tuning(fd_mfma_min=2, fd_mfma_variant=var, db_planes=planes, db_residues=int(var == 3))
+        for var, planes in ((0, 1), (1, 1), (2, 1), (0, 0), (1, 0)):
+            srv.set_tuning(fd_mfma_min=2, fd_mfma_variant=var, db_planes=planes)
             assert np.array_equal(srv.tap_first_dim_batch(sel), got), f"variant {var} planes {planes}"
     finally:
         srv.set_tuning(**saved)
@@ -526,7 +497,6 @@ def test_answer_batch_matches_single_and_oracle(cfg1, nq):
                                     dict(fd_b1_variant=3), dict(fd_mfma_min=0),
                                     dict(fd_mfma_min=6), dict(fd_mfma_variant=0),
                                     dict(fd_mfma_variant=1), dict(fd_mfma_variant=2),
-                                    dict(fd_mfma_variant=3, db_residues=1), dict(fd_mfma_variant=3, db_residues=0),
                                     dict(db_planes=0),
                                     dict(db_planes=0, fd_mfma_variant=1),
                                     dict(ntt_radix=4), dict(ntt_radix=4, fuse_min=1)])
