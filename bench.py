@@ -109,6 +109,8 @@ def main() -> int:
     ap.add_argument("--config5-workers", type=int, default=2,
                     help="batches in flight over the four databases (the serving loop's worker pool)")
     ap.add_argument("--config5-lanes", type=int, default=1, help="lanes per database in the serving loop")
+    ap.add_argument("--no-single-legs", dest="single_legs", action="store_false",
+                    help="skip the single-query legs that force graph replay and eager launches")
     ap.add_argument("--multi-db-linger-us", type=int, default=0,
                     help="how long the three-database bursts' first arrivals wait for the rest of their burst")
     ap.add_argument("--no-t65537", dest="t65537", action="store_false",
@@ -172,8 +174,10 @@ def main() -> int:
         cores.append(st["first_dim_core"])
         stages.append(st)
     log(f"[bench] single query p50 {pct(single_lat, 50):.3f} ms")
-    graph_single = single_query_graph(srv, queries, args.single_reps, single_resp)
-    eager_single = single_query_graph(srv, queries, args.single_reps, single_resp, use_graph=0)
+    graph_single = eager_single = None
+    if args.single_legs:
+        graph_single = single_query_graph(srv, queries, args.single_reps, single_resp)
+        eager_single = single_query_graph(srv, queries, args.single_reps, single_resp, use_graph=0)
 
     # ---- batched path (the timed steps) -----------------------------------------------
     for _ in range(args.warmup):
@@ -290,7 +294,7 @@ def main() -> int:
 
     out["single_query_graph"] = graph_single
     out["single_query_eager"] = eager_single
-    ok_all = bit_exact and graph_single["bit_exact"] and eager_single["bit_exact"]
+    ok_all = bit_exact and all(leg["bit_exact"] for leg in (graph_single, eager_single) if leg)
 
     # ---- two lanes: two batches in flight on two streams --------------------------------
     if not args.no_extras:
