@@ -26,10 +26,25 @@ struct NttTab {
 
 __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) { return __umul64hi(a, b); }
 
+// x w - h q mod 2^64, computed as x w + h (2^64 - q): two 32x32->64
+// multiply-adds with 64-bit addends for the low word and four 32-bit
+// multiplies for the high word, where the subtraction would take a borrow
+// chain (two instructions and a wait state).  The empty asm keeps the compiler
+// from refolding the high-word sum into a 64-bit multiply-add.
+__device__ __forceinline__ uint64_t mul_sub_lo(uint64_t x, uint64_t w, uint64_t h, uint64_t q) {
+  const uint64_t nq = 0 - q;
+  const uint32_t x0 = (uint32_t)x, x1 = (uint32_t)(x >> 32), w0 = (uint32_t)w, w1 = (uint32_t)(w >> 32);
+  const uint32_t h0 = (uint32_t)h, h1 = (uint32_t)(h >> 32), n0 = (uint32_t)nq, n1 = (uint32_t)(nq >> 32);
+  const uint64_t r = (uint64_t)h0 * n0 + (uint64_t)x0 * w0;
+  uint32_t c = x0 * w1 + x1 * w0 + h0 * n1 + h1 * n0;
+  asm("" : "+v"(c));
+  return r + ((uint64_t)c << 32);
+}
+
 // x * w mod q in [0, 2q) for any 64-bit x (Shoup; q < 2^62)
 __device__ __forceinline__ uint64_t mul_shoup_lazy(uint64_t x, uint64_t w, uint64_t ws,
                                                    uint64_t q) {
-  return x * w - mulhi64(x, ws) * q;
+  return mul_sub_lo(x, w, mulhi64(x, ws), q);
 }
 
 // x * w mod q in [0, 4q) for any 64-bit x (q < 2^62): Shoup with a truncated
@@ -41,7 +56,7 @@ __device__ __forceinline__ uint64_t mul_shoup_lazy(uint64_t x, uint64_t w, uint6
 __device__ __forceinline__ uint64_t mul_shoup_4q(uint64_t x, uint64_t w, uint64_t ws, uint64_t q) {
   const uint32_t x0 = (uint32_t)x, x1 = (uint32_t)(x >> 32), s0 = (uint32_t)ws, s1 = (uint32_t)(ws >> 32);
   const uint64_t qh = (uint64_t)x1 * s1 + __umulhi(x1, s0) + __umulhi(x0, s1);
-  return x * w - qh * q;
+  return mul_sub_lo(x, w, qh, q);
 }
 
 __device__ __forceinline__ uint64_t add_mod(uint64_t a, uint64_t b, uint64_t q) {

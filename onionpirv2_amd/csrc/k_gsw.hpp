@@ -7,7 +7,7 @@
 // Split in two launches so that every digit NTT of every ciphertext of every
 // query in the batch runs in parallel:
 //   k_decomp_ntt  -- one workgroup per (ciphertext, digit): extract the digit
-//                    polynomial, NTT it, store it (NTT form, below 8q:
+//                    polynomial, NTT it, store it (NTT form, below 2^63:
 //                    the 128-bit sums reduce it exactly);
 //   k_ext_mac_*   -- one workgroup per (ciphertext, output component):
 //                    sum_j D_j * G_j[comp] in 128-bit, Barrett mod q, then
@@ -59,7 +59,7 @@ __global__ __launch_bounds__(kNttThreads) void k_decomp_ntt(const uint64_t *src_
   else
     ntt_fwd<true>(x, lds, T);  // digits < 2^base_log2 <= 2^32
 #pragma unroll
-  for (int r = 0; r < 8; ++r) x[r] = ntt_fwd_below_8q(x[r], T.q);  // only ext_mac8 reads D
+  for (int r = 0; r < 8; ++r) x[r] = ntt_fwd_below_2p63(x[r], T.q);  // only ext_mac8 reads D
   store_blocked(D + (size_t)blockIdx.y * d_stride + ((size_t)ct * rows + j) * kN, x);
 }
 
@@ -180,7 +180,7 @@ __device__ __forceinline__ void ext_fused_core(const uint64_t *xs, const uint64_
       for (int r = 0; r < 8; ++r) d[r] = (v[r] >> sh) & mask;
       ntt_fwd<true>(d, lds, T);  // digits < 2^base_log2 <= 2^32
 #pragma unroll
-      for (int r = 0; r < 8; ++r) d[r] = ntt_fwd_below_8q(d[r], T.q);
+      for (int r = 0; r < 8; ++r) d[r] = ntt_fwd_below_2p63(d[r], T.q);
       mac_row(lo, hi, d, G + (size_t)(poly * l + kk) * 2 * kN);
     }
     if (NTT_IN) {  // the last row takes NTT(x): the leaf as read
@@ -189,7 +189,7 @@ __device__ __forceinline__ void ext_fused_core(const uint64_t *xs, const uint64_
     } else if (FOLD) {
       ntt_fwd(v, lds, T);
 #pragma unroll
-      for (int r = 0; r < 8; ++r) v[r] = ntt_fwd_below_8q(v[r], T.q);
+      for (int r = 0; r < 8; ++r) v[r] = ntt_fwd_below_2p63(v[r], T.q);
       mac_row(lo, hi, v, g_last);
     }
   }
@@ -383,7 +383,7 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
         for (int j = 0; j < 8; ++j) d[j] = (coef[t + 256 * j] >> sh) & mask;
         ntt_fwd_sched<R, true>(d, lds, T);  // digits < 2^base_log2
 #pragma unroll
-        for (int j = 0; j < 8; ++j) d[j] = ntt_fwd_below_8q(d[j], q);
+        for (int j = 0; j < 8; ++j) d[j] = ntt_fwd_below_2p63(d[j], q);
         mac_row(lo, hi, d, Gb + (size_t)(poly * l + kk) * 2 * kN);
       }
       uint64_t d[8];

@@ -4,10 +4,11 @@
 // Forward: Cooley-Tukey with bit-reversed twiddles psi^bitrev(k) and lazy
 // butterflies: the twiddle product (mul_shoup_4q, a Shoup product with a
 // truncated quotient) is in [0, 4q) for any 64-bit input, so each stage grows
-// the bound on the values by 4q, and with q < 2^60 a value may grow to 16q
-// before it must be brought down.  The input half of a butterfly is reduced by
-// 8q in stages 3, 5, 7 and 9 (of 0..10), so the values stay below 16q
-// throughout (ntt_fwd_final brings them to [0, q)).  Output slot
+// the bound on the values by 4q.  The moduli are q = 2^60 - c with c < 2^28,
+// so 8q = 2^63 - 8c: the input half of a butterfly loses 8q when its top bit
+// is set (reduce_2p63: a bit test and a multiply-add) in stages 3, 5, 7 and 9
+// (of 0..10), which leaves it below 2^63, and the values stay below
+// 2^63 + 8q < 2^64 throughout (ntt_fwd_final brings them to [0, q)).  Output slot
 // e = a(psi^(2*bitrev(e)+1)), the order SEAL's ntt_negacyclic_harvey
 // produces.  Inverse: Gentleman-Sande with psi^-bitrev(k), values in [0, 4q);
 // the N^-1 scale is applied by the caller as it reads the result out
@@ -36,13 +37,20 @@ constexpr int kNttLds = 2048 + 2048 / 16;  // padded LDS words per transform
 
 __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
 
-// Forward butterfly on values below 16q; CORRECT first reduces x from
-// [0, 16q) to [0, 8q).  Outputs are below (bound of u) + 4q.
+// v - 8q when bit 63 of v is set, else v (q = 2^60 - c, c < 2^28, so
+// 8q = 2^63 - 8c and the subtraction is clearing the bit and adding 8c):
+// below 2^63 for every v < 2^63 + 8q, and congruent to v mod q.
+__device__ __forceinline__ uint64_t reduce_2p63(uint64_t v, uint64_t q) {
+  const uint32_t c8 = (uint32_t)((1ULL << 63) - 8 * q);
+  return (uint64_t)(uint32_t)(v >> 63) * c8 + (v & 0x7FFFFFFFFFFFFFFFULL);
+}
+
+// Forward butterfly on values below 2^63 + 8q; CORRECT first brings x below
+// 2^63 (reduce_2p63).  Outputs are below (bound of u) + 4q.
 template <bool CORRECT>
 __device__ __forceinline__ void ct_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t ws, uint64_t q,
                                         uint64_t two_q) {
-  uint64_t u = x;
-  if (CORRECT && u >= 4 * two_q) u -= 4 * two_q;
+  const uint64_t u = CORRECT ? reduce_2p63(x, q) : x;
   const uint64_t v = mul_shoup_4q(y, w, ws, q);
   x = u + v;
   y = u + 2 * two_q - v;
@@ -57,7 +65,7 @@ __device__ __forceinline__ void ct_bfly_small(uint64_t &x, uint64_t &y, uint64_t
   const uint64_t u = x;
   const uint32_t y32 = (uint32_t)y;
   const uint64_t t = (uint64_t)y32 * (ws >> 32) + (((uint64_t)y32 * (uint32_t)ws) >> 32);
-  const uint64_t v = (uint64_t)y32 * w - (t >> 32) * q;
+  const uint64_t v = mul_sub_lo(y32, w, t >> 32, q);
   x = u + v;
   y = u + two_q - v;
 }
@@ -155,10 +163,10 @@ __device__ __forceinline__ void lds_get(const uint64_t *lds, uint64_t (&x)[8], i
 }
 
 // Forward NTT.  Entry: natural layout, values < 4q (SMALL: < 2^32, as digit
-// polynomials are).  Exit: blocked layout, values in [0, 16q) (q < 2^60).
-// Bounds by stage: 8q 12q 16q | 12q 16q 12q | 16q 12q 16q | 12q 16q, the
-// reductions being in stages 3 and 5 (pass B, spans 128 and 32), stage 7
-// (pass C, span 8) and stage 9 (pass D, span 2).  Uses
+// polynomials are).  Exit: blocked layout, values below 2^63 + 8q.
+// Bounds by stage (R = 2^63): 8q 12q 16q | R+4q R+8q R+4q | R+8q R+4q R+8q |
+// R+4q R+8q, the reductions being in stages 3 and 5 (pass B, spans 128 and
+// 32), stage 7 (pass C, span 8) and stage 9 (pass D, span 2).  Uses
 // lds[kNttLds]; begins with a barrier so the buffer may have been read just
 // before the call.
 template <bool SMALL = false>
@@ -402,20 +410,20 @@ __device__ __forceinline__ void ntt_inv_sched(uint64_t (&x)[8], uint64_t *lds, c
     ntt_inv(x, lds, T);
 }
 
-// Canonical value of a forward-NTT output in [0, 16q).
+// Canonical value of a forward-NTT output (any 64-bit value).
 __device__ __forceinline__ uint64_t ntt_fwd_final(uint64_t v, uint64_t q) {
-  if (v >= 8 * q) v -= 8 * q;
-  if (v >= 4 * q) v -= 4 * q;
-  if (v >= 2 * q) v -= 2 * q;
-  if (v >= q) v -= q;
-  return v;
+  // 2^60 = c mod q: v = 2^60 v_hi + v_lo = v_lo + c v_hi < 2^60 + 15c < 2q
+  const uint32_t c = (uint32_t)((1ULL << 60) - q);
+  const uint64_t r = (uint64_t)(uint32_t)(v >> 60) * c + (v & ((1ULL << 60) - 1));
+  return r >= q ? r - q : r;
 }
 
-// A forward-NTT output in [0, 16q) brought below 8q (< 2^63), enough for the
-// 128-bit multiply-accumulate of external products (mac128 with a canonical
-// second operand), whose Barrett reduction at the end returns the canonical
-// residue.
-__device__ __forceinline__ uint64_t ntt_fwd_below_8q(uint64_t v, uint64_t q) { return v >= 8 * q ? v - 8 * q : v; }
+// A forward-NTT output (below 2^63 + 8q) brought below 2^63 by one
+// conditional subtraction of 8q, enough for the 128-bit multiply-accumulate of
+// external products (mac128 with a canonical second operand), whose Barrett
+// reduction at the end returns the canonical residue.  (reduce_2p63 does the
+// same in fewer instructions, but there the compiler spills k_convert_fused.)
+__device__ __forceinline__ uint64_t ntt_fwd_below_2p63(uint64_t v, uint64_t q) { return v >= 8 * q ? v - 8 * q : v; }
 
 // Canonical value of an inverse-NTT output in [0, 4q), not scaled (for inputs
 // that carry the N^-1 factor already).

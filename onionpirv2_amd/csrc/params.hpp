@@ -36,9 +36,9 @@ inline int derive_params(const onion_params &P, onion_param_info &I, std::string
     err = "plain_bits out of range";
     return ONION_EINVAL;
   }
-  // an external product sums 2l products of a transformed digit (below 8q)
-  // and a gadget value (below q) in 128 bits: 32 of them stay below
-  // 256 q^2 < 2^128, so l and l_key are at most 16
+  // an external product sums 2l products of a transformed digit (below 2^63)
+  // and a gadget value (below q < 2^60) in 128 bits: 32 of them stay below
+  // 2^128, so l and l_key are at most 16
   if (P.gsw_l == 0 || P.gsw_l_key == 0 || P.gsw_l > 16 || P.gsw_l_key > 16) {
     err = "gsw_l / gsw_l_key out of range (1..16)";
     return ONION_EINVAL;

@@ -3,13 +3,17 @@
 onionpirv2_amd/csrc/ntt.hpp keeps butterfly values unreduced: the twiddle
 product mul_shoup_4q (arith.hpp) truncates the Shoup quotient to
 x1 s1 + hi(x1 s0) + hi(x0 s1), so it lands in [0, 4q); forward stages add up
-to 4q and reduce the input half by 8q before stages 3, 5, 7 and 9; inverse
-values stay below 4q.  This models those butterflies with Python integers and
-checks, for both 60-bit primes on worst-case and random inputs, that the
-quotient is at most 2 below the exact one, that every value stays below 2^64
-and within the stated bounds (16q forward, 4q inverse), and that the results
-agree mod q with exact arithmetic.  A reduction step dropped from the schedule
-or a wrong bound in the butterflies fails it.
+to 4q, and before stages 3, 5, 7 and 9 the input half loses 8q when its bit 63
+is set (reduce_2p63: q = 2^60 - c, so 8q = 2^63 - 8c); inverse values stay
+below 4q.  The forward output is brought to [0, q) by ntt_fwd_final (2^60 = c
+mod q, then one conditional subtraction) or below 2^63 by ntt_fwd_below_2p63.
+This models those steps with Python integers and checks, for both 60-bit
+primes on worst-case and random inputs, that the quotient is at most 2 below
+the exact one, that every value stays below 2^64 and within the stated bounds
+(2^63 + 8q forward, 4q inverse), that the final reductions are canonical or
+below 2^63, and that the results agree mod q with exact arithmetic.  A
+reduction step dropped from the schedule or a wrong bound in the butterflies
+fails it.
 """
 import random
 
@@ -31,6 +35,22 @@ def shoup4(x, w, ws, q):
     return r
 
 
+def reduce_2p63(v, q):
+    c8 = (1 << 63) - 8 * q
+    assert 0 < c8 < 1 << 32
+    r = (v >> 63) * c8 + (v & ((1 << 63) - 1))
+    assert r == (v - 8 * q if v >= 1 << 63 else v) and r < 1 << 63
+    return r
+
+
+def fwd_final(v, q):
+    c = (1 << 60) - q
+    assert 0 < c < 1 << 28
+    r = (v >> 60) * c + (v & ((1 << 60) - 1))
+    assert r < 2 * q
+    return r - q if r >= q else r
+
+
 def shoup_exact(x, w, ws, q):
     r = (x * w - ((x * ws) >> 64) * q) & M64
     assert r < 2 * q
@@ -47,8 +67,8 @@ def forward(a, q, tab, tabs, small, corrections=(3, 5, 7, 9)):
             m = N // (2 * H) + (i >> (11 - s))
             w, ws = tab[m], tabs[m]
             u = x[i]
-            if s in corrections and u >= 8 * q:
-                u -= 8 * q
+            if s in corrections:
+                u = reduce_2p63(u, q)
             if small and s == 0:  # ct_bfly_small: exact quotient for inputs below 2^32
                 v = shoup_exact(x[i + H], w, ws, q)
                 x[i], x[i + H] = u + v, u + 2 * q - v
@@ -56,7 +76,7 @@ def forward(a, q, tab, tabs, small, corrections=(3, 5, 7, 9)):
                 v = shoup4(x[i + H], w, ws, q)
                 x[i], x[i + H] = u + v, u + 4 * q - v
             assert x[i] <= M64 and x[i + H] <= M64
-        assert max(x) < 16 * q, f"stage {s}: {max(x) / q:.2f} q"
+        assert max(x) < (1 << 63) + 8 * q, f"stage {s}: {max(x) / q:.2f} q"
     return x
 
 
@@ -109,14 +129,26 @@ def test_lazy_forward_and_inverse_bounds(q):
              ([rng.randrange(4 * q) for _ in range(N)], False), ([rng.randrange(1 << 32) for _ in range(N)], True)]
     for a, small in cases:
         out = forward(a, q, tab, tabs, small)
-        assert [v % q for v in out] == exact_forward(a, q, tab)
+        ref = exact_forward(a, q, tab)
+        assert [v % q for v in out] == ref
+        assert [fwd_final(v, q) for v in out] == ref
+        assert all(reduce_2p63(v, q) % q == r for v, r in zip(out, ref))
     for a in ([4 * q - 1] * N, [rng.randrange(4 * q) for _ in range(N)]):
         out = inverse(a, q, tab, tabs)
         assert [v % q for v in out] == exact_inverse(a, q, tab)
 
 
+def test_final_reductions_on_extreme_values():
+    for q in PRIMES:
+        for v in (0, q - 1, q, 2 * q - 1, 8 * q - 1, 8 * q, (1 << 63) - 1, 1 << 63, (1 << 63) + 8 * q - 1,
+                  (1 << 64) - 1):
+            assert fwd_final(v, q) == v % q
+            if v < (1 << 63) + 8 * q:
+                assert reduce_2p63(v, q) % q == v % q
+
+
 def test_a_missing_reduction_overflows_the_bound():
-    """Without the reduction before stage 9 the worst case passes 16q."""
+    """Without the reduction before stage 9 the worst case passes 2^63 + 8q."""
     q = PRIMES[0]
     rng = random.Random(5)
     tab = [rng.randrange(q) for _ in range(N)]
