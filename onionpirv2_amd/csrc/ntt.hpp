@@ -10,9 +10,10 @@
 // (of 0..10), which leaves it below 2^63, and the values stay below
 // 2^63 + 8q < 2^64 throughout (ntt_fwd_final brings them to [0, q)).  Output slot
 // e = a(psi^(2*bitrev(e)+1)), the order SEAL's ntt_negacyclic_harvey
-// produces.  Inverse: Gentleman-Sande with psi^-bitrev(k), values in [0, 4q);
-// the N^-1 scale is applied by the caller as it reads the result out
-// (ntt_inv_final).
+// produces.  Inverse: Gentleman-Sande with psi^-bitrev(k); the sum half is
+// brought below 2^62 + 12c by reduce_2p62 (4q = 2^62 - 4c) and the difference
+// half is a twiddle product, below 4q; the N^-1 scale is applied by the caller
+// as it reads the result out (ntt_inv_final).
 //
 // The 11 radix-2 stages run as four register passes (3 + 3 + 3 + 2 stages).
 // In a pass with top span H a thread owns the 8 coefficients b + j*H/4,
@@ -70,14 +71,19 @@ __device__ __forceinline__ void ct_bfly_small(uint64_t &x, uint64_t &y, uint64_t
   y = u + two_q - v;
 }
 
-// Inverse butterfly on values below 4q; outputs below 4q.
+// (v mod 2^62) + (v >> 62) 4c: congruent to v mod q (4q = 2^62 - 4c) and
+// below 2^62 + 12c for every 64-bit v.
+__device__ __forceinline__ uint64_t reduce_2p62(uint64_t v, uint64_t q) {
+  const uint32_t c4 = (uint32_t)((1ULL << 62) - 4 * q);
+  return (uint64_t)(uint32_t)(v >> 62) * c4 + (v & ((1ULL << 62) - 1));
+}
+
+// Inverse butterfly on values below 8q; outputs below 2^62 + 12c (< 8q).
 __device__ __forceinline__ void gs_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t ws,
                                         uint64_t q, uint64_t two_q) {
-  const uint64_t u = x, v = y, four_q = 2 * two_q;
-  uint64_t s = u + v;
-  if (s >= four_q) s -= four_q;
-  x = s;
-  y = mul_shoup_4q(u + four_q - v, w, ws, q);
+  const uint64_t u = x, v = y;
+  x = reduce_2p62(u + v, q);
+  y = mul_shoup_4q(u + 4 * two_q - v, w, ws, q);
 }
 
 __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int mask) {
@@ -202,8 +208,8 @@ __device__ __forceinline__ void ntt_fwd(uint64_t (&x)[8], uint64_t *lds, const N
   fwd8<2>(x, tw, q, false);
 }
 
-// Inverse NTT without the N^-1 scale.  Entry: blocked layout, values < 4q.
-// Exit: natural layout, values in [0, 4q).  Begins with a barrier.
+// Inverse NTT without the N^-1 scale.  Entry: blocked layout, values < 8q.
+// Exit: natural layout, values below 2^62 + 12c.  Begins with a barrier.
 __device__ __forceinline__ void ntt_inv(uint64_t (&x)[8], uint64_t *lds, const NttTab &T) {
   const int t = threadIdx.x;
   const uint64_t q = T.q;
@@ -410,13 +416,16 @@ __device__ __forceinline__ void ntt_inv_sched(uint64_t (&x)[8], uint64_t *lds, c
     ntt_inv(x, lds, T);
 }
 
-// Canonical value of a forward-NTT output (any 64-bit value).
-__device__ __forceinline__ uint64_t ntt_fwd_final(uint64_t v, uint64_t q) {
+// Canonical residue of any 64-bit value (q = 2^60 - c, c < 2^28).
+__device__ __forceinline__ uint64_t canonical_2p60(uint64_t v, uint64_t q) {
   // 2^60 = c mod q: v = 2^60 v_hi + v_lo = v_lo + c v_hi < 2^60 + 15c < 2q
   const uint32_t c = (uint32_t)((1ULL << 60) - q);
   const uint64_t r = (uint64_t)(uint32_t)(v >> 60) * c + (v & ((1ULL << 60) - 1));
   return r >= q ? r - q : r;
 }
+
+// Canonical value of a forward-NTT output.
+__device__ __forceinline__ uint64_t ntt_fwd_final(uint64_t v, uint64_t q) { return canonical_2p60(v, q); }
 
 // A forward-NTT output (below 2^63 + 8q) brought below 2^63 by one
 // conditional subtraction of 8q, enough for the 128-bit multiply-accumulate of
@@ -425,15 +434,12 @@ __device__ __forceinline__ uint64_t ntt_fwd_final(uint64_t v, uint64_t q) {
 // same in fewer instructions, but there the compiler spills k_convert_fused.)
 __device__ __forceinline__ uint64_t ntt_fwd_below_2p63(uint64_t v, uint64_t q) { return v >= 8 * q ? v - 8 * q : v; }
 
-// Canonical value of an inverse-NTT output in [0, 4q), not scaled (for inputs
-// that carry the N^-1 factor already).
-__device__ __forceinline__ uint64_t ntt_inv_canonical(uint64_t v, uint64_t q) {
-  if (v >= 2 * q) v -= 2 * q;
-  return v >= q ? v - q : v;
-}
+// Canonical value of an inverse-NTT output, not scaled (for inputs that carry
+// the N^-1 factor already).
+__device__ __forceinline__ uint64_t ntt_inv_canonical(uint64_t v, uint64_t q) { return canonical_2p60(v, q); }
 
-// Canonical value of an inverse-NTT output in [0, 4q), scaled by N^-1 (the
-// Shoup product takes any input and lands in [0, 2q)).
+// Canonical value of an inverse-NTT output, scaled by N^-1 (the Shoup product
+// takes any input and lands in [0, 2q)).
 __device__ __forceinline__ uint64_t ntt_inv_final(uint64_t v, const NttTab &T) {
   const uint64_t r = mul_shoup_lazy(v, T.n_inv, T.n_inv_s, T.q);
   return r >= T.q ? r - T.q : r;

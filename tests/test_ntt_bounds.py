@@ -4,13 +4,15 @@ onionpirv2_amd/csrc/ntt.hpp keeps butterfly values unreduced: the twiddle
 product mul_shoup_4q (arith.hpp) truncates the Shoup quotient to
 x1 s1 + hi(x1 s0) + hi(x0 s1), so it lands in [0, 4q); forward stages add up
 to 4q, and before stages 3, 5, 7 and 9 the input half loses 8q when its bit 63
-is set (reduce_2p63: q = 2^60 - c, so 8q = 2^63 - 8c); inverse values stay
-below 4q.  The forward output is brought to [0, q) by ntt_fwd_final (2^60 = c
+is set (reduce_2p63: q = 2^60 - c, so 8q = 2^63 - 8c); the inverse's sum half
+is (s mod 2^62) + (s >> 62) 4c (reduce_2p62: 4q = 2^62 - 4c), below
+2^62 + 12c, and its difference half a twiddle product of u + 8q - v.  The
+forward output is brought to [0, q) by ntt_fwd_final (2^60 = c
 mod q, then one conditional subtraction) or below 2^63 by ntt_fwd_below_2p63.
 This models those steps with Python integers and checks, for both 60-bit
 primes on worst-case and random inputs, that the quotient is at most 2 below
 the exact one, that every value stays below 2^64 and within the stated bounds
-(2^63 + 8q forward, 4q inverse), that the final reductions are canonical or
+(2^63 + 8q forward, 2^62 + 12c inverse), that the final reductions are canonical or
 below 2^63, and that the results agree mod q with exact arithmetic.  A
 reduction step dropped from the schedule or a wrong bound in the butterflies
 fails it.
@@ -40,6 +42,14 @@ def reduce_2p63(v, q):
     assert 0 < c8 < 1 << 32
     r = (v >> 63) * c8 + (v & ((1 << 63) - 1))
     assert r == (v - 8 * q if v >= 1 << 63 else v) and r < 1 << 63
+    return r
+
+
+def reduce_2p62(v, q):
+    c4 = (1 << 62) - 4 * q
+    assert 0 < c4 < 1 << 32 and v <= M64
+    r = (v >> 62) * c4 + (v & ((1 << 62) - 1))
+    assert r % q == v % q and r < (1 << 62) + 3 * c4
     return r
 
 
@@ -80,7 +90,7 @@ def forward(a, q, tab, tabs, small, corrections=(3, 5, 7, 9)):
     return x
 
 
-def inverse(a, q, tab, tabs):
+def inverse(a, q, tab, tabs, reduce=True):
     x = list(a)
     for s in range(11):
         H = 1 << s
@@ -89,11 +99,9 @@ def inverse(a, q, tab, tabs):
                 continue
             m = N // (2 * H) + (i >> (s + 1))
             u, v = x[i], x[i + H]
-            sm = u + v
-            if sm >= 4 * q:
-                sm -= 4 * q
-            x[i], x[i + H] = sm, shoup4(u + 4 * q - v, tab[m], tabs[m], q)
-        assert max(x) < 4 * q
+            assert u + v <= M64 and v < 8 * q and u + 8 * q <= M64
+            x[i], x[i + H] = reduce_2p62(u + v, q) if reduce else u + v, shoup4(u + 8 * q - v, tab[m], tabs[m], q)
+        assert max(x) < (1 << 62) + 12 * ((1 << 60) - q)
     return x
 
 
@@ -133,9 +141,11 @@ def test_lazy_forward_and_inverse_bounds(q):
         assert [v % q for v in out] == ref
         assert [fwd_final(v, q) for v in out] == ref
         assert all(reduce_2p63(v, q) % q == r for v, r in zip(out, ref))
-    for a in ([4 * q - 1] * N, [rng.randrange(4 * q) for _ in range(N)]):
+    for a in ([8 * q - 1] * N, [4 * q - 1] * N, [rng.randrange(8 * q) for _ in range(N)]):
         out = inverse(a, q, tab, tabs)
-        assert [v % q for v in out] == exact_inverse(a, q, tab)
+        ref = exact_inverse(a, q, tab)
+        assert [v % q for v in out] == ref
+        assert [fwd_final(v, q) for v in out] == ref
 
 
 def test_final_reductions_on_extreme_values():
@@ -155,3 +165,13 @@ def test_a_missing_reduction_overflows_the_bound():
     tabs = [(w << 64) // q for w in tab]
     with pytest.raises(AssertionError):
         forward([4 * q - 1] * N, q, tab, tabs, False, corrections=(3, 5, 7))
+
+
+def test_inverse_without_the_sum_reduction_overflows():
+    """Without reduce_2p62 the sum half doubles each stage and passes 8q."""
+    q = PRIMES[1]
+    rng = random.Random(6)
+    tab = [rng.randrange(q) for _ in range(N)]
+    tabs = [(w << 64) // q for w in tab]
+    with pytest.raises(AssertionError):
+        inverse([4 * q - 1] * N, q, tab, tabs, reduce=False)
