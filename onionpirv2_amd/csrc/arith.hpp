@@ -94,20 +94,81 @@ __device__ __forceinline__ void add128(uint64_t &lo, uint64_t &hi, uint64_t blo,
   hi += bhi + (lo < blo);
 }
 
-// (lo, hi) += a * b for a < 2^63 and b < 2^60 (high words below 2^31 and
-// 2^28), as three 32x32 partial products: the middle pair sums below
-// 2^63 + 2^60 without a carry, the low product carries into hi once.
-__device__ __forceinline__ void mac128(uint64_t &lo, uint64_t &hi, uint64_t a, uint64_t b) {
-  const uint32_t al = (uint32_t)a, ah = (uint32_t)(a >> 32), bl = (uint32_t)b, bh = (uint32_t)(b >> 32);
-  const uint64_t ll = (uint64_t)al * bl;
-  const uint64_t mid = (uint64_t)al * bh + (uint64_t)ah * bl;
-  const uint64_t hh = (uint64_t)ah * bh;
-  const uint64_t t = lo + ll;
-  const uint64_t c1 = t < ll;
-  const uint64_t u = t + (mid << 32);
-  const uint64_t c2 = u < t;
-  lo = u;
-  hi += hh + (mid >> 32) + c1 + c2;
+// Unsigned 128-bit sum as four 32-bit words, least significant first.
+struct U128 {
+  uint32_t w[4];
+};
+
+// s[k] += a[k] * b[k] for k < 4, with a[k] < 2^63 and b[k] < 2^60.  With
+// a = a1 2^32 + a0 and b = b1 2^32 + b0 the product is ll + m 2^32 + hh 2^64
+// (ll = a0 b0, hh = a1 b1, m = a0 b1 + a1 b0 < 2^63 + 2^60): one carry chain
+// adds ll + hh 2^64 to the words, a second adds m 2^32.  The four sums' chains
+// are interleaved, each with its own SGPR pair for the carry, so a carry is
+// read three instructions after the VALU write of it (two wait states are
+// required).  Eleven instructions per product; the compiler's lowering of the
+// same sums (64-bit adds, compares and selects for the carries) takes about 18.
+__device__ __forceinline__ void mac128x4(U128 *s, const uint64_t *a, const uint64_t *b) {
+  uint32_t p[4][6];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t a0 = (uint32_t)a[k], a1 = (uint32_t)(a[k] >> 32), b0 = (uint32_t)b[k], b1 = (uint32_t)(b[k] >> 32);
+    const uint64_t ll = (uint64_t)a0 * b0, hh = (uint64_t)a1 * b1;
+    const uint64_t m = (uint64_t)a0 * b1 + (uint64_t)a1 * b0;
+    p[k][0] = (uint32_t)ll;
+    p[k][1] = (uint32_t)(ll >> 32);
+    p[k][2] = (uint32_t)hh;
+    p[k][3] = (uint32_t)(hh >> 32);
+    p[k][4] = (uint32_t)m;
+    p[k][5] = (uint32_t)(m >> 32);
+  }
+  uint64_t c0, c1, c2, c3;
+  asm("v_add_co_u32 %0, %16, %0, %20\n\t"
+      "v_add_co_u32 %4, %17, %4, %26\n\t"
+      "v_add_co_u32 %8, %18, %8, %32\n\t"
+      "v_add_co_u32 %12, %19, %12, %38\n\t"
+      "v_addc_co_u32 %1, %16, %1, %21, %16\n\t"
+      "v_addc_co_u32 %5, %17, %5, %27, %17\n\t"
+      "v_addc_co_u32 %9, %18, %9, %33, %18\n\t"
+      "v_addc_co_u32 %13, %19, %13, %39, %19\n\t"
+      "v_addc_co_u32 %2, %16, %2, %22, %16\n\t"
+      "v_addc_co_u32 %6, %17, %6, %28, %17\n\t"
+      "v_addc_co_u32 %10, %18, %10, %34, %18\n\t"
+      "v_addc_co_u32 %14, %19, %14, %40, %19\n\t"
+      "v_addc_co_u32 %3, %16, %3, %23, %16\n\t"
+      "v_addc_co_u32 %7, %17, %7, %29, %17\n\t"
+      "v_addc_co_u32 %11, %18, %11, %35, %18\n\t"
+      "v_addc_co_u32 %15, %19, %15, %41, %19\n\t"
+      "v_add_co_u32 %1, %16, %1, %24\n\t"
+      "v_add_co_u32 %5, %17, %5, %30\n\t"
+      "v_add_co_u32 %9, %18, %9, %36\n\t"
+      "v_add_co_u32 %13, %19, %13, %42\n\t"
+      "v_addc_co_u32 %2, %16, %2, %25, %16\n\t"
+      "v_addc_co_u32 %6, %17, %6, %31, %17\n\t"
+      "v_addc_co_u32 %10, %18, %10, %37, %18\n\t"
+      "v_addc_co_u32 %14, %19, %14, %43, %19\n\t"
+      "v_addc_co_u32 %3, %16, %3, 0, %16\n\t"
+      "v_addc_co_u32 %7, %17, %7, 0, %17\n\t"
+      "v_addc_co_u32 %11, %18, %11, 0, %18\n\t"
+      "v_addc_co_u32 %15, %19, %15, 0, %19"
+      : "+v"(s[0].w[0]), "+v"(s[0].w[1]), "+v"(s[0].w[2]), "+v"(s[0].w[3]), "+v"(s[1].w[0]), "+v"(s[1].w[1]),
+        "+v"(s[1].w[2]), "+v"(s[1].w[3]), "+v"(s[2].w[0]), "+v"(s[2].w[1]), "+v"(s[2].w[2]), "+v"(s[2].w[3]),
+        "+v"(s[3].w[0]), "+v"(s[3].w[1]), "+v"(s[3].w[2]), "+v"(s[3].w[3]), "=&s"(c0), "=&s"(c1), "=&s"(c2),
+        "=&s"(c3)
+      : "v"(p[0][0]), "v"(p[0][1]), "v"(p[0][2]), "v"(p[0][3]), "v"(p[0][4]), "v"(p[0][5]), "v"(p[1][0]),
+        "v"(p[1][1]), "v"(p[1][2]), "v"(p[1][3]), "v"(p[1][4]), "v"(p[1][5]), "v"(p[2][0]), "v"(p[2][1]),
+        "v"(p[2][2]), "v"(p[2][3]), "v"(p[2][4]), "v"(p[2][5]), "v"(p[3][0]), "v"(p[3][1]), "v"(p[3][2]),
+        "v"(p[3][3]), "v"(p[3][4]), "v"(p[3][5]));
+}
+
+__device__ __forceinline__ void mac128x8(U128 (&s)[8], const uint64_t (&a)[8], const uint64_t (&b)[8]) {
+  __builtin_amdgcn_sched_barrier(0);  // one group's products live at a time
+  mac128x4(s, a, b);
+  __builtin_amdgcn_sched_barrier(0);
+  mac128x4(s + 4, a + 4, b + 4);
+}
+
+__device__ __forceinline__ uint64_t barrett128(const U128 &s, const Mod &m) {
+  return barrett128(((uint64_t)s.w[1] << 32) | s.w[0], ((uint64_t)s.w[3] << 32) | s.w[2], m);
 }
 
 __device__ __forceinline__ uint64_t mulmod_barrett(uint64_t a, uint64_t b, const Mod &m) {

@@ -66,18 +66,15 @@ __global__ __launch_bounds__(kNttThreads) void k_decomp_ntt(const uint64_t *src_
 // 128-bit accumulation of the 2l digit products, blocked slots 8t .. 8t+7
 __device__ __forceinline__ void ext_mac8(const uint64_t *Dc, const uint64_t *G, uint32_t rows, int comp,
                                          const Mod &mq, uint64_t (&out)[8]) {
-  uint64_t lo[8], hi[8];
-#pragma unroll
-  for (int r = 0; r < 8; ++r) lo[r] = hi[r] = 0;
+  U128 acc[8] = {};
   for (uint32_t j = 0; j < rows; ++j) {
     uint64_t d[8], g[8];
     load_blocked(d, Dc + (size_t)j * kN);
     load_blocked(g, G + (size_t)j * 2 * kN + (size_t)comp * kN);
-#pragma unroll
-    for (int r = 0; r < 8; ++r) mac128(lo[r], hi[r], d[r], g[r]);
+    mac128x8(acc, d, g);
   }
 #pragma unroll
-  for (int r = 0; r < 8; ++r) out[r] = barrett128(lo[r], hi[r], mq);
+  for (int r = 0; r < 8; ++r) out[r] = barrett128(acc[r], mq);
 }
 
 // Conversion: out row for ciphertext c, component comp (NTT form):
@@ -121,14 +118,13 @@ __global__ __launch_bounds__(kNttThreads) void k_ext_mac_intt_add(const uint64_t
 
 // 128-bit multiply-accumulate of one transformed digit with RGSW row g (both
 // components; blocked slots 8t .. 8t+7).
-__device__ __forceinline__ void mac_row(uint64_t (&lo)[2][8], uint64_t (&hi)[2][8], const uint64_t (&d)[8],
+__device__ __forceinline__ void mac_row(U128 (&acc)[2][8], const uint64_t (&d)[8],
                                         const uint64_t *g) {
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
     uint64_t gv[8];
     load_blocked(gv, g + (size_t)c * kN);
-#pragma unroll
-    for (int r = 0; r < 8; ++r) mac128(lo[c][r], hi[c][r], d[r], gv[r]);
+    mac128x8(acc[c], d, gv);
   }
 }
 
@@ -149,11 +145,7 @@ __device__ __forceinline__ void ext_fused_core(const uint64_t *xs, const uint64_
                                                uint64_t *top = nullptr) {
   static_assert(FOLD || !NTT_IN, "NTT-form input needs the folded gadget matrix");
   const uint64_t mask = (1ULL << base_log2) - 1;
-  uint64_t lo[2][8], hi[2][8];
-#pragma unroll
-  for (int c = 0; c < 2; ++c)
-#pragma unroll
-    for (int r = 0; r < 8; ++r) lo[c][r] = hi[c][r] = 0;
+  U128 acc[2][8] = {};
 #pragma unroll 1
   for (uint32_t poly = 0; poly < 2; ++poly) {
     const uint64_t *g_last = G + (size_t)(poly * l + l - 1) * 2 * kN;
@@ -181,22 +173,22 @@ __device__ __forceinline__ void ext_fused_core(const uint64_t *xs, const uint64_
       ntt_fwd<true>(d, lds, T);  // digits < 2^base_log2 <= 2^32
 #pragma unroll
       for (int r = 0; r < 8; ++r) d[r] = ntt_fwd_below_2p63(d[r], T.q);
-      mac_row(lo, hi, d, G + (size_t)(poly * l + kk) * 2 * kN);
+      mac_row(acc, d, G + (size_t)(poly * l + kk) * 2 * kN);
     }
     if (NTT_IN) {  // the last row takes NTT(x): the leaf as read
       load_blocked(v, xs + (size_t)poly * kN);
-      mac_row(lo, hi, v, g_last);
+      mac_row(acc, v, g_last);
     } else if (FOLD) {
       ntt_fwd(v, lds, T);
 #pragma unroll
       for (int r = 0; r < 8; ++r) v[r] = ntt_fwd_below_2p63(v[r], T.q);
-      mac_row(lo, hi, v, g_last);
+      mac_row(acc, v, g_last);
     }
   }
 #pragma unroll
   for (int c = 0; c < 2; ++c)
 #pragma unroll
-    for (int r = 0; r < 8; ++r) res[c][r] = barrett128(lo[c][r], hi[c][r], mq);
+    for (int r = 0; r < 8; ++r) res[c][r] = barrett128(acc[c][r], mq);
 }
 
 // Mux fold, fused: x_c += INTT(ext(RGSW, y_c - x_c)) for ciphertext blockIdx.x
@@ -361,11 +353,7 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
         for (int j = 0; j < 8; ++j) v[j] = diff[poly][8 * t + j];
       }
     };
-    uint64_t lo[2][8], hi[2][8];
-#pragma unroll
-    for (int c = 0; c < 2; ++c)
-#pragma unroll
-      for (int j = 0; j < 8; ++j) lo[c][j] = hi[c][j] = 0;
+    U128 acc[2][8] = {};
 #pragma unroll 1
     for (uint32_t poly = 0; poly < 2; ++poly) {
       {
@@ -384,11 +372,11 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
         ntt_fwd_sched<R, true>(d, lds, T);  // digits < 2^base_log2
 #pragma unroll
         for (int j = 0; j < 8; ++j) d[j] = ntt_fwd_below_2p63(d[j], q);
-        mac_row(lo, hi, d, Gb + (size_t)(poly * l + kk) * 2 * kN);
+        mac_row(acc, d, Gb + (size_t)(poly * l + kk) * 2 * kN);
       }
       uint64_t d[8];
       y_minus_x(poly, d);
-      mac_row(lo, hi, d, Gb + (size_t)(poly * l + l - 1) * 2 * kN);
+      mac_row(acc, d, Gb + (size_t)(poly * l + l - 1) * 2 * kN);
     }
     // X + E into cts row r.  The fence keeps these loads after the loop:
     // hoisted, they would stay live through it.
@@ -399,7 +387,7 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) w[c][j] = add_mod(w[c][j], barrett128(lo[c][j], hi[c][j], mq), q);
+      for (int j = 0; j < 8; ++j) w[c][j] = add_mod(w[c][j], barrett128(acc[c][j], mq), q);
       store_blocked(x + (size_t)c * kN, w[c]);
     }
   }
