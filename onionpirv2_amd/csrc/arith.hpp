@@ -171,6 +171,34 @@ __device__ __forceinline__ uint64_t barrett128(const U128 &s, const Mod &m) {
   return barrett128(((uint64_t)s.w[1] << 32) | s.w[0], ((uint64_t)s.w[3] << 32) | s.w[2], m);
 }
 
+// Canonical residue of any 128-bit value w3:w2:w1:w0 modulo q = 2^60 - c with
+// c < 2^20 (the same result as barrett128, since both are exact).  With
+// 2^60 = c mod q and H = floor(V / 2^60) = h0 + 2^32 h1 + 2^64 h2:
+//   V = L + c h0 + 2^32 c h1 + 2^64 c h2 (mod q),  L = V mod 2^60;
+//   A = L + c h0 < 2^61;
+//   B = c h1 + floor(A / 2^32) + 2^32 c h2 < 2^52 + 2^29 + 2^56 (h2 < 16);
+//   T = 2^32 B + (A mod 2^32) = 2^60 T1 + T0 with T1 = floor(B / 2^28) < 2^29;
+//   R = T0 + c T1 < 2^60 + 2^49 < 2q, and one conditional subtraction.
+// About 15 instructions where the Barrett quotient takes about 50.
+__device__ __forceinline__ uint64_t reduce128(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, uint64_t q) {
+  const uint32_t c = (uint32_t)((1ULL << 60) - q);
+  const uint32_t h0 = __builtin_amdgcn_alignbit(w2, w1, 28);
+  const uint32_t h1 = __builtin_amdgcn_alignbit(w3, w2, 28);
+  const uint32_t h2 = w3 >> 28;
+  const uint64_t A = (uint64_t)h0 * c + (((uint64_t)(w1 & 0x0FFFFFFFu) << 32) | w0);
+  const uint64_t B = (uint64_t)h1 * c + (A >> 32) + ((uint64_t)(h2 * c) << 32);
+  const uint64_t T0 = ((B & 0x0FFFFFFFu) << 32) | (uint32_t)A;
+  const uint32_t T1 = __builtin_amdgcn_alignbit((uint32_t)(B >> 32), (uint32_t)B, 28);
+  const uint64_t R = (uint64_t)T1 * c + T0;
+  return R >= q ? R - q : R;
+}
+__device__ __forceinline__ uint64_t reduce128(uint64_t lo, uint64_t hi, const Mod &m) {
+  return reduce128((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32), m.q);
+}
+__device__ __forceinline__ uint64_t reduce128(const U128 &s, const Mod &m) {
+  return reduce128(s.w[0], s.w[1], s.w[2], s.w[3], m.q);
+}
+
 __device__ __forceinline__ uint64_t mulmod_barrett(uint64_t a, uint64_t b, const Mod &m) {
   uint64_t lo, hi;
   mul128(a, b, lo, hi);

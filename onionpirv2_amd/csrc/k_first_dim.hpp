@@ -5,7 +5,8 @@
 //
 // For every NTT slot l: C_l[r][c] = sum_j A_l[r][j] * B_l[j][c], exact and
 // wrapping mod 2^128 like the reference's uint128 accumulator, then
-// (C mod 2^128) mod q with the same Barrett reduction.
+// the canonical residue of C mod 2^128 (reduce128, exact like the
+// reference's barrett_reduce_128).
 //
 // Exact 64x64 products from 30-bit limbs: a = a1*2^30 + a0, b = b1*2^30 + b0
 // (all limbs < 2^30), so every 32x32 partial product is < 2^60 and a plain
@@ -181,7 +182,7 @@ __global__ __launch_bounds__(256) void k_first_dim_b1(const uint64_t *__restrict
       add128(w1lo, w1hi, c, d);
     }
     if (active && cg < 2) {
-      const uint64_t res = cg == 0 ? barrett128(w0lo, w0hi, mq) : barrett128(w1lo, w1hi, mq);
+      const uint64_t res = cg == 0 ? reduce128(w0lo, w0hi, mq) : reduce128(w1lo, w1hi, mq);
       out[((size_t)r * 2 + cg) * kN + slot] = res;
     }
   }
@@ -211,8 +212,8 @@ __global__ __launch_bounds__(256) void k_first_dim_generic(const uint64_t *__res
     }
     fold(w0lo, w0hi, x0);
     fold(w1lo, w1hi, x1);
-    out[((size_t)r * 2 + 0) * kN + slot] = barrett128(w0lo, w0hi, mq);
-    out[((size_t)r * 2 + 1) * kN + slot] = barrett128(w1lo, w1hi, mq);
+    out[((size_t)r * 2 + 0) * kN + slot] = reduce128(w0lo, w0hi, mq);
+    out[((size_t)r * 2 + 1) * kN + slot] = reduce128(w1lo, w1hi, mq);
   }
 }
 
@@ -296,7 +297,7 @@ __global__ __launch_bounds__(256) void k_first_dim_bat(const uint64_t *__restric
     for (int k = 0; k < TC; ++k) {
       const uint32_t col = cg + k * CG, b = col >> 1, c = col & 1;
       if (r < other && b < nq)
-        fd[(size_t)b * fd_stride + ((size_t)r * 2 + c) * kN + slot] = barrett128(wlo[i][k], whi[i][k], mq);
+        fd[(size_t)b * fd_stride + ((size_t)r * 2 + c) * kN + slot] = reduce128(wlo[i][k], whi[i][k], mq);
     }
   }
 }

@@ -16,8 +16,8 @@
 // it is exact in int32 for fst <= 2^14.  One v_mfma_i32_32x32x32_i8 adds a
 // 32x32 tile of dA_i * dB_k over 32 values of j; a wave issues the 64 (i, k)
 // digit products of each 32-wide step of j into 15 diagonal accumulators.
-// The epilogue forms C mod 2^128 from the 15 signed sums and applies the same
-// Barrett reduction as the 64-bit path, so results are bit-identical.
+// The epilogue forms C mod 2^128 from the 15 signed sums and reduces it mod q
+// exactly as the 64-bit path does, so results are bit-identical.
 //
 // Operand layout.  MFMA lane l holds 16 bytes of row (l % 32) of the A tile
 // and 16 bytes of column (l % 32) of the B tile, both in the same k order;
@@ -270,7 +270,7 @@ __device__ __forceinline__ uint64_t diagonals_mod_q(const v16i_t (&acc)[15], int
   uint64_t lo = (uint64_t)P[0], hi = (uint64_t)(P[0] >> 63);
   add128(lo, hi, (uint64_t)P[1] << 32, (uint64_t)(P[1] >> 32));
   hi += (uint64_t)P[2] + ((uint64_t)P[3] << 32);
-  return barrett128(lo, hi, mq);
+  return reduce128(lo, hi, mq);
 }
 
 // Common tile geometry.  CB = column blocks of 32 (NCP = 32 CB padded
@@ -426,7 +426,7 @@ __global__ __launch_bounds__(256) void k_first_dim_mfma_pre(const uint8_t *__res
 // while the previous tile ends.  At a tile's end each wave forms C mod 2^128
 // of its 16 outputs from the accumulators (the serial part) and parks them in
 // LDS; during the next tile's steps 0..15 it reduces one parked output per
-// step (Barrett, the store), VALU work that issues between the matrix
+// step (reduce128, the store), VALU work that issues between the matrix
 // products.  Outputs outside the matrix are dropped by the buffer store's
 // range check (offset 2^31 against a 2^31-byte range, or a range of 0 while
 // nothing is parked), so every step is branch-free.  Requires fst % 64 == 0
@@ -516,7 +516,7 @@ __global__ __launch_bounds__(256) void k_first_dim_mfma_tiles(const uint8_t *__r
   const bool col_ok = cb * 32 + li < ncols;
   auto put = [&](const OutTile &prev, uint32_t e, bool live) {
     const uint4 c = st[e * 64];
-    const uint64_t v = barrett128((uint64_t)c.x | (uint64_t)c.y << 32, (uint64_t)c.z | (uint64_t)c.w << 32, mq);
+    const uint64_t v = reduce128(c.x, c.y, c.z, c.w, mq.q);
     const uint32_t urow = prev.row0 + 8 * (e >> 2) + (e & 3);
     uint64_t *base = out2 + ((size_t)urow * kN + prev.slot) * ncols + cb * 32;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(base, 0, live ? 0x80000000u : 0u, 0x00020000);

@@ -10,7 +10,7 @@
 //                    polynomial, NTT it, store it (NTT form, below 2^63:
 //                    the 128-bit sums reduce it exactly);
 //   k_ext_mac_*   -- one workgroup per (ciphertext, output component):
-//                    sum_j D_j * G_j[comp] in 128-bit, Barrett mod q, then
+//                    sum_j D_j * G_j[comp] in 128-bit, reduced mod q, then
 //                    either store (conversion, NTT form) or INTT and add into
 //                    the mux accumulator (other-dimension fold).
 // Fused variants run one workgroup per ciphertext (k_mux_fused,
@@ -74,7 +74,7 @@ __device__ __forceinline__ void ext_mac8(const uint64_t *Dc, const uint64_t *G, 
     mac128x8(acc, d, g);
   }
 #pragma unroll
-  for (int r = 0; r < 8; ++r) out[r] = barrett128(acc[r], mq);
+  for (int r = 0; r < 8; ++r) out[r] = reduce128(acc[r], mq);
 }
 
 // Conversion: out row for ciphertext c, component comp (NTT form):
@@ -188,7 +188,7 @@ __device__ __forceinline__ void ext_fused_core(const uint64_t *xs, const uint64_
 #pragma unroll
   for (int c = 0; c < 2; ++c)
 #pragma unroll
-    for (int r = 0; r < 8; ++r) res[c][r] = barrett128(acc[c][r], mq);
+    for (int r = 0; r < 8; ++r) res[c][r] = reduce128(acc[c][r], mq);
 }
 
 // Mux fold, fused: x_c += INTT(ext(RGSW, y_c - x_c)) for ciphertext blockIdx.x
@@ -366,13 +366,21 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
 #pragma unroll 1
       for (uint32_t kk = 0; kk + 1 < l; ++kk) {
         const uint32_t sh = (l - 1 - kk) * base_log2;
+        const uint64_t *g = Gb + (size_t)(poly * l + kk) * 2 * kN;
+        // the row's first component is loaded before the transform, so its
+        // latency hides behind the butterflies rather than before the products
+        uint64_t g0[8];
+        load_blocked(g0, g);
         uint64_t d[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) d[j] = (coef[t + 256 * j] >> sh) & mask;
         ntt_fwd_sched<R, true>(d, lds, T);  // digits < 2^base_log2
 #pragma unroll
         for (int j = 0; j < 8; ++j) d[j] = ntt_fwd_below_2p63(d[j], q);
-        mac_row(acc, d, Gb + (size_t)(poly * l + kk) * 2 * kN);
+        uint64_t g1[8];
+        load_blocked(g1, g + kN);
+        mac128x8(acc[0], d, g0);
+        mac128x8(acc[1], d, g1);
       }
       uint64_t d[8];
       y_minus_x(poly, d);
@@ -387,7 +395,7 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) w[c][j] = add_mod(w[c][j], barrett128(acc[c][j], mq), q);
+      for (int j = 0; j < 8; ++j) w[c][j] = add_mod(w[c][j], reduce128(acc[c][j], mq), q);
       store_blocked(x + (size_t)c * kN, w[c]);
     }
   }

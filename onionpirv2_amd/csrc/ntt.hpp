@@ -429,8 +429,8 @@ __device__ __forceinline__ uint64_t ntt_fwd_final(uint64_t v, uint64_t q) { retu
 
 // A forward-NTT output (below 2^63 + 8q) brought below 2^63 by one
 // conditional subtraction of 8q, enough for the 128-bit multiply-accumulate of
-// external products (mac128x4 with a canonical second operand), whose Barrett
-// reduction at the end returns the canonical residue.  (reduce_2p63 does the
+// external products (mac128x4 with a canonical second operand), whose 128-bit
+// reduction at the end (reduce128) returns the canonical residue.  (reduce_2p63 does the
 // same in fewer instructions, but there the compiler spills k_convert_fused.)
 __device__ __forceinline__ uint64_t ntt_fwd_below_2p63(uint64_t v, uint64_t q) { return v >= 8 * q ? v - 8 * q : v; }
 

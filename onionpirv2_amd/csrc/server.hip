@@ -1133,10 +1133,11 @@ int onion_gpu_create(const onion_params *params, onion_server **out) {
   if (derive_params(P, info, err, kN) != ONION_OK) return fail(ONION_EINVAL, err);
   if (P.fst_dim > (uint32_t)kMaxFst) return fail(ONION_EINVAL, "fst_dim above 1024");
   // the transforms' reductions take moduli 2^60 - c with c < 2^28 (reduce_2p63,
-  // ntt_fwd_final); the N = 2048 primes are 2^60 - 98303 and 2^60 - 16383
+  // ntt_fwd_final) and the 128-bit sums' reduction c < 2^20 (reduce128); the
+  // N = 2048 primes are 2^60 - 98303 and 2^60 - 16383
   for (uint64_t m : {info.q, info.p})
-    if (m >= (1ULL << 60) || (1ULL << 60) - m >= (1ULL << 28))
-      return fail(ONION_EINVAL, "moduli must be 2^60 - c with c < 2^28");
+    if (m >= (1ULL << 60) || (1ULL << 60) - m >= (1ULL << 20))
+      return fail(ONION_EINVAL, "moduli must be 2^60 - c with c < 2^20");
 
   onion_server *s = new onion_server();
   s->P = P;

@@ -15,7 +15,8 @@ the exact one, that every value stays below 2^64 and within the stated bounds
 (2^63 + 8q forward, 2^62 + 12c inverse), that the final reductions are canonical or
 below 2^63, and that the results agree mod q with exact arithmetic.  A
 reduction step dropped from the schedule or a wrong bound in the butterflies
-fails it.
+fails it.  The same holds for reduce128 (arith.hpp), which takes the 128-bit
+multiply-accumulate sums to their canonical residue through 2^60 = c mod q.
 """
 import random
 
@@ -175,3 +176,48 @@ def test_inverse_without_the_sum_reduction_overflows():
     tabs = [(w << 64) // q for w in tab]
     with pytest.raises(AssertionError):
         inverse([4 * q - 1] * N, q, tab, tabs, reduce=False)
+
+
+def reduce128(v, q, drop_h2=False):
+    """Word-level model of reduce128 (arith.hpp): the canonical residue of a
+    128-bit sum mod q = 2^60 - c, c < 2^20, through 2^60 = c mod q."""
+    assert 0 <= v < 1 << 128
+    c = (1 << 60) - q
+    assert 0 < c < 1 << 20
+    w0, w1, w2, w3 = [(v >> (32 * i)) & 0xFFFFFFFF for i in range(4)]
+    h0 = ((w2 << 32 | w1) >> 28) & 0xFFFFFFFF  # v_alignbit_b32
+    h1 = ((w3 << 32 | w2) >> 28) & 0xFFFFFFFF
+    h2 = 0 if drop_h2 else w3 >> 28
+    a = h0 * c + ((w1 & 0x0FFFFFFF) << 32 | w0)
+    assert a < 1 << 61
+    b = h1 * c + (a >> 32) + ((h2 * c) << 32)
+    assert h2 * c < 1 << 32 and b < 1 << 57
+    t0 = (b & 0x0FFFFFFF) << 32 | (a & 0xFFFFFFFF)
+    t1 = (b >> 28) & 0xFFFFFFFF
+    assert t1 == b >> 28
+    r = t1 * c + t0
+    assert r < 2 * q and r <= M64
+    return r - q if r >= q else r
+
+
+def test_reduce128_is_the_canonical_residue():
+    """The 128-bit multiply-accumulate sums (external products, the first
+    dimension's C mod 2^128) are reduced through 2^60 = c mod q: every value
+    below 2^128 gives v mod q, for both primes and for the largest c allowed."""
+    rng = random.Random(11)
+    moduli = list(PRIMES) + [(1 << 60) - ((1 << 20) - 1), (1 << 60) - 1]
+    for q in moduli:
+        edges = [0, 1, q - 1, q, 2 * q - 1, 2 * q, (1 << 60) - 1, 1 << 60, (1 << 64) - 1, 1 << 64,
+                 (1 << 124) - 1, 1 << 124, (1 << 127) - 1, 1 << 127, (1 << 128) - 1,
+                 ((1 << 128) - 1) // q * q, ((1 << 128) - 1) // q * q - 1, 20 * (8 * q - 1) * (q - 1)]
+        vals = edges + [rng.getrandbits(128) for _ in range(4000)] + \
+            [rng.getrandbits(rng.randrange(1, 129)) for _ in range(4000)]
+        for v in vals:
+            assert reduce128(v, q) == v % q, (q, v)
+
+
+def test_reduce128_without_the_top_bits_fails():
+    """Dropping h2 (bits 124..127) gives wrong residues for large sums."""
+    q = PRIMES[0]
+    bad = [v for v in (1 << 127, (1 << 128) - 1, 3 << 125) if reduce128(v, q, drop_h2=True) != v % q]
+    assert bad
