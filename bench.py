@@ -117,6 +117,9 @@ def main() -> int:
                     help="skip the batch-64 line at t = 65537 (PlainMod 17)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (profiler byte-counter scale)")
+    ap.add_argument("--valu-probe", type=int, default=0, metavar="ROUNDS",
+                    help="also run the VALU issue-rate probe (ROUNDS radix-8 butterfly passes per thread) "
+                         "at 1..4 waves per SIMD: the ceiling the transform-bound kernels' counters are read against")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -294,6 +297,12 @@ def main() -> int:
 
     out["single_query_graph"] = graph_single
     out["single_query_eager"] = eager_single
+    if args.valu_probe:
+        # blocks = 256 w: w workgroups per compute unit, one wave per SIMD each
+        out["valu_probe"] = [{"waves_per_simd": w, "blocks": 256 * w, "rounds": args.valu_probe,
+                              "ms": srv.tap_valu_probe(256 * w, args.valu_probe),
+                              "butterflies": 256 * w * 256 * 12 * args.valu_probe} for w in (1, 2, 3, 4)]
+        log(f"[bench] VALU probe ms by waves per SIMD: {[round(p['ms'], 3) for p in out['valu_probe']]}")
     ok_all = bit_exact and all(leg["bit_exact"] for leg in (graph_single, eager_single) if leg)
 
     # ---- two lanes: two batches in flight on two streams --------------------------------

@@ -277,6 +277,13 @@ int onion_gpu_tap_mod_switch(onion_server *srv, const uint64_t *ct, uint64_t *ou
  * no reference counterpart: calibrates profiler byte counters.  *copied gets
  * the size actually copied (clamped to the scratch size). */
 int onion_gpu_tap_copy(onion_server *srv, uint64_t n_bytes, uint64_t *copied);
+/* VALU issue-rate probe: `blocks` workgroups of 256 threads run `rounds`
+ * radix-8 passes of the transforms' butterflies on registers (no memory
+ * traffic, LDS or barriers); *ms gets the kernel time.  No reference
+ * counterpart: it measures the instruction-rate ceiling that the
+ * transform-bound kernels are compared against (blocks = 256 w gives w
+ * waves per SIMD). */
+int onion_gpu_tap_valu_probe(onion_server *srv, uint32_t blocks, uint32_t rounds, double *ms);
 
 #ifdef __cplusplus
 }

@@ -99,6 +99,7 @@ _SIGS = {
     "onion_gpu_tap_galois": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, U64P, U64P]),
     "onion_gpu_tap_mod_switch": (C.c_int, [C.c_void_p, U64P, U64P]),
     "onion_gpu_tap_copy": (C.c_int, [C.c_void_p, C.c_uint64, U64P]),
+    "onion_gpu_tap_valu_probe": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_double)]),
 }
 
 

@@ -159,6 +159,18 @@ __device__ __forceinline__ int pass_base(int t, int s0) {
   return (t & ((1 << s0) - 1)) | ((t >> s0) << (s0 + 3));
 }
 
+// Synchronisation of an exchange that stays inside one wave: thread t's
+// coefficients in passes B, C and D all lie in [512 w, 512 w + 512) for its
+// wave w = t / 64, so in the B <-> C and C <-> D exchanges every value a wave
+// reads was written by the same wave, and a wave's LDS operations complete in
+// issue order.  The fences only stop the compiler from moving LDS accesses
+// across the exchange; no wave waits for another.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ void lds_put(uint64_t *lds, const uint64_t (&x)[8], int b, int stride) {
 #pragma unroll
   for (int j = 0; j < 8; ++j) lds[lds_pad(b + j * stride)] = x[j];
@@ -174,7 +186,10 @@ __device__ __forceinline__ void lds_get(const uint64_t *lds, uint64_t (&x)[8], i
 // R+4q R+8q, the reductions being in stages 3 and 5 (pass B, spans 128 and
 // 32), stage 7 (pass C, span 8) and stage 9 (pass D, span 2).  Uses
 // lds[kNttLds]; begins with a barrier so the buffer may have been read just
-// before the call.
+// before the call.  Only the A -> B exchange crosses waves (two workgroup
+// barriers); B -> C and C -> D synchronise per wave.  On return other waves
+// may still be reading their part of the buffer: the next writer of it
+// starts with a barrier (every transform does).
 template <bool SMALL = false>
 __device__ __forceinline__ void ntt_fwd(uint64_t (&x)[8], uint64_t *lds, const NttTab &T) {
   const int t = threadIdx.x;
@@ -194,22 +209,23 @@ __device__ __forceinline__ void ntt_fwd(uint64_t (&x)[8], uint64_t *lds, const N
   // pass C: spans 16, 8, 4 (stride 4): m = 64 + t/4
   const int bC = pass_base(t, 2);
   load_tw8(tw, T.rp, T.rps, 64 + (t >> 2));
-  __syncthreads();
+  wave_sync();
   lds_put(lds, x, bB, 32);
-  __syncthreads();
+  wave_sync();
   lds_get(lds, x, bC, 4);
   fwd8<2>(x, tw, q, true);
   // pass D: spans 2, 1 on slots 8t .. 8t+7: m = 256 + t
   load_tw8(tw, T.rp, T.rps, 256 + t);
-  __syncthreads();
+  wave_sync();
   lds_put(lds, x, bC, 4);
-  __syncthreads();
+  wave_sync();
   lds_get(lds, x, 8 * t, 1);
   fwd8<2>(x, tw, q, false);
 }
 
 // Inverse NTT without the N^-1 scale.  Entry: blocked layout, values < 8q.
 // Exit: natural layout, values below 2^62 + 12c.  Begins with a barrier.
+// D -> C and C -> B synchronise per wave; B -> A crosses waves (barrier).
 __device__ __forceinline__ void ntt_inv(uint64_t (&x)[8], uint64_t *lds, const NttTab &T) {
   const int t = threadIdx.x;
   const uint64_t q = T.q;
@@ -222,20 +238,20 @@ __device__ __forceinline__ void ntt_inv(uint64_t (&x)[8], uint64_t *lds, const N
   load_tw8(tw, T.irp, T.irps, 64 + (t >> 2));
   __syncthreads();
   lds_put(lds, x, 8 * t, 1);
-  __syncthreads();
+  wave_sync();
   lds_get(lds, x, bC, 4);
   inv8(x, tw, q, true);
   // pass B: spans 32, 64, 128 (stride 32)
   const int bB = pass_base(t, 5);
   load_tw8(tw, T.irp, T.irps, 8 + (t >> 5));
-  __syncthreads();
+  wave_sync();
   lds_put(lds, x, bC, 4);
-  __syncthreads();
+  wave_sync();
   lds_get(lds, x, bB, 32);
   inv8(x, tw, q, true);
   // pass A: spans 256, 512, 1024 (stride 256)
   load_tw8(tw, T.irp, T.irps, 1);
-  __syncthreads();
+  wave_sync();
   lds_put(lds, x, bB, 32);
   __syncthreads();
   lds_get(lds, x, t, 256);

@@ -1754,6 +1754,25 @@ int onion_gpu_tap_copy(onion_server *s, uint64_t n_bytes, uint64_t *copied) {
   return ONION_OK;
 }
 
+int onion_gpu_tap_valu_probe(onion_server *s, uint32_t blocks, uint32_t rounds, double *ms) {
+  if (!s || !blocks || !rounds) return fail(ONION_EINVAL, "null server or empty probe");
+  Shared g(s->gate);
+  Lane *L = s->lanes[0].get();
+  std::lock_guard<std::mutex> lk(L->mu);
+  int rc;
+  if ((rc = ensure_scratch(s, L, 1))) return rc;
+  if ((uint64_t)blocks * kNttThreads > L->cap * s->digit_words) return fail(ONION_EINVAL, "too many probe blocks");
+  HIPCHK(hipEventRecord(L->ev[6], L->st));
+  k_valu_probe<<<blocks, kNttThreads, 0, L->st>>>(s->Tq, rounds, L->digits);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(L->ev[7], L->st));
+  HIPCHK(hipStreamSynchronize(L->st));
+  float f = 0;
+  HIPCHK(hipEventElapsedTime(&f, L->ev[6], L->ev[7]));
+  if (ms) *ms = f;
+  return ONION_OK;
+}
+
 int onion_gpu_tap_mod_switch(onion_server *s, const uint64_t *ct, uint64_t *out) {
   if (!s || !ct || !out) return fail(ONION_EINVAL, "null argument");
   Shared g(s->gate);

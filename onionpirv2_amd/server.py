@@ -225,6 +225,12 @@ class GpuServer:
         check(self._L.onion_gpu_tap_copy(self._h, n_bytes, C.byref(out)))
         return out.value
 
+    def tap_valu_probe(self, blocks: int, rounds: int) -> float:
+        """Kernel time (ms) of the VALU issue-rate probe (blocks x 256 threads)."""
+        ms = C.c_double()
+        check(self._L.onion_gpu_tap_valu_probe(self._h, blocks, rounds, C.byref(ms)))
+        return ms.value
+
     def tap_mod_switch(self, ct) -> np.ndarray:
         c = _u64(ct).reshape(-1)
         out = np.zeros(2 * self.N, np.uint64)
