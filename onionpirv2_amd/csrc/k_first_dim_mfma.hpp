@@ -426,7 +426,7 @@ __global__ __launch_bounds__(256) void k_first_dim_mfma_pre(const uint8_t *__res
 // while the previous tile ends.  At a tile's end each wave forms C mod 2^128
 // of its 16 outputs from the accumulators (the serial part) and parks them in
 // LDS; during the next tile's steps 0..15 it reduces one parked output per
-// step (reduce128, the store), VALU work that issues between the matrix
+// step (Barrett, the store), VALU work that issues between the matrix
 // products.  Outputs outside the matrix are dropped by the buffer store's
 // range check (offset 2^31 against a 2^31-byte range, or a range of 0 while
 // nothing is parked), so every step is branch-free.  Requires fst % 64 == 0
@@ -516,7 +516,11 @@ __global__ __launch_bounds__(256) void k_first_dim_mfma_tiles(const uint8_t *__r
   const bool col_ok = cb * 32 + li < ncols;
   auto put = [&](const OutTile &prev, uint32_t e, bool live) {
     const uint4 c = st[e * 64];
-    const uint64_t v = reduce128(c.x, c.y, c.z, c.w, mq.q);
+    // the Barrett form here, not reduce128: with the shorter reduction the
+    // stores issue earlier in each step and the kernel read 1.4x the HBM bytes
+    // (FETCH_SIZE 6.8e6 against 4.8e6 KiB per batch of 64 at 512x1024) in the
+    // same time; the matrix cores set the pace either way
+    const uint64_t v = barrett128((uint64_t)c.x | (uint64_t)c.y << 32, (uint64_t)c.z | (uint64_t)c.w << 32, mq);
     const uint32_t urow = prev.row0 + 8 * (e >> 2) + (e & 3);
     uint64_t *base = out2 + ((size_t)urow * kN + prev.slot) * ncols + cb * 32;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(base, 0, live ? 0x80000000u : 0u, 0x00020000);
