@@ -262,9 +262,10 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
                                                                const uint64_t *G, uint64_t g_stride, uint32_t l,
                                                                uint32_t base_log2, NttTab T, Mod mq) {
   static_assert(Q == 1 || (FIRST && Q == 2), "query pairs only at the first level");
-  __shared__ uint64_t lds[kNttLds];
-  __shared__ uint64_t coef[kN];     // y - x of the current polynomial; slots t + 256 j of thread t
-  __shared__ uint64_t diff[2][kN];  // Y - X of both polynomials (of query b)
+  // radix 8: digit transforms run two at a time (ntt_fwd2), each with half
+  // of lds; the inverse transforms use the first half
+  __shared__ uint64_t lds[(R == 8 ? 2 : 1) * kNttLds];
+  __shared__ uint64_t coef[kN];  // y - x of the current polynomial; slots t + 256 j of thread t
   const int t = threadIdx.x;
   const uint32_t nqw = nq / Q;  // workgroups per row pair
   uint32_t r, b;                // row pair (r, r + half); first query b
@@ -281,16 +282,16 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
   }
   const uint64_t *ox = out2 + (size_t)r * kN * ncols + 2 * b;  // FIRST: X of query b at ox[slot * ncols + poly]
   const uint64_t *oy = ox + (size_t)half * kN * ncols;
-  // Q = 2: Y - X of query b + 1, parked in its row r + half, which no level
+  // FIRST: Y - X of query b + i, parked in its row r + half, which no level
   // after the first reads (the levels fold rows below half)
-  uint64_t *dpark = cts + (size_t)(b + Q - 1) * cts_stride + (size_t)(r + half) * 2 * kN;
+  auto dpark = [&](uint32_t i) { return cts + (size_t)(b + i) * cts_stride + (size_t)(r + half) * 2 * kN; };
   const uint64_t q = T.q, mask = (1ULL << base_log2) - 1;
-  // Y - X (canonical NTT form) into diff; FIRST also parks X in its
+  // FIRST: Y - X (canonical NTT form) into the parks, and X into its
   // destination row for the final sum, instead of reading out2 at the slot
   // stride again
   if (FIRST && Q == 1) {
     uint64_t *x = cts + (size_t)b * cts_stride + (size_t)r * 2 * kN;
-    uint64_t xs[2][8];
+    uint64_t xs[2][8], ds[2][8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const size_t o = (size_t)(8 * t + j) * ncols;
@@ -298,16 +299,19 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
       const uint4 w = *reinterpret_cast<const uint4 *>(oy + o);
       xs[0][j] = ((uint64_t)u.y << 32) | u.x;
       xs[1][j] = ((uint64_t)u.w << 32) | u.z;
-      diff[0][8 * t + j] = sub_mod(((uint64_t)w.y << 32) | w.x, xs[0][j], q);
-      diff[1][8 * t + j] = sub_mod(((uint64_t)w.w << 32) | w.z, xs[1][j], q);
+      ds[0][j] = sub_mod(((uint64_t)w.y << 32) | w.x, xs[0][j], q);
+      ds[1][j] = sub_mod(((uint64_t)w.w << 32) | w.z, xs[1][j], q);
     }
     store_blocked(x, xs[0]);
     store_blocked(x + kN, xs[1]);
+    store_blocked(dpark(0), ds[0]);  // read back below by this thread only
+    store_blocked(dpark(0) + kN, ds[1]);
   } else if (FIRST) {
     // lanes 2i and 2i + 1 read queries b and b + 1 of one slot: the 32
     // contiguous bytes of each row, so a wave load touches 32 lines, not 64
     const uint32_t h = t & 1;
     uint64_t *xh = cts + (size_t)(b + h) * cts_stride + (size_t)r * 2 * kN;
+    uint64_t *dp = dpark(h);
 #pragma unroll 4
     for (int i = 0; i < kN / 128; ++i) {
       const uint32_t sl = 128 * i + (t >> 1);
@@ -315,30 +319,12 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
       const uint4 u = *reinterpret_cast<const uint4 *>(ox + o);
       const uint4 w = *reinterpret_cast<const uint4 *>(oy + o);
       const uint64_t x0 = ((uint64_t)u.y << 32) | u.x, x1 = ((uint64_t)u.w << 32) | u.z;
-      const uint64_t d0 = sub_mod(((uint64_t)w.y << 32) | w.x, x0, q);
-      const uint64_t d1 = sub_mod(((uint64_t)w.w << 32) | w.z, x1, q);
       xh[sl] = x0;
       xh[kN + sl] = x1;
-      if (h == 0) {
-        diff[0][sl] = d0;
-        diff[1][sl] = d1;
-      } else {
-        dpark[sl] = d0;
-        dpark[kN + sl] = d1;
-      }
+      dp[sl] = sub_mod(((uint64_t)w.y << 32) | w.x, x0, q);
+      dp[kN + sl] = sub_mod(((uint64_t)w.w << 32) | w.z, x1, q);
     }
-    __syncthreads();  // parks and diff are read below in other threads' slot order
-  } else {
-    const uint64_t *x = cts + (size_t)b * cts_stride + (size_t)r * 2 * kN;
-    const uint64_t *y = x + half * 2 * kN;
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      uint64_t u[8], w[8];
-      load_blocked(u, x + (size_t)c * kN);
-      load_blocked(w, y + (size_t)c * kN);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) diff[c][8 * t + j] = sub_mod(w[j], u[j], q);
-    }
+    __syncthreads();  // the parks are read below in other threads' slot order
   }
 #pragma unroll 1
   for (int qi = 0; qi < Q; ++qi) {
@@ -346,11 +332,14 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
     const uint64_t *Gb = G + (size_t)(b + qi) * g_stride;
     // this thread's slots 8t .. 8t+7 of Y - X
     auto y_minus_x = [&](uint32_t poly, uint64_t(&v)[8]) {
-      if (Q == 2 && qi == 1) {
-        load_blocked(v, dpark + (size_t)poly * kN);
+      if (FIRST) {
+        load_blocked(v, dpark(qi) + (size_t)poly * kN);
       } else {
+        uint64_t u[8];
+        load_blocked(u, x + (size_t)poly * kN);
+        load_blocked(v, x + half * 2 * kN + (size_t)poly * kN);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = diff[poly][8 * t + j];
+        for (int j = 0; j < 8; ++j) v[j] = sub_mod(v[j], u[j], q);
       }
     };
     U128 acc[2][8] = {};
@@ -363,28 +352,43 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
 #pragma unroll
         for (int j = 0; j < 8; ++j) coef[t + 256 * j] = ntt_inv_final(v[j], T);
       }
+      const uint64_t *grow = Gb + (size_t)(poly * l) * 2 * kN;  // row kk at grow + kk * 2N
+      uint32_t kk = 0;
+      if (R == 8) {
 #pragma unroll 1
-      for (uint32_t kk = 0; kk + 1 < l; ++kk) {
+        for (; kk + 2 < l; kk += 2) {  // digits kk, kk + 1 (of 0 .. l - 2) together
+          const uint32_t sh = (l - 1 - kk) * base_log2;
+          uint64_t d[8], e[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const uint64_t c = coef[t + 256 * j];
+            d[j] = (c >> sh) & mask;
+            e[j] = (c >> (sh - base_log2)) & mask;
+          }
+          ntt_fwd2<true>(d, e, lds, T);  // digits < 2^base_log2
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            d[j] = ntt_fwd_below_2p63(d[j], q);
+            e[j] = ntt_fwd_below_2p63(e[j], q);
+          }
+          mac_row(acc, d, grow + (size_t)kk * 2 * kN);
+          mac_row(acc, e, grow + (size_t)(kk + 1) * 2 * kN);
+        }
+      }
+#pragma unroll 1
+      for (; kk + 1 < l; ++kk) {
         const uint32_t sh = (l - 1 - kk) * base_log2;
-        const uint64_t *g = Gb + (size_t)(poly * l + kk) * 2 * kN;
-        // the row's first component is loaded before the transform, so its
-        // latency hides behind the butterflies rather than before the products
-        uint64_t g0[8];
-        load_blocked(g0, g);
         uint64_t d[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) d[j] = (coef[t + 256 * j] >> sh) & mask;
         ntt_fwd_sched<R, true>(d, lds, T);  // digits < 2^base_log2
 #pragma unroll
         for (int j = 0; j < 8; ++j) d[j] = ntt_fwd_below_2p63(d[j], q);
-        uint64_t g1[8];
-        load_blocked(g1, g + kN);
-        mac128x8(acc[0], d, g0);
-        mac128x8(acc[1], d, g1);
+        mac_row(acc, d, grow + (size_t)kk * 2 * kN);
       }
       uint64_t d[8];
       y_minus_x(poly, d);
-      mac_row(acc, d, Gb + (size_t)(poly * l + l - 1) * 2 * kN);
+      mac_row(acc, d, grow + (size_t)(l - 1) * 2 * kN);
     }
     // X + E into cts row r.  The fence keeps these loads after the loop:
     // hoisted, they would stay live through it.

@@ -223,6 +223,50 @@ __device__ __forceinline__ void ntt_fwd(uint64_t (&x)[8], uint64_t *lds, const N
   fwd8<2>(x, tw, q, false);
 }
 
+// Two forward transforms at once: the same passes, twiddle loads and
+// barriers serve both, and each pass has twice the independent butterflies.
+// x uses lds2[0, kNttLds), y lds2[kNttLds, 2 kNttLds).  Same entry and exit
+// conditions as ntt_fwd.
+template <bool SMALL = false>
+__device__ __forceinline__ void ntt_fwd2(uint64_t (&x)[8], uint64_t (&y)[8], uint64_t *lds2, const NttTab &T) {
+  const int t = threadIdx.x;
+  const uint64_t q = T.q;
+  uint64_t *l0 = lds2, *l1 = lds2 + kNttLds;
+  Tw8 tw;
+  load_tw8(tw, T.rp, T.rps, 1);
+  fwd8<0, SMALL>(x, tw, q, true);
+  fwd8<0, SMALL>(y, tw, q, true);
+  const int bB = pass_base(t, 5);
+  load_tw8(tw, T.rp, T.rps, 8 + (t >> 5));
+  __syncthreads();
+  lds_put(l0, x, t, 256);
+  lds_put(l1, y, t, 256);
+  __syncthreads();
+  lds_get(l0, x, bB, 32);
+  lds_get(l1, y, bB, 32);
+  fwd8<5>(x, tw, q, true);
+  fwd8<5>(y, tw, q, true);
+  const int bC = pass_base(t, 2);
+  load_tw8(tw, T.rp, T.rps, 64 + (t >> 2));
+  wave_sync();
+  lds_put(l0, x, bB, 32);
+  lds_put(l1, y, bB, 32);
+  wave_sync();
+  lds_get(l0, x, bC, 4);
+  lds_get(l1, y, bC, 4);
+  fwd8<2>(x, tw, q, true);
+  fwd8<2>(y, tw, q, true);
+  load_tw8(tw, T.rp, T.rps, 256 + t);
+  wave_sync();
+  lds_put(l0, x, bC, 4);
+  lds_put(l1, y, bC, 4);
+  wave_sync();
+  lds_get(l0, x, 8 * t, 1);
+  lds_get(l1, y, 8 * t, 1);
+  fwd8<2>(x, tw, q, false);
+  fwd8<2>(y, tw, q, false);
+}
+
 // Inverse NTT without the N^-1 scale.  Entry: blocked layout, values < 8q.
 // Exit: natural layout, values below 2^62 + 12c.  Begins with a barrier.
 // D -> C and C -> B synchronise per wave; B -> A crosses waves (barrier).
