@@ -25,6 +25,19 @@
 
 #include <stddef.h>
 #include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+/* Zeroed allocation that stops the process when memory runs out (the oracle
+ * is test infrastructure: a failed allocation is a fault of the test host). */
+static inline void *orc_alloc(size_t n) {
+  void *p = calloc(n ? n : 1, 1);
+  if (!p) {
+    fprintf(stderr, "oracle: out of memory (%zu bytes)\n", n);
+    abort();
+  }
+  return p;
+}
 
 #ifdef __cplusplus
 extern "C" {

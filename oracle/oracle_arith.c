@@ -206,10 +206,10 @@ static void ntt_init(orc_ntt_tables *T, uint64_t m, uint32_t N, uint32_t logN) {
   T->logN = logN;
   mod_init(&T->mod, m);
   T->psi = minimal_root(m, N);
-  T->rp = malloc(sizeof(uint64_t) * N);
-  T->rp_shoup = malloc(sizeof(uint64_t) * N);
-  T->irp = malloc(sizeof(uint64_t) * N);
-  T->irp_shoup = malloc(sizeof(uint64_t) * N);
+  T->rp = orc_alloc(sizeof(uint64_t) * N);
+  T->rp_shoup = orc_alloc(sizeof(uint64_t) * N);
+  T->irp = orc_alloc(sizeof(uint64_t) * N);
+  T->irp_shoup = orc_alloc(sizeof(uint64_t) * N);
   uint64_t psi_inv = orc_invmod(T->psi, m);
   uint64_t pw = 1, ipw = 1;
   for (uint32_t i = 0; i < N; ++i) {

@@ -64,7 +64,7 @@ static void poly_mul(const orc_ctx *c, int which, const uint64_t *a, const uint6
                      uint64_t *out) {
   const uint32_t N = c->N;
   const uint64_t m = which == 0 ? c->q : which == 1 ? c->p : c->small_q;
-  uint64_t *x = malloc(sizeof(uint64_t) * N), *y = malloc(sizeof(uint64_t) * N);
+  uint64_t *x = orc_alloc(sizeof(uint64_t) * N), *y = orc_alloc(sizeof(uint64_t) * N);
   memcpy(x, a, sizeof(uint64_t) * N);
   memcpy(y, b, sizeof(uint64_t) * N);
   orc_ntt_fwd(c, which, x);
@@ -94,7 +94,7 @@ static void enc_zero_mod(const orc_ctx *c, int which, uint64_t seed, uint64_t un
                          const uint64_t *sk, const int64_t *e, uint64_t *c0, uint64_t *c1) {
   const uint32_t N = c->N;
   const uint64_t m = which == 0 ? c->q : c->p;
-  uint64_t *s = malloc(sizeof(uint64_t) * N), *as = malloc(sizeof(uint64_t) * N);
+  uint64_t *s = orc_alloc(sizeof(uint64_t) * N), *as = orc_alloc(sizeof(uint64_t) * N);
   sk_mod(c, sk, m, s);
   sample_uniform(seed, uni, nonce, (uint64_t)which, m, N, c1);
   poly_mul(c, which, c1, s, as);
@@ -105,7 +105,7 @@ static void enc_zero_mod(const orc_ctx *c, int which, uint64_t seed, uint64_t un
 
 static void encrypt_zero_kind(const orc_ctx *c, uint64_t seed, int key, uint64_t nonce, const uint64_t *sk,
                               uint64_t *ct) {
-  int64_t *e = malloc(sizeof(int64_t) * c->N);
+  int64_t *e = orc_alloc(sizeof(int64_t) * c->N);
   sample_error(seed, key ? K_KEY_ERR : K_ERR, nonce, c->N, e);
   enc_zero_mod(c, 0, seed, key ? K_KEY_UNI : K_UNI, nonce, sk, e, ct, ct + c->N);
   free(e);
@@ -134,9 +134,9 @@ void orc_client_encrypt(const orc_ctx *c, uint64_t seed, uint64_t nonce, const u
  * gk layout: [a][comp][limb][N]. */
 void orc_client_galois_keys(const orc_ctx *c, uint64_t seed, const uint64_t *sk, uint64_t *gk) {
   const uint32_t N = c->N;
-  int64_t *e = malloc(sizeof(int64_t) * N);
-  uint64_t *gs = malloc(sizeof(uint64_t) * N);
-  uint64_t *c0 = malloc(sizeof(uint64_t) * N), *c1 = malloc(sizeof(uint64_t) * N);
+  int64_t *e = orc_alloc(sizeof(int64_t) * N);
+  uint64_t *gs = orc_alloc(sizeof(uint64_t) * N);
+  uint64_t *c0 = orc_alloc(sizeof(uint64_t) * N), *c1 = orc_alloc(sizeof(uint64_t) * N);
   const uint64_t p_mod_q = c->p % c->q;
   for (uint32_t a = 0; a < c->h; ++a) {
     const uint32_t g = 1 + (N >> a);
@@ -165,7 +165,7 @@ void orc_client_galois_keys(const orc_ctx *c, uint64_t seed, const uint64_t *sk,
  * Coefficient form, rk layout [2*l_key][c0|c1][N]. */
 void orc_client_rgsw_key(const orc_ctx *c, uint64_t seed, const uint64_t *sk, uint64_t *rk) {
   const uint32_t N = c->N, L = c->l_key;
-  uint64_t *g = malloc(sizeof(uint64_t) * L);
+  uint64_t *g = orc_alloc(sizeof(uint64_t) * L);
   orc_gadget(L, c->base_log2_key, c->q, g);
   for (uint32_t half = 0; half < 2; ++half)
     for (uint32_t k = 0; k < L; ++k) {
@@ -182,7 +182,7 @@ void orc_client_rgsw_key(const orc_ctx *c, uint64_t seed, const uint64_t *sk, ui
 void orc_client_rgsw(const orc_ctx *c, uint64_t seed, uint64_t nonce, const uint64_t *sk,
                      uint64_t bit, uint64_t *rgsw_ntt) {
   const uint32_t N = c->N, L = c->l;
-  uint64_t *g = malloc(sizeof(uint64_t) * L);
+  uint64_t *g = orc_alloc(sizeof(uint64_t) * L);
   orc_gadget(L, c->base_log2, c->q, g);
   for (uint32_t half = 0; half < 2; ++half)
     for (uint32_t k = 0; k < L; ++k) {
@@ -213,13 +213,13 @@ void orc_client_query_indices(const orc_ctx *c, uint64_t pt_index, uint64_t *idx
 void orc_client_query(const orc_ctx *c, uint64_t seed, uint64_t nonce, const uint64_t *sk,
                       uint64_t pt_index, uint64_t *ct) {
   const uint32_t N = c->N;
-  uint64_t *idx = malloc(sizeof(uint64_t) * c->ndim);
+  uint64_t *idx = orc_alloc(sizeof(uint64_t) * c->ndim);
   uint64_t *msg = calloc(N, sizeof(uint64_t));
   orc_client_query_indices(c, pt_index, idx);
   const uint64_t w = (uint64_t)1 << c->h;
   msg[orc_bit_reverse(idx[0], c->h)] = orc_invmod(w % c->t, c->t);
   orc_client_encrypt(c, seed, nonce, sk, msg, ct);
-  uint64_t *g = malloc(sizeof(uint64_t) * c->l);
+  uint64_t *g = orc_alloc(sizeof(uint64_t) * c->l);
   orc_gadget(c->l, c->base_log2, c->q, g);
   const uint64_t inv_q = orc_invmod(w % c->q, c->q);
   for (uint32_t i = 1; i < c->ndim; ++i) {
@@ -251,7 +251,7 @@ void orc_client_decrypt_small(const orc_ctx *c, const uint64_t *sk, const uint64
                               uint64_t *pt) {
   const uint32_t N = c->N;
   const uint64_t m = c->small_q;
-  uint64_t *s = malloc(sizeof(uint64_t) * N), *c1 = malloc(sizeof(uint64_t) * N);
+  uint64_t *s = orc_alloc(sizeof(uint64_t) * N), *c1 = orc_alloc(sizeof(uint64_t) * N);
   sk_mod(c, sk, m, s);
   for (uint32_t i = 0; i < N; ++i) c1[i] = resp[N + i] % m;
   poly_mul(c, 2, c1, s, pt);

@@ -77,8 +77,8 @@ void orc_keyswitch_add(const orc_ctx *c, const uint64_t *target, const uint64_t 
   const uint64_t q = c->q, p = c->p, half_p = p >> 1;
   const uint64_t half_p_mod_q = half_p % q;
   const uint64_t p_inv_q = orc_invmod(p % q, q);
-  uint64_t *tq = malloc(sizeof(uint64_t) * N), *tp = malloc(sizeof(uint64_t) * N);
-  uint64_t *xq = malloc(sizeof(uint64_t) * N), *xp = malloc(sizeof(uint64_t) * N);
+  uint64_t *tq = orc_alloc(sizeof(uint64_t) * N), *tp = orc_alloc(sizeof(uint64_t) * N);
+  uint64_t *xq = orc_alloc(sizeof(uint64_t) * N), *xp = orc_alloc(sizeof(uint64_t) * N);
   memcpy(tq, target, sizeof(uint64_t) * N);
   memcpy(tp, target, sizeof(uint64_t) * N); /* q < p: no reduction needed */
   orc_ntt_fwd(c, 0, tq);
@@ -108,7 +108,7 @@ void orc_keyswitch_add(const orc_ctx *c, const uint64_t *target, const uint64_t 
 void orc_apply_galois(const orc_ctx *c, const uint64_t *ct, uint32_t g, const uint64_t *key,
                       uint64_t *out) {
   const uint32_t N = c->N;
-  uint64_t *tmp = malloc(sizeof(uint64_t) * N);
+  uint64_t *tmp = orc_alloc(sizeof(uint64_t) * N);
   orc_galois_coeff(c, ct, g, c->q, out);
   orc_galois_coeff(c, ct + N, g, c->q, tmp);
   memset(out + N, 0, sizeof(uint64_t) * N);
@@ -137,7 +137,7 @@ void orc_decompose(const orc_ctx *c, const uint64_t *ct, uint32_t l, uint32_t ba
 void orc_external_product(const orc_ctx *c, const uint64_t *rgsw, uint32_t l, uint32_t base_log2,
                           const uint64_t *ct, uint64_t *out_ntt) {
   const uint32_t N = c->N, rows = 2 * l;
-  uint64_t *dig = malloc(sizeof(uint64_t) * (size_t)rows * N);
+  uint64_t *dig = orc_alloc(sizeof(uint64_t) * (size_t)rows * N);
   u128 *acc = calloc(2 * (size_t)N, sizeof(u128));
   orc_decompose(c, ct, l, base_log2, dig);
   for (uint32_t j = 0; j < rows; ++j) orc_ntt_fwd(c, 0, dig + (size_t)j * N);
@@ -170,7 +170,7 @@ void orc_expand(const orc_ctx *c, const uint64_t *query, const uint64_t *galois_
   const size_t ct_sz = 2 * (size_t)N;
   const uint64_t w = (uint64_t)1 << c->h, q = c->q;
   uint64_t *cts = calloc(2 * w * ct_sz, sizeof(uint64_t));
-  uint64_t *cp = malloc(sizeof(uint64_t) * ct_sz);
+  uint64_t *cp = orc_alloc(sizeof(uint64_t) * ct_sz);
   memcpy(cts + ct_sz, query, sizeof(uint64_t) * ct_sz);
   for (uint64_t i = 1; i < w; ++i) {
     uint32_t lvl = 63 - (uint32_t)__builtin_clzll(i);
@@ -232,7 +232,7 @@ void orc_first_dim_raw(const orc_ctx *c, const uint64_t *db, const uint64_t *sel
                        orc_u128 *inter) {
   const uint32_t N = c->N, fst = c->fst, other = c->other;
   const size_t ct_sz = 2 * (size_t)N;
-  uint64_t *qd = malloc(sizeof(uint64_t) * (size_t)fst * ct_sz);
+  uint64_t *qd = orc_alloc(sizeof(uint64_t) * (size_t)fst * ct_sz);
   /* prep_query: query_data[slot*(2*fst) + j*2 + poly] */
   for (uint32_t j = 0; j < fst; ++j)
     for (uint32_t s = 0; s < N; ++s) {
@@ -264,10 +264,10 @@ void orc_first_dim(const orc_ctx *c, const uint64_t *db, const uint64_t *sel_coe
                    uint64_t *out, orc_times *tm) {
   const uint32_t N = c->N, fst = c->fst, other = c->other;
   const size_t ct_sz = 2 * (size_t)N;
-  uint64_t *sel = malloc(sizeof(uint64_t) * fst * ct_sz);
+  uint64_t *sel = orc_alloc(sizeof(uint64_t) * fst * ct_sz);
   memcpy(sel, sel_coeff, sizeof(uint64_t) * fst * ct_sz);
   for (size_t j = 0; j < 2 * (size_t)fst; ++j) orc_ntt_fwd(c, 0, sel + j * N);
-  u128 *inter = malloc(sizeof(u128) * (size_t)N * 2 * other);
+  u128 *inter = orc_alloc(sizeof(u128) * (size_t)N * 2 * other);
   double t0 = now_s();
   orc_first_dim_raw(c, db, sel, inter);
   if (tm) tm->first_dim_core = now_s() - t0;
@@ -292,7 +292,7 @@ void orc_mux_level(const orc_ctx *c, uint64_t *cts, uint64_t n_cts, const uint64
   const uint32_t N = c->N;
   const size_t ct_sz = 2 * (size_t)N;
   const uint64_t q = c->q, half = n_cts / 2;
-  uint64_t *y = malloc(sizeof(uint64_t) * ct_sz);
+  uint64_t *y = orc_alloc(sizeof(uint64_t) * ct_sz);
   for (uint64_t i = 0; i < half; ++i) {
     uint64_t *x = cts + i * ct_sz, *yy = cts + (i + half) * ct_sz;
     for (size_t j = 0; j < ct_sz; ++j) y[j] = sub_mod(yy[j], x[j], q);
@@ -393,15 +393,15 @@ int orc_answer(const orc_ctx *c, const uint64_t *db, const uint64_t *galois_keys
   if (!tm) tm = &local;
   memset(tm, 0, sizeof(*tm));
   double t_start = now_s(), t;
-  uint64_t *leaves = malloc(sizeof(uint64_t) * ct_sz * c->u);
+  uint64_t *leaves = orc_alloc(sizeof(uint64_t) * ct_sz * c->u);
   t = now_s();
   orc_expand(c, query, galois_keys, leaves);
   tm->expand = now_s() - t;
-  uint64_t *rgsw = malloc(sizeof(uint64_t) * ((size_t)c->ndim - 1 + 1) * 2 * c->l * ct_sz);
+  uint64_t *rgsw = orc_alloc(sizeof(uint64_t) * ((size_t)c->ndim - 1 + 1) * 2 * c->l * ct_sz);
   t = now_s();
   orc_convert(c, leaves, rgsw_key_ntt, rgsw);
   tm->convert = now_s() - t;
-  uint64_t *res = malloc(sizeof(uint64_t) * ct_sz * c->other);
+  uint64_t *res = orc_alloc(sizeof(uint64_t) * ct_sz * c->other);
   t = now_s();
   orc_first_dim(c, db, leaves, res, tm);
   tm->first_dim = now_s() - t;
@@ -446,7 +446,7 @@ void orc_db_plaintext(const orc_ctx *c, uint64_t seed, int mode, uint64_t pt, ui
   /* bits per coefficient = bit_count(t) - 1 (src/pir.cpp:97-99) */
   const uint32_t bpc = c->plain_bits - 1;
   const size_t entry_bytes = (size_t)bpc * N / 8;
-  uint8_t *e = malloc(entry_bytes);
+  uint8_t *e = orc_alloc(entry_bytes);
   for (size_t b = 0; b < entry_bytes; b += 8) {
     uint64_t r = orc_rand64(seed, ORC_STREAM_DB, pt * N + b / 8);
     for (int k = 0; k < 8 && b + k < entry_bytes; ++k) e[b + k] = (uint8_t)(r >> (8 * k));
@@ -479,7 +479,7 @@ void orc_plain_to_ntt(const orc_ctx *c, const uint64_t *coeffs, uint64_t *out) {
 void orc_db_build_range(const orc_ctx *c, uint64_t seed, int mode, uint64_t pt_lo, uint64_t pt_hi,
                         uint64_t *db) {
   const uint32_t N = c->N;
-  uint64_t *co = malloc(sizeof(uint64_t) * N), *nt = malloc(sizeof(uint64_t) * N);
+  uint64_t *co = orc_alloc(sizeof(uint64_t) * N), *nt = orc_alloc(sizeof(uint64_t) * N);
   for (uint64_t pt = pt_lo; pt < pt_hi; ++pt) {
     orc_db_plaintext(c, seed, mode, pt, co);
     orc_plain_to_ntt(c, co, nt);
