@@ -573,3 +573,15 @@ def test_autotune_keeps_results_bit_identical(cfg1):
         assert all(np.array_equal(g, exp) for g in got)
     finally:
         srv.set_tuning(**saved)
+
+
+def test_measurement_taps(cfg1):
+    """The profiler-calibration copy moves what it reports, and the VALU
+    issue-rate probe runs and refuses a grid larger than its sink."""
+    from onionpirv2_amd._capi import OnionError
+    srv = cfg1["srv"]
+    assert srv.tap_copy(1 << 20) == 1 << 20
+    ms = srv.tap_valu_probe(256, 64)
+    assert 0 < ms < 1e3
+    with pytest.raises(OnionError):
+        srv.tap_valu_probe(1 << 30, 1)
