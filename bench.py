@@ -305,9 +305,6 @@ def main() -> int:
         log(f"[bench] VALU probe ms by waves per SIMD: {[round(p['ms'], 3) for p in out['valu_probe']]}")
     ok_all = bit_exact and all(leg["bit_exact"] for leg in (graph_single, eager_single) if leg)
 
-    # ---- the batch pipeline captured as one hipGraph and replayed ------------------------
-    out["batched_graph"] = batched_graph(srv, ids, queries, resps, args.steps)
-    ok_all = ok_all and out["batched_graph"]["bit_exact"]
 
     # ---- two lanes: two batches in flight on two streams --------------------------------
     if not args.no_extras:
@@ -361,29 +358,6 @@ def single_query_graph(srv, queries, reps, single_resp, use_graph=1):
     log(f"[bench] single query ({'graph' if use_graph else 'eager'}) p50 {pct(lat, 50):.3f} ms")
     return {"queries_per_s": 1e3 / float(np.mean(lat)), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
             "bit_exact": bool(same)}
-
-
-def batched_graph(srv, ids, queries, ref, steps):
-    """The batch-of-64 pipeline (about 45 launches) captured once as a hipGraph
-    (use_graph=1) and replayed `steps` times; every replay must equal the
-    eager responses.  Device time from the events around each replay."""
-    saved = srv.get_tuning()
-    srv.set_tuning(use_graph=1)
-    try:
-        srv.answer_batch(ids, queries)  # capture
-        dev, wall, same = [], [], True
-        for _ in range(steps):
-            t = time.perf_counter()
-            got = srv.answer_batch(ids, queries)
-            wall.append((time.perf_counter() - t) * 1e3)
-            dev.append(srv.stage_times()["total"])
-            same = same and np.array_equal(got, ref)
-    finally:
-        srv.set_tuning(**saved)
-    B = len(ids)
-    log(f"[bench] batch of {B} as a graph replay: {B * 1e3 / float(np.mean(dev)):.1f} queries/s")
-    return {"queries_per_s": B * 1e3 / float(np.mean(dev)), "step_ms": float(np.mean(dev)),
-            "pcie_inclusive_queries_per_s": B * 1e3 / float(np.mean(wall)), "steps": steps, "bit_exact": bool(same)}
 
 
 def two_lanes(srv, ids, queries, ref, steps):

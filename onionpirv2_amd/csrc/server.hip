@@ -124,12 +124,8 @@ struct KeyStore {
 // One pipeline: stream, events, per-query scratch for `cap` queries, the
 // device arrays of per-query key pointers, and the graphs captured for it.
 struct Lane {
+  // one stream: no stage waits on another stream's event
   hipStream_t st = nullptr;
-  // the conversion runs on st2 beside the selection stage on st (default
-  // priority, like st); fork / join order it after the expansion and before
-  // the first-dimension product
-  hipStream_t st2 = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
   hipEvent_t ev[8]{};
   uint32_t cap = 0;
   uint64_t *trees = nullptr, *rgsw = nullptr, *fd = nullptr, *digits = nullptr;
@@ -262,23 +258,16 @@ int ensure_scratch(const onion_server *s, Lane *L, uint32_t cap) {
 
 void destroy_lane(Lane *L) {
   if (L->st) (void)hipStreamSynchronize(L->st);
-  if (L->st2) (void)hipStreamSynchronize(L->st2);
   free_scratch(L);
   for (auto &e : L->ev)
     if (e) (void)hipEventDestroy(e);
-  for (hipEvent_t e : {L->fork, L->join})
-    if (e) (void)hipEventDestroy(e);
   if (L->st) (void)hipStreamDestroy(L->st);
-  if (L->st2) (void)hipStreamDestroy(L->st2);
-  L->st = L->st2 = nullptr;
+  L->st = nullptr;
 }
 
 int make_lane(std::unique_ptr<Lane> &out) {
   auto L = std::make_unique<Lane>();
-  bool ok = hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking) == hipSuccess &&
-            hipStreamCreateWithFlags(&L->st2, hipStreamNonBlocking) == hipSuccess &&
-            hipEventCreateWithFlags(&L->fork, hipEventDisableTiming) == hipSuccess &&
-            hipEventCreateWithFlags(&L->join, hipEventDisableTiming) == hipSuccess;
+  bool ok = hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking) == hipSuccess;
   for (auto &e : L->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
   if (!ok) {
     destroy_lane(L.get());
@@ -508,8 +497,8 @@ void launch_convert_fused(const onion_server *s, Lane *L, uint32_t nq, uint32_t 
 // copies them and returns them to coefficient form in registers, the split
 // kernels after k_cts_to_coeff has done so in place.
 void run_convert(const onion_server *s, Lane *L, uint32_t nq, uint64_t *conv, uint64_t conv_stride,
-                 bool leaves_ntt = false, hipStream_t st = nullptr) {
-  if (!st) st = L->st;
+                 bool leaves_ntt = false) {
+  hipStream_t st = L->st;
   const uint32_t nd = s->I.ndim - 1, l = s->P.gsw_l, lk = s->P.gsw_l_key;
   if (nd == 0) return;
   const uint32_t n_ct = nd * l;
@@ -610,7 +599,7 @@ void launch_gather(const onion_server *s, Lane *L, uint32_t nq) {
 // run_mux_first instead of transforming them into L->fd.  sel_ntt: the
 // selection vectors are in NTT form already.
 void run_first_dim_mfma(const onion_server *s, Lane *L, uint32_t nq, uint64_t *sel, uint64_t sel_stride,
-                        bool time_core, bool keep_ntt, bool sel_ntt, hipEvent_t join) {
+                        bool time_core, bool keep_ntt, bool sel_ntt) {
   const uint32_t fst = s->P.fst_dim, other = s->P.other_dim, ncp = cpad_for(nq), ncols = 2 * nq;
   uint8_t *Bp = reinterpret_cast<uint8_t *>(L->Qs);
   const dim3 chunk_grid(kN / 8, ncp / 32, fst / 32);
@@ -621,7 +610,6 @@ void run_first_dim_mfma(const onion_server *s, Lane *L, uint32_t nq, uint64_t *s
     k_sel_digits<<<dim3(fst * 2, nq), kNttThreads, 0, L->st>>>(sel, sel_stride, s->Tq);
     k_sel_chunks<false><<<chunk_grid, 256, 0, L->st>>>(sel, sel_stride, Bp, fst, ncols);
   }
-  if (join) (void)hipStreamWaitEvent(L->st, join, 0);
   if (time_core) (void)hipEventRecord(L->ev[6], L->st);
   const uint32_t rb = 128 / ncp;  // row blocks of 32 per workgroup
   const uint32_t n_rt = (other + 32 * rb - 1) / (32 * rb);
@@ -638,12 +626,11 @@ void run_first_dim_mfma(const onion_server *s, Lane *L, uint32_t nq, uint64_t *s
 
 // First dimension: fst selection cts per query (coefficient form, or NTT
 // form with sel_ntt; stride sel_stride) -> L->fd (other cts per query,
-// coefficient form).  join: an event the product waits for after the
-// selection stage (the conversion on the lane's side stream).
+// coefficient form).
 void run_first_dim(const onion_server *s, Lane *L, uint32_t nq, uint64_t *sel, uint64_t sel_stride,
-                   bool time_core, bool keep_ntt = false, bool sel_ntt = false, hipEvent_t join = nullptr) {
+                   bool time_core, bool keep_ntt = false, bool sel_ntt = false) {
   if (use_mfma(s, L, nq)) {
-    run_first_dim_mfma(s, L, nq, sel, sel_stride, time_core, keep_ntt, sel_ntt, join);
+    run_first_dim_mfma(s, L, nq, sel, sel_stride, time_core, keep_ntt, sel_ntt);
     return;
   }
   const uint32_t fst = s->P.fst_dim, other = s->P.other_dim, cpad = cpad_for(nq);
@@ -652,7 +639,6 @@ void run_first_dim(const onion_server *s, Lane *L, uint32_t nq, uint64_t *sel, u
     k_sel_ntt<true><<<dim3(fst * 2, nq), kNttThreads, 0, L->st>>>(sel, sel_stride, L->Qs, fst, cpad, s->Tq);
   else
     k_sel_ntt<false><<<dim3(fst * 2, nq), kNttThreads, 0, L->st>>>(sel, sel_stride, L->Qs, fst, cpad, s->Tq);
-  if (join) (void)hipStreamWaitEvent(L->st, join, 0);
   if (time_core) (void)hipEventRecord(L->ev[6], L->st);
   if (nq == 1) {
     if (fst % 16 == 0) {
@@ -785,23 +771,14 @@ void answer_device(const onion_server *s, Lane *L, uint32_t nq, bool timed) {
   if (timed) (void)hipEventRecord(L->ev[0], L->st);
   const bool leaves_ntt = run_expand(s, L, nq);
   if (timed) (void)hipEventRecord(L->ev[1], L->st);
-  // a batch's conversion (VALU) runs on the side stream beside the selection
-  // stage (memory); the first-dimension product waits for both.  They read
-  // disjoint leaves and write different buffers (RGSWs and, in split form,
-  // the digit scratch, which the product overwrites only after the join).
-  // A single query's stages are too short to gain from the overlap.
-  const bool side = nq > 1;
-  hipStream_t cst = side ? L->st2 : L->st;
-  if (side) {
-    (void)hipEventRecord(L->fork, L->st);
-    (void)hipStreamWaitEvent(L->st2, L->fork, 0);
-  }
-  run_convert(s, L, nq, leaves + (size_t)fst * 2 * kN, s->tree_words, leaves_ntt, cst);
-  if (timed) (void)hipEventRecord(L->ev[2], cst);
-  if (side) (void)hipEventRecord(L->join, L->st2);
+  // Every stage runs on the lane's one stream: no event orders one stream
+  // after another, inside a captured graph or out of it.  (A batch's
+  // conversion on a second stream beside the selection stage, forked and
+  // joined by events, saved about 0.15 ms per batch of 64.)
+  run_convert(s, L, nq, leaves + (size_t)fst * 2 * kN, s->tree_words, leaves_ntt);
+  if (timed) (void)hipEventRecord(L->ev[2], L->st);
   const bool first_fused = mux_first_fused(s, L, nq), fd_ntt = first_dim_keeps_ntt(s, L, nq);
-  run_first_dim(s, L, nq, leaves, s->tree_words, timed, first_fused || fd_ntt, leaves_ntt,
-                side ? L->join : nullptr);
+  run_first_dim(s, L, nq, leaves, s->tree_words, timed, first_fused || fd_ntt, leaves_ntt);
   if (timed) (void)hipEventRecord(L->ev[3], L->st);
   uint64_t n = s->P.other_dim;
   uint32_t d = 0;
