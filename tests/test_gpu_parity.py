@@ -356,11 +356,13 @@ def test_first_mux_level_from_ntt_form(shape):
         assert np.array_equal(even[3], exp4)
 
 
-@pytest.mark.parametrize("l,l_key", [(3, 4), (13, 11), (5, 16)])
+@pytest.mark.parametrize("l,l_key", [(3, 4), (13, 11), (5, 16), (4, 10), (6, 10)])
 def test_other_gadget_lengths(l, l_key):
     """Gadget lengths other than (5, 10) decide which folds apply: the
-    multiplexer's (l - 1) B < 60 holds for l = 3 (B = 20) and 5, not for
-    l = 13 (B = 5: 60), which keeps coefficient-form levels; the conversion
+    multiplexer's (l - 1) B < 60 holds for l = 3 (B = 20), 4 (B = 15), 5 and
+    6 (B = 10), not for l = 13 (B = 5: 60), which keeps coefficient-form
+    levels; its NTT-form kernel transforms the l - 1 digits two at a time, so
+    l = 4 and 6 also take the single transform after the pairs; the conversion
     key's holds for l_key = 4 (B = 15), not for 11 (B = 6) or 16 (B = 4, and
     32 products per 128-bit sum, the most the bound allows), which take the
     unfolded conversion of NTT-form leaves (top rows copied, then returned to
