@@ -136,6 +136,10 @@ struct Lane {
   uint64_t graph_epoch = 0;                   // tuning epoch the graphs were captured under
   std::mutex mu;                              // one batch at a time per lane
   bool borrowed = false;  // buffers are carved from the static scratch, never freed
+  // pinned host staging of `cap` queries and responses (2N words each), so the
+  // copies run as direct DMA instead of through the runtime's pageable path;
+  // none on a borrowed lane, which allocates nothing
+  uint64_t *hq = nullptr, *hr = nullptr;
 };
 
 struct onion_server {
@@ -207,6 +211,18 @@ void dfree(void *p) {
   if (p) (void)hipFree(p);
 }
 
+// Pinned (page-locked) host memory, allocated as such (hipHostMalloc).
+template <typename T>
+int halloc(T **p, size_t count) {
+  hipError_t e = hipHostMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(ONION_ENOMEM, std::string("hipHostMalloc of ") + std::to_string(count * sizeof(T)) +
+                                  " bytes failed: " + hipGetErrorString(e));
+  }
+  return ONION_OK;
+}
+
 uint32_t cpad_for(uint32_t nq) {
   if (nq <= 1) return 2;
   if (nq <= 16) return 32;
@@ -227,6 +243,9 @@ void drop_graphs(Lane *L) {
 
 void free_scratch(Lane *L) {
   drop_graphs(L);
+  for (uint64_t *h : {L->hq, L->hr})
+    if (h) (void)hipHostFree(h);
+  L->hq = L->hr = nullptr;
   void *bufs[] = {L->trees, L->rgsw, L->fd, L->digits, L->Qs, L->gk_ptrs, L->rk_ptrs};
   if (!L->borrowed)
     for (void *b : bufs) dfree(b);
@@ -245,7 +264,8 @@ int ensure_scratch(const onion_server *s, Lane *L, uint32_t cap) {
   if ((rc = dalloc(&L->trees, cap * s->tree_words)) || (rc = dalloc(&L->rgsw, cap * s->rgsw_words)) ||
       (rc = dalloc(&L->fd, cap * s->fd_words)) || (rc = dalloc(&L->digits, cap * s->digit_words)) ||
       (rc = dalloc(&L->Qs, qs_words(s, cap))) || (rc = dalloc(&L->gk_ptrs, cap)) ||
-      (rc = dalloc(&L->rk_ptrs, cap))) {
+      (rc = dalloc(&L->rk_ptrs, cap)) || (rc = halloc(&L->hq, (size_t)cap * 2 * kN)) ||
+      (rc = halloc(&L->hr, (size_t)cap * 2 * kN))) {
     std::string msg = g_err;
     free_scratch(L);
     return fail(rc, msg + " (batch scratch for " + std::to_string(cap) + " queries)");
@@ -865,7 +885,13 @@ int answer_chunk(onion_server *s, Lane *L, uint32_t n, const uint64_t *ids, cons
                  uint64_t *resps, onion_stage_times *times, bool eager_only = false) {
   int rc;
   if ((rc = bind_keys(s, L, n, ids, s->I.ndim > 1))) return rc;
-  HIPCHK(hipMemcpy2DAsync(L->trees + 2 * kN, s->tree_words * 8, queries, 2 * kN * 8, 2 * kN * 8, n,
+  const size_t qbytes = (size_t)n * 2 * kN * 8;
+  const uint64_t *qsrc = queries;
+  if (L->hq) {  // the previous call's copies have completed (it synchronised)
+    std::memcpy(L->hq, queries, qbytes);
+    qsrc = L->hq;
+  }
+  HIPCHK(hipMemcpy2DAsync(L->trees + 2 * kN, s->tree_words * 8, qsrc, 2 * kN * 8, 2 * kN * 8, n,
                           hipMemcpyHostToDevice, L->st));
   const bool graph = graph_for_size(s, n) && !eager_only;
   if (graph) {
@@ -878,9 +904,10 @@ int answer_chunk(onion_server *s, Lane *L, uint32_t n, const uint64_t *ids, cons
     answer_device(s, L, n, true);
     HIPCHK(hipGetLastError());
   }
-  HIPCHK(hipMemcpy2DAsync(resps, 2 * kN * 8, L->fd, s->fd_words * 8, 2 * kN * 8, n, hipMemcpyDeviceToHost,
-                          L->st));
+  HIPCHK(hipMemcpy2DAsync(L->hr ? L->hr : resps, 2 * kN * 8, L->fd, s->fd_words * 8, 2 * kN * 8, n,
+                          hipMemcpyDeviceToHost, L->st));
   HIPCHK(hipStreamSynchronize(L->st));
+  if (L->hr) std::memcpy(resps, L->hr, qbytes);
   return collect_times(s, L, !graph, times);
 }
 
