@@ -266,6 +266,7 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
   // of lds; the inverse transforms use the first half
   __shared__ uint64_t lds[(R == 8 ? 2 : 1) * kNttLds];
   __shared__ uint64_t coef[kN];  // y - x of the current polynomial; slots t + 256 j of thread t
+  __shared__ uint64_t ymx[kN];   // Y - X of the current polynomial; slots 8t .. 8t+7 of thread t
   const int t = threadIdx.x;
   const uint32_t nqw = nq / Q;  // workgroups per row pair
   uint32_t r, b;                // row pair (r, r + half); first query b
@@ -348,6 +349,8 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
       {
         uint64_t v[8];
         y_minus_x(poly, v);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) ymx[8 * t + j] = v[j];  // read back by this thread only
         ntt_inv_sched<R>(v, lds, T);
 #pragma unroll
         for (int j = 0; j < 8; ++j) coef[t + 256 * j] = ntt_inv_final(v[j], T);
@@ -387,7 +390,8 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
         mac_row(acc, d, grow + (size_t)kk * 2 * kN);
       }
       uint64_t d[8];
-      y_minus_x(poly, d);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) d[j] = ymx[8 * t + j];
       mac_row(acc, d, grow + (size_t)(l - 1) * 2 * kN);
     }
     // X + E into cts row r.  The fence keeps these loads after the loop:
