@@ -252,10 +252,11 @@ __global__ __launch_bounds__(256) void k_rgsw_fold_last(uint64_t *G, uint64_t g_
 // FIRST = true (the first level, straight from the matrix-core product's
 // NTT-form outputs out2[row][slot][column], column = 2 query + poly, ncols
 // columns): pair (r, r + half) of query b, result in cts row r; with Q = 2,
-// of queries b and b + 1 (an even batch).  The workgroups of one row pair
-// run on one XCD one after another (grid half * nq / Q, block index mod 8 =
-// XCD), so the pair's two 2 MiB rows come from HBM once and the 16-byte loads
-// at the 1 KiB slot stride hit that L2.
+// of queries b and b + 1 (an even batch).  Grid half * nq / Q, block index
+// mod 8 = XCD: the workgroups that read one 128-byte line of out2 run on one
+// XCD one after another, so the line comes from HBM once and the 16-byte
+// loads at the 1 KiB slot stride hit that L2; with 8 | nq, an XCD also keeps
+// to 8 queries over many row pairs, so their level RGSWs stay in its L2.
 template <bool FIRST, int Q = 1, int R = 8>
 __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, uint64_t cts_stride, uint64_t half,
                                                                const uint64_t *out2, uint32_t ncols, uint32_t nq,
@@ -273,6 +274,15 @@ __global__ __launch_bounds__(kNttThreads) void k_mux_fused_ntt(uint64_t *cts, ui
   if (!FIRST) {
     r = blockIdx.x;
     b = blockIdx.y;
+  } else if (Q == 2 && half % 8 == 0 && nqw % 4 == 0) {
+    // units u = (group g of 4 workgroups = 8 queries, row pair r), r fastest;
+    // XCD blockIdx % 8 takes the units u with u % 8 = its index, each unit's
+    // 4 workgroups one after another: they share every 128-byte line of out2
+    // they read (16 columns), and the group's 8 RGSWs stay in that XCD's L2
+    // over its row pairs
+    const uint32_t k = blockIdx.x >> 3, u = (k >> 2) * 8 + (blockIdx.x & 7);
+    r = u % half;
+    b = Q * (4 * (u / half) + (k & 3));
   } else if (half % 8 == 0) {
     const uint32_t k = blockIdx.x >> 3;
     r = 8 * (k / nqw) + (blockIdx.x & 7);
