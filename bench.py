@@ -410,15 +410,18 @@ def serving(srv, queries, single_resp, rates, seconds):
                 while time.perf_counter() < due:
                     pass
                 tickets.append((k % B, sv.submit(db, k % B, queries[k % B])))
-            same = True
-            for i, t in tickets:
-                same = same and np.array_equal(sv.wait(t), single_resp[i])
+            t_last = time.perf_counter()
+            resps = [sv.wait(t) for i, t in tickets]
+            drain_ms = (time.perf_counter() - t_last) * 1e3  # last arrival -> last response
+            same = all(np.array_equal(r, single_resp[i]) for r, (i, t) in zip(resps, tickets))
             st = sv.stats(db)
             offered = n / (due - t0)
             log(f"[bench] serving at {rate:.0f} q/s offered: {st['queries_per_s']:.1f} q/s done, "
-                f"p50 {st['p50_ms']:.2f} ms p99 {st['p99_ms']:.2f} ms, mean batch {st['mean_batch']:.1f}")
+                f"p50 {st['p50_ms']:.2f} ms p99 {st['p99_ms']:.2f} ms, drain {drain_ms:.1f} ms, "
+                f"mean batch {st['mean_batch']:.1f}")
             out.append({"offered_qps": offered, "target_qps": rate, "queries": n,
-                        "completed_qps": st["queries_per_s"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+                        "completed_qps": st["queries_per_s"], "drain_ms": drain_ms,
+                        "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
                         "max_ms": st["max_ms"], "mean_batch": st["mean_batch"], "batches": st["batches"],
                         "bit_exact": bool(same and st["failed"] == 0)})
     return {"lanes": 2, "max_batch": 64, "batch_cap": cap, "rates": out}
@@ -513,6 +516,22 @@ def config5(args):
     with GpuServe(max_batch=64, lanes_per_db=args.config5_lanes, scratch_budget=120 << 30,
                   workers=args.config5_workers) as sv:
         caps = [sv.add_db(s)[1] for s in servers]
+        # unloaded latency: one query at a time through the loop, per database
+        unloaded = []
+        for d in range(4):
+            lat = []
+            for k in range(7):
+                c = int(rng.integers(nc))
+                i = int(rng.integers(servers[d].info["num_pt"]))
+                q = client(d, c).query(i, nonce=50000 + 10 * d + k)
+                t = time.perf_counter()
+                r = sv.wait(sv.submit(d, c, q), db_id=d)
+                lat.append((time.perf_counter() - t) * 1e3)
+                ok_all = ok_all and np.array_equal(client(d, c).decrypt(r), dbgen.plaintext(
+                    args.db_seed + 11 + d, 0, i, t=servers[d].info["t"]))
+            unloaded.append(float(np.median(lat[2:])))
+        log("[bench] config5 unloaded p50 per database: " +
+            ", ".join(f"{a}x{b} {u:.2f} ms" for (a, b), u in zip(shapes, unloaded)))
         for rate in rates:
             n = max(40, int(rate * args.serve_seconds))
             work = []
@@ -531,9 +550,13 @@ def config5(args):
                 while time.perf_counter() < due:
                     pass
                 tickets.append(sv.submit(d, c, q))
+            t_last = time.perf_counter()
+            resps = [sv.wait(tk, db_id=d) for (d, c, i, q), tk in zip(work, tickets)]
+            # from the last arrival to the last response: about one batch when the
+            # loop keeps up, growing with the run's length when it does not
+            drain_ms = (time.perf_counter() - t_last) * 1e3
             good = 0
-            for (d, c, i, q), tk in zip(work, tickets):
-                r = sv.wait(tk, db_id=d)
+            for (d, c, i, q), r in zip(work, resps):
                 good += int(np.array_equal(client(d, c).decrypt(r),
                                            dbgen.plaintext(args.db_seed + 11 + d, 0, i, t=servers[d].info["t"])))
             st = sv.stats()
@@ -542,18 +565,21 @@ def config5(args):
                 sd = sv.stats(d)
                 per_db.append({"shape": f"{shapes[d][0]}x{shapes[d][1]}", "queries": sd["completed"],
                                "queries_per_s": sd["queries_per_s"], "p50_ms": sd["p50_ms"], "p99_ms": sd["p99_ms"],
-                               "mean_batch": sd["mean_batch"]})
+                               "p50_over_unloaded": sd["p50_ms"] / unloaded[d], "mean_batch": sd["mean_batch"]})
             ok_all = ok_all and good == n
             log(f"[bench] config5 at {rate:.0f} q/s: {st['queries_per_s']:.1f} q/s done, p50 {st['p50_ms']:.1f} ms "
-                f"p99 {st['p99_ms']:.1f} ms, {good}/{n} records")
+                f"p99 {st['p99_ms']:.1f} ms, drain {drain_ms:.0f} ms, {good}/{n} records; p50 / unloaded per "
+                "database " + " ".join(f"{x['p50_over_unloaded']:.1f}" for x in per_db))
             runs.append({"target_qps": rate, "offered_qps": n / (due - t0), "queries": n,
-                         "completed_qps": st["queries_per_s"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+                         "completed_qps": st["queries_per_s"], "drain_ms": drain_ms,
+                         "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
                          "max_ms": st["max_ms"], "mean_batch": st["mean_batch"], "records_ok": good,
                          "per_db": per_db})
     for s in reversed(servers):
         s.close()
     return {"resident_GiB": resident / 2**30, "shapes": [f"{a}x{b}" for a, b in shapes], "clients": nc,
             "key_store_GiB": key_bytes / 2**30, "batch_caps": caps, "scratch_budget_GiB": 120,
+            "unloaded_p50_ms": unloaded,
             "workers": args.config5_workers, "lanes_per_db": args.config5_lanes, "runs": runs,
             "bit_exact": bool(ok_all)}
 
