@@ -137,8 +137,9 @@ __device__ __forceinline__ void mac_row(U128 (&acc)[2][8], const uint64_t (&d)[8
 // NTT(x) itself.  NTT_IN (conversion of NTT-form leaves, ys unused): xs is in
 // NTT form; each polynomial is copied to top + poly * N as it is (the RGSW's
 // top row), returns to coefficient form in registers for its digits, and is
-// read again for the last row.
-template <bool FOLD = false, bool NTT_IN = false>
+// read again for the last row.  PAIR: lds holds 2 kNttLds words and the
+// digits are transformed two at a time (ntt_fwd2).
+template <bool FOLD = false, bool NTT_IN = false, bool PAIR = false>
 __device__ __forceinline__ void ext_fused_core(const uint64_t *xs, const uint64_t *ys, const uint64_t *G,
                                                uint32_t l, uint32_t base_log2, uint64_t *lds,
                                                const NttTab &T, const Mod &mq, uint64_t (&res)[2][8],
@@ -164,8 +165,30 @@ __device__ __forceinline__ void ext_fused_core(const uint64_t *xs, const uint64_
         v[r] = ys ? sub_mod(ys[(size_t)poly * kN + e], xa, T.q) : xa;
       }
     }
+    const uint32_t ndig = FOLD ? l - 1 : l;  // digits transformed
+    uint32_t kk = 0;
+    if (PAIR) {
 #pragma unroll 1
-    for (uint32_t kk = 0; kk < (FOLD ? l - 1 : l); ++kk) {
+      for (; kk + 1 < ndig; kk += 2) {  // digits kk, kk + 1
+        const uint32_t sh = (l - 1 - kk) * base_log2;
+        uint64_t d[8], e[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          d[r] = (v[r] >> sh) & mask;
+          e[r] = (v[r] >> (sh - base_log2)) & mask;
+        }
+        ntt_fwd2<true>(d, e, lds, T);  // digits < 2^base_log2 <= 2^32
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          d[r] = ntt_fwd_below_2p63(d[r], T.q);
+          e[r] = ntt_fwd_below_2p63(e[r], T.q);
+        }
+        mac_row(acc, d, G + (size_t)(poly * l + kk) * 2 * kN);
+        mac_row(acc, e, G + (size_t)(poly * l + kk + 1) * 2 * kN);
+      }
+    }
+#pragma unroll 1
+    for (; kk < ndig; ++kk) {
       const uint32_t sh = (l - 1 - kk) * base_log2;
       uint64_t d[8];
 #pragma unroll
@@ -441,13 +464,13 @@ __global__ __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(2))
                                                                uint32_t base_log2_key, uint64_t *out,
                                                                uint64_t out_stride, uint32_t l_out, NttTab T,
                                                                Mod mq) {
-  __shared__ uint64_t lds[kNttLds];
+  __shared__ uint64_t lds[2 * kNttLds];  // digit transforms in pairs
   const uint64_t ct = blockIdx.x;
   const uint64_t *src = leaves + (size_t)blockIdx.y * leaf_stride + ct * 2 * kN;
   uint64_t *rgsw = out + (size_t)blockIdx.y * out_stride + (ct / l_out) * (2 * l_out) * 2 * kN;
   uint64_t *dst = rgsw + (l_out + ct % l_out) * 2 * kN;
   uint64_t res[2][8];
-  ext_fused_core<FOLD, NTT_IN>(src, nullptr, g_ptrs[blockIdx.y], lk, base_log2_key, lds, T, mq, res,
+  ext_fused_core<FOLD, NTT_IN, true>(src, nullptr, g_ptrs[blockIdx.y], lk, base_log2_key, lds, T, mq, res,
                                rgsw + (ct % l_out) * 2 * kN);
   store_blocked(dst, res[0]);
   store_blocked(dst + kN, res[1]);
