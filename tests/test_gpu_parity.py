@@ -319,7 +319,8 @@ def test_first_mux_level_from_ntt_form(shape):
     (k_mux_fused_ntt) and return to coefficient form for the narrow split
     kernels: fuse_min = 5 / 10 / 20 put that switch after the first, second
     or third level.  The batch of 3 takes the first level one query per
-    workgroup, a batch of 4 two queries per workgroup."""
+    workgroup, a batch of 4 two queries per workgroup, and a batch of 8 at
+    half = 8 two queries per workgroup in groups of 8 queries per XCD."""
     from onionpirv2_amd import GpuServer
     fst, other = shape
     o = Oracle(fst=fst, other=other)
@@ -354,6 +355,13 @@ def test_first_mux_level_from_ntt_form(shape):
         even = srv.answer_batch([0, 1, 2, 0], np.concatenate([qs, q4[None]]))
         assert np.array_equal(even[:3], fused)
         assert np.array_equal(even[3], exp4)
+        c8 = [i % 3 for i in range(8)]
+        q8 = np.stack([o.query(keys[c][0], 200 + i, keys[c][1], (5 * i + 1) % o.num_pt) for i, c in enumerate(c8)])
+        got8 = srv.answer_batch(c8, q8)
+        for i, c in enumerate(c8):
+            assert np.array_equal(got8[i], srv.answer(c, q8[i])), f"batch of 8, query {i}"
+        exp8, _ = o.answer(db, keys[1][2], o.rgsw_key_ntt(keys[1][3]), q8[7])
+        assert np.array_equal(got8[7], exp8)
 
 
 @pytest.mark.parametrize("l,l_key", [(3, 4), (13, 11), (5, 16), (4, 10), (6, 10)])
