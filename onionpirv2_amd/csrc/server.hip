@@ -140,6 +140,9 @@ struct Lane {
   // copies run as direct DMA instead of through the runtime's pageable path;
   // none on a borrowed lane, which allocates nothing
   uint64_t *hq = nullptr, *hr = nullptr;
+  // a call returned (on an error) after queueing copies that read or write
+  // hq / hr and before synchronising: the next call waits for the stream first
+  bool in_flight = false;
 };
 
 struct onion_server {
@@ -884,13 +887,18 @@ bool graph_for_size(const onion_server *s, uint32_t n) {
 int answer_chunk(onion_server *s, Lane *L, uint32_t n, const uint64_t *ids, const uint64_t *queries,
                  uint64_t *resps, onion_stage_times *times, bool eager_only = false) {
   int rc;
+  if (L->in_flight) {
+    HIPCHK(hipStreamSynchronize(L->st));
+    L->in_flight = false;
+  }
   if ((rc = bind_keys(s, L, n, ids, s->I.ndim > 1))) return rc;
   const size_t qbytes = (size_t)n * 2 * kN * 8;
   const uint64_t *qsrc = queries;
-  if (L->hq) {  // the previous call's copies have completed (it synchronised)
+  if (L->hq) {  // no copy of an earlier call reads it any more (each synchronised)
     std::memcpy(L->hq, queries, qbytes);
     qsrc = L->hq;
   }
+  L->in_flight = true;
   HIPCHK(hipMemcpy2DAsync(L->trees + 2 * kN, s->tree_words * 8, qsrc, 2 * kN * 8, 2 * kN * 8, n,
                           hipMemcpyHostToDevice, L->st));
   const bool graph = graph_for_size(s, n) && !eager_only;
@@ -907,6 +915,7 @@ int answer_chunk(onion_server *s, Lane *L, uint32_t n, const uint64_t *ids, cons
   HIPCHK(hipMemcpy2DAsync(L->hr ? L->hr : resps, 2 * kN * 8, L->fd, s->fd_words * 8, 2 * kN * 8, n,
                           hipMemcpyDeviceToHost, L->st));
   HIPCHK(hipStreamSynchronize(L->st));
+  L->in_flight = false;
   if (L->hr) std::memcpy(resps, L->hr, qbytes);
   return collect_times(s, L, !graph, times);
 }
