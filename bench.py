@@ -109,6 +109,8 @@ def main() -> int:
     ap.add_argument("--config5-workers", type=int, default=2,
                     help="batches in flight over the four databases (the serving loop's worker pool)")
     ap.add_argument("--config5-lanes", type=int, default=1, help="lanes per database in the serving loop")
+    ap.add_argument("--config5-only", action="store_true",
+                    help="run only the four-database serving workload and print its results as the JSON line")
     ap.add_argument("--no-single-legs", dest="single_legs", action="store_false",
                     help="skip the single-query legs that force graph replay and eager launches")
     ap.add_argument("--multi-db-linger-us", type=int, default=0,
@@ -127,6 +129,11 @@ def main() -> int:
               "the database would shard by rows, each GPU answering its rows and one 32 KiB gather "
               "per GPU combining them", file=sys.stderr)
         return 2
+
+    if args.config5_only:
+        c5 = config5(args)
+        print(json.dumps({"config5": c5}), flush=True)
+        return 0 if c5["bit_exact"] else 1
 
     from onionpirv2_amd import GpuServer, dbgen
     from onionpirv2_amd.client import GpuClient
